@@ -1,0 +1,213 @@
+#!/usr/bin/env python3
+"""Benchmark: batched Klein-Gordon 2D, 100 trajectories of 300^2 (side 20,
+Gautschi, mu2 = 1, -u^3, dt = 0.005, Krylov dimension from the autotuner in
+[8, 16], ring/kink initial data from seed 2), fused FP32-Krylov / FP64-state
+path on one MI355X.
+
+Prints ONE JSON line: whole-job cell-steps/s over the timed steps, the
+roofline of the dominant kernel (the fused Lanczos pass) from HIP events on the
+batch stream, the plain CSR-SpMV FP64 drop-in timed on the same box, the
+relative L2 error of a 300^2 trajectory against the FP64 oracle, and the FP64
+CPU oracle timed on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "nlsolvers-datagen_amd"))
+
+METRIC = "KGE-2D 300² cell-steps/s batched vs plain at relL2≤1e-5; p50/p99 traj latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_inputs(ntraj, n, L, seed):
+    from nls.samplers import kink_ring_2d
+    out = []
+    for t in range(ntraj):
+        u0, v0, _ = kink_ring_2d(n, L, seed, t)
+        out.append((np.ones((n, n)), np.ones((n, n)), u0, v0))
+    return out
+
+
+def fused_run(args, inputs):
+    from nls.batch import FusedBatch
+    n, L = args.n, args.L
+    b = FusedBatch([n] * args.ntraj, L, args.dt, krylov_dim=args.krylov, mu2=1.0,
+                   use_graph=not args.no_graph, cells_per_thread=args.cpt)
+    for t, (m, c, u0, v0) in enumerate(inputs):
+        b.set_initial(t, m, c, u0, v0)
+    b.run(1)  # autotune + graph capture
+    if args.warmup > 1:
+        b.run(args.warmup - 1)
+    from nls._lib import call
+    call("nls_dev_synchronize")
+    t0 = time.perf_counter()
+    ms = b.run(args.steps)
+    call("nls_dev_synchronize")
+    wall = time.perf_counter() - t0
+    info = b.info()
+    # dominant-kernel timing: a few eager steps with HIP events around each
+    # Lanczos pass launch, on the batch stream
+    b.set_profiling(True)
+    b.run(args.prof_steps)
+    pt = b.pass_timing()
+    b.set_profiling(False)
+    return b, ms, wall, info, pt
+
+
+def plain_run(args, inputs):
+    """Plain drop-in: FP64 CSR SpMV + reference-granularity Lanczos, one trajectory
+    after another (the rate does not depend on how many run in sequence)."""
+    from nls.solvers import KGESolverDevice
+    n, L = args.n, args.L
+    m, c, u0, v0 = inputs[0]
+    steps = args.plain_steps
+    s = KGESolverDevice(n, L, args.dt * (steps + 1), steps + 1, 1, m, c, u0, v0,
+                        krylov_dim=args.krylov_used, mu2=1.0)
+    s.step()
+    s.apply_bc()
+    s.state()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step()
+        s.apply_bc()
+    s.state()
+    wall = time.perf_counter() - t0
+    return n * n * steps / wall, wall, steps
+
+
+def parity_check(args, inputs, krylov):
+    """rel-L2 of a fused 300^2 trajectory vs the FP64 oracle at step S: both take
+    the reference cadence with nt = 2S, num_snapshots = 2 (snapshot 1 = step S)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    from nls.batch import FusedBatch
+    n, L, S = args.n, args.L, args.parity_steps
+    m, c, u0, v0 = inputs[0]
+    b = FusedBatch([n], L, args.dt, krylov_dim=krylov, mu2=1.0, snapshot_every=S,
+                   max_snapshots=2, cells_per_thread=8)
+    b.set_initial(0, m, c, u0, v0)
+    b.run(S)
+    u, _ = b.snapshots(0)
+    ur, _ = orc.kge_run(n, L, args.dt * 2 * S, 2 * S, 2, m, c, u0, v0, krylov=krylov, mu2=1.0)
+    err = float(np.linalg.norm(u[1] - ur[1]) / np.linalg.norm(ur[1]))
+    return err, S
+
+
+def cpu_baseline(args, inputs, krylov):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    threads = orc.omp_threads()
+    ntraj = threads
+    steps = args.cpu_steps
+    n, L = args.n, args.L
+    sample = [inputs[t % len(inputs)] for t in range(ntraj)]
+    t0 = time.perf_counter()
+    orc.kge_run_batch(n, L, args.dt * (steps + 1), steps + 1, 1, [s[0] for s in sample],
+                      [s[1] for s in sample], [s[2] for s in sample], [s[3] for s in sample],
+                      krylov=krylov, mu2=1.0)
+    wall = time.perf_counter() - t0
+    return {"value": ntraj * n * n * steps / wall, "unit": "cell-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{ntraj} trajectories x {steps} steps of the same 300^2 workload, "
+                      f"FP64 reference-semantics oracle (CSR SpMV, Lanczos with Gram-Schmidt, "
+                      f"QL), one trajectory per OpenMP thread, {wall:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--ntraj", type=int, default=100)
+    ap.add_argument("--n", type=int, default=300)
+    ap.add_argument("--L", type=float, default=10.0)
+    ap.add_argument("--dt", type=float, default=0.005)
+    ap.add_argument("--krylov", type=int, default=0, help="0 = autotuner")
+    ap.add_argument("--cpt", type=int, default=0, help="cells per thread, 0 = autotuner")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--prof-steps", type=int, default=10)
+    ap.add_argument("--plain-steps", type=int, default=40)
+    ap.add_argument("--parity-steps", type=int, default=100)
+    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-parity", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not "
+                                   "supported (trajectories would shard across GPUs with no "
+                                   "communication, see DESIGN.md)"}))
+        sys.exit(2)
+    from nls._lib import load
+    load()
+    log(f"[bench] building inputs: {args.ntraj} x {args.n}^2")
+    inputs = make_inputs(args.ntraj, args.n, args.L, seed=2)
+    log("[bench] fused batched run")
+    b, ms, wall, info, pt = fused_run(args, inputs)
+    cells = info["cells"]
+    value = cells * args.steps / (ms / 1e3)
+    args.krylov_used = info["krylov_dim"]
+    achieved = pt["bytes_per_launch"] / (pt["avg_ms"] / 1e3) / 1e9 if pt["avg_ms"] > 0 else None
+    log(f"[bench] fused: {value:.3e} cell-steps/s ({ms / args.steps:.3f} ms/step), "
+        f"m={info['krylov_dim']}, pass avg {pt['avg_ms'] * 1e3:.1f} us -> {achieved:.0f} GB/s")
+    del b
+    log("[bench] plain FP64 CSR drop-in")
+    plain_rate, plain_wall, plain_steps = plain_run(args, inputs)
+    log(f"[bench] plain: {plain_rate:.3e} cell-steps/s")
+    err = None
+    if not args.skip_parity:
+        log("[bench] parity vs FP64 oracle")
+        err, psteps = parity_check(args, inputs, info["krylov_dim"])
+        log(f"[bench] rel-L2 after {psteps} steps: {err:.2e}")
+    cpu = None
+    if not args.skip_cpu:
+        log("[bench] CPU baseline (oracle)")
+        cpu = cpu_baseline(args, inputs, info["krylov_dim"])
+        log(f"[bench] cpu: {cpu['value']:.3e} cell-steps/s on {cpu['cores']} threads")
+    m = info["krylov_dim"]
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 Krylov vectors / fp64 state and small stage",
+        "data": "synthetic (ring/kink initial data from seed 2, m = c = 1)",
+        "config": {"workload": f"KGE-2D batched, {args.ntraj} x {args.n}^2, side 20, Gautschi, "
+                               f"mu2=1, dt={args.dt}, Krylov m={m}, HIP graph step capture",
+                   "ntraj": args.ntraj, "n": args.n, "dt": args.dt, "krylov_dim": m,
+                   "cells": cells, "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     "traffic": None,
+                     "kernel": "k_kge_pass<2,2,CPT> (fused stencil+Lanczos pass)",
+                     "bytes_per_launch": pt["bytes_per_launch"],
+                     "avg_launch_ms": pt["avg_ms"], "launches_timed": pt["launches"]},
+        "plain": {"value": plain_rate, "unit": "cell-steps/s",
+                  "path": "FP64 CSR SpMV + reference-granularity Lanczos (CGS), one trajectory "
+                          "at a time", "steps": plain_steps},
+        "speedup_vs_plain": value / plain_rate,
+        "rel_l2_vs_fp64_oracle": err,
+        "latency": None,
+        "cpu_baseline": cpu,
+        "wall_s_timed": wall,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

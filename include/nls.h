@@ -1,0 +1,282 @@
+/*
+ * nls.h -- C ABI of libnls, the MI355X (gfx950) implementation of the
+ * nlsolvers-datagen time steppers: nonlinear Klein-Gordon (Gautschi, sEWI,
+ * Stormer-Verlet), cubic NLSE (Strang, SEWI) and stochastic phi^4, with their
+ * Krylov (Lanczos) matrix functions on 5-point / 7-point no-flux operators.
+ *
+ * The reference has no C ABI: its Python layer runs one CUDA executable per
+ * trajectory (scripts/real_launcher_2d.py:147-170) and the executables drive C++
+ * classes.  Each entry point below names the reference interface it replaces
+ * (paths relative to the reference repository root).
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a non-zero status on failure;
+ *    nls_last_error() then returns a thread-local message.  Nothing throws.
+ *  - "host" buffers are caller-owned, C-contiguous; "device" buffers are device
+ *    pointers (from nls_dev_malloc or any HIP allocation on the current device).
+ *  - Complex arrays are interleaved (re, im) pairs: complex128 = 2 doubles,
+ *    complex64 = 2 floats.
+ *  - Grids are the FULL array including the ghost ring (n = nx = ny [= nz]),
+ *    flat index i*n + j (2D) or (a*n + b)*n + k (3D); the domain is [-L, L]^dim,
+ *    dx = 2L/(n-1), dt = T/nt (device/drivers/kg_driver_dev_2d.cpp:51-55).
+ *  - Solver objects own their device memory and one HIP stream (default
+ *    priority).  Calls are stream-ordered; functions that return host data
+ *    synchronise.
+ */
+#ifndef NLS_H
+#define NLS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NLS_API __attribute__((visibility("default")))
+
+/* ---------------------------------------------------------------- status */
+/* Message of the last failing call on this thread ("" if none).
+ * Replaces CHECK_CUDA / CHECK_CUSPARSE print-and-throw (device/include/pragmas.hpp:8-26). */
+NLS_API const char *nls_last_error(void);
+NLS_API int nls_version(void);
+/* Device name, compute-unit count and HBM bytes of the current device. */
+NLS_API int nls_device_info(char *name, int name_len, int *cu_count, size_t *hbm_bytes);
+
+/* ---------------------------------------------------------------- device memory helpers */
+/* Thin wrappers so ctypes callers need no other runtime
+ * (the reference allocates with cudaMalloc in its drivers, kg_driver_dev_2d.cpp:125-137). */
+NLS_API int nls_dev_malloc(void **ptr, size_t bytes);
+NLS_API int nls_dev_free(void *ptr);
+NLS_API int nls_memcpy_h2d(void *dst_device, const void *src_host, size_t bytes);
+NLS_API int nls_memcpy_d2h(void *dst_host, const void *src_device, size_t bytes);
+NLS_API int nls_dev_synchronize(void);
+
+/* ---------------------------------------------------------------- operator */
+/* nnz of the full-array operator; q6 only matters for dim 3. */
+NLS_API int64_t nls_laplacian_nnz(int dim, int n, int q6);
+/* CSR (ascending columns) of div(c grad .) with no-flux ends, scaled by 1/(dx*dy),
+ * optionally negated, then `shift` added to the diagonal.  Passing row_ptr = NULL
+ * only returns *nnz.
+ * Replaces build_anisotropic_laplacian_noflux (common/include/laplacians.hpp:54-103)
+ * as called by the drivers (kg_driver_dev_2d.cpp:105-108). */
+NLS_API int nls_build_laplacian_2d(int n, double dx, double dy, const double *c, int negate,
+                                   double shift, int32_t *row_ptr, int32_t *col, double *val,
+                                   int64_t *nnz);
+/* 3D 7-point version, scaled by 1/(dx*dx); q6 = 1 keeps the reference's
+ * cross-plane "y" couplings.  Replaces build_anisotropic_laplacian_noflux_3d
+ * (laplacians.hpp:158-218). */
+NLS_API int nls_build_laplacian_3d(int n, double dx, const double *c, int negate, double shift,
+                                   int q6, int32_t *row_ptr, int32_t *col, double *val,
+                                   int64_t *nnz);
+
+/* ---------------------------------------------------------------- plain CSR SpMV (device pointers) */
+/* y = A x with int32 CSR, one call per product, on `stream` (NULL = default).
+ * Replaces DeviceSpMV<T>::multiply (device/include/spmv.hpp:65-72, cuSPARSE CSR ALG2). */
+NLS_API int nls_csr_spmv_f64(int64_t n, const int32_t *row_ptr, const int32_t *col,
+                             const double *val, const double *x, double *y, void *stream);
+NLS_API int nls_csr_spmv_f32(int64_t n, const int32_t *row_ptr, const int32_t *col,
+                             const float *val, const float *x, float *y, void *stream);
+NLS_API int nls_csr_spmv_c128(int64_t n, const int32_t *row_ptr, const int32_t *col,
+                              const double *val, const double *x, double *y, void *stream);
+NLS_API int nls_csr_spmv_c64(int64_t n, const int32_t *row_ptr, const int32_t *col,
+                             const float *val, const float *x, float *y, void *stream);
+
+/* ---------------------------------------------------------------- matrix functions (FP64, plain) */
+typedef struct nls_matfunc nls_matfunc;
+/* Function kinds.  Real: result = beta0 V f(T) e1 with f of t*sqrt|lambda|.
+ * Complex: f of dt*lambda with complex dt. */
+enum {
+  NLS_F_COS_SQRT = 0,   /* cos(t sqrt|l|)                    matfunc_real.hpp:221-228 */
+  NLS_F_SINC2_SQRT = 1, /* (sin x/x)^2, x = t sqrt|l|        matfunc_real.hpp:212-219 */
+  NLS_F_ID_SQRT = 2,    /* t sqrt|l|                         matfunc_real.hpp:230-237 */
+  NLS_F_SINC2_HALF = 3, /* (sin(x/2)/(x/2))^2 (sEWI filter, no reference counterpart) */
+  NLS_F_COSM1_SQRT = 4, /* cos(x) - 1 (increment form, no reference counterpart) */
+  NLS_F_EXP = 8,        /* exp(dt l)                         matfunc_complex.hpp:281-287 */
+  NLS_F_SINC = 9        /* sin(dt l)/(dt l)                  matfunc_complex.hpp:293-300 */
+};
+/* Uploads a host CSR and sizes a Krylov workspace of dimension m (2..32).
+ * complex_vectors selects complex128 vectors.  Replaces the constructors of
+ * MatrixFunctionApplicatorReal (matfunc_real.hpp:127-167) and
+ * MatrixFunctionApplicatorComplex (matfunc_complex.hpp:124-176). */
+NLS_API int nls_matfunc_create(int64_t n, int64_t nnz, const int32_t *row_ptr, const int32_t *col,
+                               const double *val, int m, int complex_vectors, nls_matfunc **out);
+NLS_API int nls_matfunc_destroy(nls_matfunc *h);
+/* out = f(A) in on device pointers (in may alias out), reference Lanczos
+ * (lanczos.hpp:208-267: T[m-1,m-1] = 0, recomputed lower subdiagonal, one
+ * classical Gram-Schmidt pass per iteration).
+ * Replaces MatrixFunctionApplicatorReal::apply (matfunc_real.hpp:190-313). */
+NLS_API int nls_matfunc_apply_real(nls_matfunc *h, double *out, const double *in, double t,
+                                   int kind);
+/* Replaces MatrixFunctionApplicatorComplex::apply (matfunc_complex.hpp:254-375). */
+NLS_API int nls_matfunc_apply_complex(nls_matfunc *h, double *out, const double *in, double dt_re,
+                                      double dt_im, int kind);
+/* Copies the m x m column-major T of the last apply to host (complex: 2*m*m doubles)
+ * and returns beta0 (test access to KrylovInfo, lanczos.hpp:23-31). */
+NLS_API int nls_matfunc_last_krylov(nls_matfunc *h, double *T_host, double *beta0);
+
+/* FP64 GPU small stage on host data: eigen-decomposition of the symmetric
+ * tridiagonal (d[m], e[m-1]) and y = beta0 Q f(L) Q^T e1 (yr, yi host arrays of m).
+ * Replaces cusolverDnDsyevd + f(lambda) + GEMMs (matfunc_real.hpp:252-292,385-392). */
+NLS_API int nls_small_matfunc(int m, const double *d, const double *e, int kind, double t_re,
+                              double t_im, double beta0, double *yr, double *yi);
+
+/* ---------------------------------------------------------------- Klein-Gordon solver (plain FP64) */
+enum { NLS_KGE_GAUTSCHI = 0, NLS_KGE_SV = 1, NLS_KGE_SEWI = 2 };
+typedef struct {
+  int dim;           /* 2 or 3 */
+  int n;             /* full side */
+  double L;          /* domain [-L, L]^dim */
+  double T;          /* final time */
+  int nt;            /* time steps; nt-1 are taken (kg_driver_dev_2d.cpp:144) */
+  int num_snapshots; /* snapshot k at step k*(nt/num_snapshots) */
+  int krylov_dim;    /* reference: 10 (kg_driver_dev_2d.cpp:138) */
+  int method;        /* NLS_KGE_* */
+  double mu2;        /* mass term: operator -L + mu2 I (0 = reference equation) */
+  int q6;            /* 3D: keep the reference's cross-plane couplings */
+} nls_kge_params;
+typedef struct nls_kge nls_kge;
+/* Assembles the operator from c (negated for Gautschi/sEWI, as the drivers do),
+ * uploads m, u0, v0, sets u_{-1} = u0 - dt v0 and stores snapshot 0.
+ * Replaces KGESolverDevice::KGESolverDevice (device/include/kg_dev.hpp:32-80). */
+NLS_API int nls_kge_create(const nls_kge_params *p, const double *m, const double *c,
+                           const double *u0, const double *v0, nls_kge **out);
+/* Same, with a caller-assembled host CSR (the constructor's CSR arguments). */
+NLS_API int nls_kge_create_csr(const nls_kge_params *p, int64_t nnz, const int32_t *row_ptr,
+                               const int32_t *col, const double *val, const double *m,
+                               const double *u0, const double *v0, nls_kge **out);
+NLS_API int nls_kge_destroy(nls_kge *h);
+/* KGESolverDevice::step (kg_dev.hpp:97-100 -> kg_single.cuh:57-127). */
+NLS_API int nls_kge_step(nls_kge *h);
+/* KGESolverDevice::step_sv (kg_dev.hpp:102-105 -> kg_single.cuh:129-179). */
+NLS_API int nls_kge_step_sv(nls_kge *h);
+/* KGESolverDevice::apply_bc (kg_dev.hpp:116-121 -> boundaries.cuh:9-81). */
+NLS_API int nls_kge_apply_bc(nls_kge *h);
+/* KGESolverDevice::store_snapshot (kg_dev.hpp:123-131). */
+NLS_API int nls_kge_store_snapshot(nls_kge *h, int idx);
+/* KGESolverDevice::transfer_snapshots (kg_dev.hpp:107-114): host u, v of num_snapshots*N. */
+NLS_API int nls_kge_transfer_snapshots(nls_kge *h, double *u, double *v);
+/* The driver time loop (kg_driver_dev_2d.cpp:144-153): nt-1 steps of the
+ * configured method, Neumann copy after each, snapshot cadence nt/num_snapshots. */
+NLS_API int nls_kge_run(nls_kge *h);
+/* Current u (and v if non-NULL) to host. */
+NLS_API int nls_kge_get_state(nls_kge *h, double *u, double *v);
+
+/* ---------------------------------------------------------------- NLSE solver (plain complex128) */
+enum { NLS_NLSE_STRANG = 0, NLS_NLSE_SEWI = 1 };
+typedef struct {
+  int dim, n;
+  double L, T;
+  int nt, num_snapshots;
+  int krylov_dim; /* reference: 20 (2D Strang), 10, 25, 15 (nlse_cubic_*_driver_*.cpp) */
+  int method;     /* NLS_NLSE_* */
+  int q6;
+} nls_nlse_params;
+typedef struct nls_nlse nls_nlse;
+/* Replaces NLSESolverDevice::NLSESolverDevice (device/include/nlse_dev.hpp:78-163) and
+ * the snapshot-0 store of the driver (nlse_cubic_driver_2d.cpp:106). u0 complex128. */
+NLS_API int nls_nlse_create(const nls_nlse_params *p, const double *m, const double *c,
+                            const double *u0, nls_nlse **out);
+NLS_API int nls_nlse_destroy(nls_nlse *h);
+/* NLSESolverDevice::step (nlse_dev.hpp:187-203); snapshot when step % freq == 0. */
+NLS_API int nls_nlse_step(nls_nlse *h, int step_number);
+/* NLSESolverDevice::step_sewi (nlse_dev.hpp:205-239). */
+NLS_API int nls_nlse_step_sewi(nls_nlse *h, int step_number);
+/* NLSESolverDevice::apply_bc (nlse_dev.hpp:178-185). */
+NLS_API int nls_nlse_apply_bc(nls_nlse *h);
+/* Driver loop (nlse_cubic_driver_2d.cpp:107-110) and the snapshot array (ns*N complex128). */
+NLS_API int nls_nlse_run(nls_nlse *h);
+NLS_API int nls_nlse_transfer_snapshots(nls_nlse *h, double *u);
+NLS_API int nls_nlse_get_state(nls_nlse *h, double *u);
+
+/* ---------------------------------------------------------------- stochastic phi^4 (plain FP64) */
+enum { NLS_SP4_REFERENCE = 0, NLS_SP4_DAMPED = 1 };
+typedef struct {
+  int dim, n;
+  double L, T;
+  int nt, num_snapshots;
+  int mode;       /* NLS_SP4_REFERENCE: stochastic_phi4.cuh:19-77 form;
+                     NLS_SP4_DAMPED: u_tt = Lu + m(u - u^3) - gamma u_t + sigma xi */
+  int q6;
+  double gamma, sigma;
+  uint64_t seed;  /* Philox key */
+  uint32_t stream_id;
+} nls_sp4_params;
+typedef struct nls_sp4 nls_sp4;
+/* Replaces SP4SolverDevice::SP4SolverDevice (device/include/stochastic_phi4_dev.hpp:29-88). */
+NLS_API int nls_sp4_create(const nls_sp4_params *p, const double *m, const double *c,
+                           const double *u0, const double *v0, nls_sp4 **out);
+NLS_API int nls_sp4_destroy(nls_sp4 *h);
+/* SP4SolverDevice::step (stochastic_phi4_dev.hpp:105-109 -> stochastic_phi4.cuh:19-77). */
+NLS_API int nls_sp4_step(nls_sp4 *h);
+NLS_API int nls_sp4_apply_bc(nls_sp4 *h);
+NLS_API int nls_sp4_store_snapshot(nls_sp4 *h, int idx);
+NLS_API int nls_sp4_transfer_snapshots(nls_sp4 *h, double *u, double *v);
+/* Driver loop (device/drivers/sp4_sv_driver_dev_2d.cpp:141-150). */
+NLS_API int nls_sp4_run(nls_sp4 *h);
+
+/* Philox4x32-10 normals for `count` cells of (seed, stream, step) to host
+ * (replaces the per-cell thrust::default_random_engine draw, stochastic_phi4.cuh:37-53). */
+NLS_API int nls_philox_normals(uint64_t seed, uint32_t stream_id, uint32_t step, int64_t count,
+                               double *out_host);
+
+/* ---------------------------------------------------------------- fused batched path */
+/* Many trajectories (uneven grids allowed) advanced together by the fused
+ * FP32 matrix-free stencil+Lanczos kernels with FP64 state and FP64 small
+ * stage; steps captured into HIP graphs.  No reference counterpart as an
+ * object: it replaces running one kg_gautschi_2d_dev / nlse_2d_dev / sp4 process
+ * per trajectory (scripts/real_launcher_2d.py:260-306). */
+enum { NLS_EQ_KGE = 0, NLS_EQ_NLSE = 1, NLS_EQ_SP4 = 2 };
+typedef struct {
+  int equation;      /* NLS_EQ_* */
+  int method;        /* KGE: NLS_KGE_*; NLSE: NLS_NLSE_STRANG; SP4: NLS_SP4_* */
+  int dim;
+  double dt;
+  int krylov_dim;    /* 0 = pick with the autotuner */
+  double mu2;        /* KGE mass term */
+  int q6;
+  double gamma;      /* SP4 damping */
+  int use_graph;     /* capture steps into HIP graphs */
+  int snapshot_every;/* steps between snapshots (0 = none kept on device) */
+  int max_snapshots; /* snapshot slots per trajectory */
+  int cells_per_thread; /* tile: 4, 8 or 16 cells per thread; 0 = autotune.  A fixed
+                           tile makes every trajectory's reduction order, hence its
+                           result, independent of the other trajectories in the batch */
+} nls_batch_params;
+typedef struct {
+  int n;             /* full side */
+  double L;          /* half-width */
+  double sigma;      /* SP4 noise amplitude */
+  uint64_t seed;     /* SP4 Philox key */
+} nls_traj_desc;
+typedef struct nls_batch nls_batch;
+NLS_API int nls_batch_create(const nls_batch_params *p, int ntraj, const nls_traj_desc *trajs,
+                             nls_batch **out);
+NLS_API int nls_batch_destroy(nls_batch *b);
+/* Host inputs of trajectory t: m, c (N doubles), u0 (N doubles, or N complex128
+ * for NLSE), v0 (N doubles, ignored for NLSE). Resets its step counter and
+ * stores snapshot 0. */
+NLS_API int nls_batch_set_initial(nls_batch *b, int t, const double *m, const double *c,
+                                  const double *u0, const double *v0);
+/* Advances every trajectory by nsteps; *ms receives the HIP-event time of the
+ * region on the batch stream (NULL to skip). */
+NLS_API int nls_batch_run(nls_batch *b, int nsteps, float *ms);
+/* Snapshots taken so far of trajectory t (count*N doubles each; complex128 for NLSE). */
+NLS_API int nls_batch_get_snapshots(nls_batch *b, int t, double *u, double *v, int *count);
+/* Current state of trajectory t. */
+NLS_API int nls_batch_get_state(nls_batch *b, int t, double *u, double *v);
+/* Krylov dimension in use, total cells, bytes of device memory held. */
+NLS_API int nls_batch_info(nls_batch *b, int *krylov_dim, int64_t *cells, size_t *device_bytes);
+/* Per-kernel timing of the last nls_batch_run with profiling on: average
+ * duration (ms) and launches of the dominant Lanczos pass kernel. */
+NLS_API int nls_batch_set_profiling(nls_batch *b, int on);
+NLS_API int nls_batch_pass_timing(nls_batch *b, double *avg_ms, int *launches,
+                                  double *bytes_per_launch);
+/* Device bytes a batch with these trajectories would hold (admission control). */
+NLS_API int nls_batch_estimate_bytes(const nls_batch_params *p, int ntraj,
+                                     const nls_traj_desc *trajs, size_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NLS_H */

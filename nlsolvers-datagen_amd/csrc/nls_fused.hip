@@ -1,0 +1,881 @@
+// Fused batched path: many trajectories (uneven grids allowed) advanced together.
+//
+// Klein-Gordon Gautschi / sEWI step (reference: device/include/kg_single.cuh:57-127)
+//   u+ = 2 cos(tau W) u - u- + tau^2 psi(tau W) (-m u^3),   W = sqrt(A), A = -L_c + mu2 I
+// is evaluated in increment form
+//   u+ = 2u - u- + 2 [(cos(tau W) - I) u] + tau^2 [psi(tau W) g],  g = -m u^3,
+// so the two Krylov results are small corrections and FP32 Krylov vectors cost
+// no accuracy in the FP64 state (the two-step recurrence amplifies an absolute
+// error e in u+ by ~1/(tau w_min), which FP32 state would not survive over
+// 10^4 steps).
+//
+// Both Krylov sequences (seq 0 on u, seq 1 on g) advance together.  One pass j
+// per Lanczos iteration does, for every cell of every trajectory:
+//   v_j   = (w'_{j-1} - alpha_{j-1} v_{j-1}) / b_j      at the cell and its neighbours
+//   A v_j  by the matrix-free 5-point / 7-point stencil of div(c grad) (+ mu2)
+//   w'_j  = A v_j - b_j v_{j-1}
+// stores v_j and w'_j and emits per-chunk partial sums <w',v>, <w',w'>, <v,v>.
+// The next pass reduces the partials of its own trajectory in a fixed order
+// (alpha_j = <w',v>/<v,v>, b_{j+1}^2 = <w',w'> - <w',v>^2/<v,v>), so each pass
+// needs exactly one global reduction, results do not depend on which other
+// trajectories share the launch, and the stencil, axpy, dot and norm of the
+// reference's iteration (lanczos.hpp:234-263) are one sweep over memory.
+// As in the reference, alpha_{m-1} is not computed (T[m-1,m-1] = 0, Q1).
+// No Gram-Schmidt pass is made (the reference makes one per iteration, Q3).
+//
+// Then one wavefront per (trajectory, sequence) eigen-decomposes T in FP64 and
+// forms y = b0 Q f(L) Q^T e1 (nls_small.hpp), and the combine pass forms
+// sum_k y_k v_k, the FP64 update, the velocity at snapshot steps, and the
+// Neumann ghost copy (boundaries.cuh:9-81: ghost = value at clamped coordinates).
+#include "nls.h"
+#include "nls_common.hpp"
+#include "nls_philox.hpp"
+#include "nls_plain.hpp"
+#include "nls_small.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+namespace nls {
+namespace fused {
+
+constexpr int FBS = 256; // threads per workgroup
+constexpr int NSEQ = 2;  // Krylov sequences per KGE step
+constexpr int PN = 3;    // partial sums per sequence
+constexpr int NXCD = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin
+
+struct TrajDev {
+  int n, dim, nchunks, chunk0;
+  int64_t off, N;
+  float s, mu2;
+  double tau;
+};
+
+struct TrajScal {
+  double b0[NSEQ];
+  double alpha[NSEQ][kMaxKrylov];
+  double beta[NSEQ][kMaxKrylov + 1]; // beta[s][j] = b_j, j >= 1
+  double y[NSEQ][kMaxKrylov];
+  int status;
+};
+
+struct KgeArgs {
+  const TrajDev *traj;
+  const int2 *cmap;
+  TrajScal *scal;
+  double *part; // [2 parity][total_chunks][NSEQ*PN]
+  int total_chunks;
+  const float *mf, *cf; // coefficient fields m (cubic) and c (conductivity)
+  const double *u, *up;
+  double *un;
+  float *V; // [NSEQ][m-2][cells]
+  float *W; // [NSEQ][2][cells]
+  int64_t cells;
+  int m;
+  int q6;
+  int kind1; // f of sequence 1
+};
+
+// ---------------------------------------------------------------- neighbour helpers
+struct Nbr {
+  int64_t off[6];
+  bool has[6];
+  int cnt;
+};
+
+template <int DIM>
+__device__ inline void neighbours(int64_t i, int n, int64_t N, int q6, Nbr &nb) {
+  if (DIM == 2) {
+    const int64_t r = i / n, col = i - r * n;
+    nb.cnt = 4;
+    nb.off[0] = -n; nb.has[0] = r > 0;
+    nb.off[1] = -1; nb.has[1] = col > 0;
+    nb.off[2] = 1;  nb.has[2] = col < n - 1;
+    nb.off[3] = n;  nb.has[3] = r < n - 1;
+  } else {
+    const int64_t n2 = (int64_t)n * n;
+    const int64_t col = i % n, inplane = i % n2;
+    nb.cnt = 6;
+    nb.off[0] = -n2; nb.has[0] = i >= n2;
+    nb.off[1] = -n;  nb.has[1] = q6 ? (i >= n) : (inplane >= n);
+    nb.off[2] = -1;  nb.has[2] = col > 0;
+    nb.off[3] = 1;   nb.has[3] = col < n - 1;
+    nb.off[4] = n;   nb.has[4] = q6 ? (i + n < N) : (inplane < n2 - n);
+    nb.off[5] = n2;  nb.has[5] = i + n2 < N;
+  }
+}
+
+template <int DIM> __device__ inline int64_t clamp_src(int64_t i, int n) {
+  auto cl = [n](int64_t a) { return a < 1 ? (int64_t)1 : (a > n - 2 ? (int64_t)n - 2 : a); };
+  if (DIM == 2) {
+    const int64_t r = i / n, col = i - r * n;
+    return cl(r) * n + cl(col);
+  } else {
+    const int64_t n2 = (int64_t)n * n;
+    const int64_t a = i / n2, rem = i - a * n2, b = rem / n, k = rem - b * n;
+    return (cl(a) * n + cl(b)) * n + cl(k);
+  }
+}
+
+// Seed vectors of the two sequences: x0 = u, x1 = g = -m u^3 (FP64 -> FP32).
+__device__ inline float seed_x(int s, const KgeArgs &a, int64_t g) {
+  const double uu = a.u[g];
+  if (s == 0) return (float)uu;
+  return (float)(-(double)a.mf[g] * uu * uu * uu);
+}
+
+// ---------------------------------------------------------------- per-trajectory reduction
+// Sums the partials of this trajectory's chunks (fixed order) into red[0..NSEQ*PN).
+__device__ inline void reduce_partials(const KgeArgs &a, const TrajDev &T, int parity,
+                                       double *red) {
+  if (threadIdx.x < kWave) {
+    const int lane = threadIdx.x;
+    double acc[NSEQ * PN];
+#pragma unroll
+    for (int k = 0; k < NSEQ * PN; ++k) acc[k] = 0.0;
+    const double *p = a.part + ((size_t)parity * a.total_chunks + T.chunk0) * (NSEQ * PN);
+    for (int g = lane; g < T.nchunks; g += kWave) {
+#pragma unroll
+      for (int k = 0; k < NSEQ * PN; ++k) acc[k] += p[(size_t)g * (NSEQ * PN) + k];
+    }
+#pragma unroll
+    for (int k = 0; k < NSEQ * PN; ++k) {
+      const double v = wave_sum(acc[k]);
+      if (lane == 0) red[k] = v;
+    }
+  }
+  __syncthreads();
+}
+
+struct Coef {
+  float A, B, C; // v = (w - A*vprev) * B ; w' = Av - C * vprev(center)
+  double alpha, b;
+};
+
+// From <w',v>, <w',w'>, <v,v> of pass j-1: alpha_{j-1}, b_j.
+__device__ inline void lanczos_scalars(double q0, double q1, double q2, double &alpha,
+                                       double &b) {
+  alpha = q2 > 0.0 ? q0 / q2 : 0.0;
+  double b2 = q2 > 0.0 ? q1 - q0 * q0 / q2 : 0.0;
+  // Breakdown (invariant subspace, or cancellation down to FP32 noise): b = 0.
+  b = (b2 > 1e-14 * q1 && b2 > 0.0) ? sqrt(b2) : 0.0;
+}
+
+// ---------------------------------------------------------------- Lanczos pass
+// STAGE 0: pass j = 0 from the seeds; STAGE 1: j = 1; STAGE 2: j >= 2.
+template <int DIM, int STAGE, int CPT>
+__global__ __launch_bounds__(FBS) void k_kge_pass(KgeArgs a, int j) {
+  const int2 cm = a.cmap[blockIdx.x];
+  if (cm.x < 0) return;
+  const TrajDev T = a.traj[cm.x];
+  __shared__ double red[NSEQ * PN];
+  __shared__ double sh[FBS / kWave];
+  Coef cf[NSEQ];
+  double inv_b0[NSEQ];
+  if (STAGE >= 1) {
+    reduce_partials(a, T, (j - 1) & 1, red);
+#pragma unroll
+    for (int s = 0; s < NSEQ; ++s) {
+      const double q0 = red[s * PN], q1 = red[s * PN + 1], q2 = red[s * PN + 2];
+      if (STAGE == 1) {
+        // pass-0 partials are <x,x>, <x,z>, <z,z> with z = A x (x unnormalised)
+        const double b0 = sqrt(q0);
+        const double al = q0 > 0.0 ? q1 / q0 : 0.0;
+        double b1sq = q0 > 0.0 ? (q2 - q1 * q1 / q0) / q0 : 0.0;
+        const double b1 = (b1sq > 1e-14 * (q0 > 0.0 ? q2 / q0 : 0.0) && b1sq > 0.0) ? sqrt(b1sq) : 0.0;
+        inv_b0[s] = b0 > 0.0 ? 1.0 / b0 : 0.0;
+        cf[s].alpha = al;
+        cf[s].b = b1;
+        cf[s].A = (float)al;
+        cf[s].B = (b0 > 0.0 && b1 > 0.0) ? (float)(1.0 / (b0 * b1)) : 0.0f;
+        cf[s].C = (float)(b1 * inv_b0[s]);
+        if (cm.y == 0 && threadIdx.x == 0) {
+          a.scal[cm.x].b0[s] = b0;
+          a.scal[cm.x].alpha[s][0] = al;
+          a.scal[cm.x].beta[s][1] = b1;
+        }
+      } else {
+        double al, b;
+        lanczos_scalars(q0, q1, q2, al, b);
+        cf[s].alpha = al;
+        cf[s].b = b;
+        cf[s].A = (float)al;
+        cf[s].B = b > 0.0 ? (float)(1.0 / b) : 0.0f;
+        cf[s].C = (float)b;
+        if (cm.y == 0 && threadIdx.x == 0) {
+          a.scal[cm.x].alpha[s][j - 1] = al;
+          a.scal[cm.x].beta[s][j] = b;
+        }
+      }
+    }
+  }
+  const int n = T.n;
+  const float sc = T.s, mu2 = T.mu2;
+  const int64_t cells = a.cells;
+  double acc[NSEQ][PN];
+#pragma unroll
+  for (int s = 0; s < NSEQ; ++s)
+#pragma unroll
+    for (int k = 0; k < PN; ++k) acc[s][k] = 0.0;
+
+  const int64_t base = (int64_t)cm.y * (FBS * CPT);
+#pragma unroll
+  for (int kk = 0; kk < CPT; ++kk) {
+    const int64_t i = base + threadIdx.x + (int64_t)kk * FBS;
+    if (i >= T.N) break;
+    const int64_t gi = T.off + i;
+    Nbr nb;
+    neighbours<DIM>(i, n, T.N, a.q6, nb);
+    const float ci = a.cf[gi];
+    float w[6];
+    float wsum = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 2 * DIM; ++q) {
+      w[q] = nb.has[q] ? 0.5f * (ci + a.cf[gi + nb.off[q]]) : 0.0f;
+      wsum += w[q];
+    }
+#pragma unroll
+    for (int s = 0; s < NSEQ; ++s) {
+      auto val = [&](int64_t g) -> float {
+        if (STAGE == 0) return seed_x(s, a, g);
+        if (STAGE == 1) {
+          const float z = a.W[((size_t)s * 2 + 0) * cells + g];
+          return (z - cf[s].A * seed_x(s, a, g)) * cf[s].B;
+        }
+        const float wv = a.W[((size_t)s * 2 + ((j - 1) & 1)) * cells + g];
+        const float vp = a.V[((size_t)s * (a.m - 2) + (j - 2)) * cells + g];
+        return (wv - cf[s].A * vp) * cf[s].B;
+      };
+      const float vi = val(gi);
+      float nbsum = 0.0f;
+#pragma unroll
+      for (int q = 0; q < 2 * DIM; ++q)
+        if (nb.has[q]) nbsum += w[q] * val(gi + nb.off[q]);
+      const float Av = sc * (wsum * vi - nbsum) + mu2 * vi;
+      if (STAGE == 0) {
+        a.W[((size_t)s * 2 + 0) * cells + gi] = Av;
+        acc[s][0] += (double)vi * vi;
+        acc[s][1] += (double)vi * Av;
+        acc[s][2] += (double)Av * Av;
+      } else {
+        float vprev;
+        if (STAGE == 1) vprev = seed_x(s, a, gi);
+        else vprev = a.V[((size_t)s * (a.m - 2) + (j - 2)) * cells + gi];
+        const float wp = Av - cf[s].C * vprev;
+        a.V[((size_t)s * (a.m - 2) + (j - 1)) * cells + gi] = vi;
+        a.W[((size_t)s * 2 + (j & 1)) * cells + gi] = wp;
+        acc[s][0] += (double)wp * vi;
+        acc[s][1] += (double)wp * wp;
+        acc[s][2] += (double)vi * vi;
+      }
+    }
+  }
+  // block partials -> part[parity j&1][global chunk]
+  double *dst = a.part + ((size_t)(j & 1) * a.total_chunks + T.chunk0 + cm.y) * (NSEQ * PN);
+#pragma unroll
+  for (int s = 0; s < NSEQ; ++s)
+#pragma unroll
+    for (int k = 0; k < PN; ++k) {
+      const double t = block_sum<FBS>(acc[s][k], sh);
+      if (threadIdx.x == 0) dst[s * PN + k] = t;
+    }
+}
+
+// ---------------------------------------------------------------- small stage
+// One wavefront per (trajectory, sequence): alpha_{m-2}, b_{m-1} from the last
+// pass, T = tridiag(b, [alpha_0..alpha_{m-2}, 0], b), y = b0 Q f(L) Q^T e1.
+__global__ __launch_bounds__(kWave) void k_kge_small(KgeArgs a, int last_pass) {
+  const int t = blockIdx.x / NSEQ, s = blockIdx.x % NSEQ;
+  const TrajDev T = a.traj[t];
+  __shared__ SmallWS ws;
+  __shared__ double red[NSEQ * PN];
+  __shared__ double yr[kMaxKrylov];
+  const int m = a.m;
+  // reduce last-pass partials (one wave = whole block)
+  {
+    const int lane = threadIdx.x;
+    double acc[PN] = {0.0, 0.0, 0.0};
+    const double *p =
+        a.part + ((size_t)(last_pass & 1) * a.total_chunks + T.chunk0) * (NSEQ * PN) + s * PN;
+    for (int g = lane; g < T.nchunks; g += kWave)
+      for (int k = 0; k < PN; ++k) acc[k] += p[(size_t)g * (NSEQ * PN) + k];
+    for (int k = 0; k < PN; ++k) {
+      const double v = wave_sum(acc[k]);
+      if (lane == 0) red[k] = v;
+    }
+  }
+  __syncthreads();
+  TrajScal &S = a.scal[t];
+  if (threadIdx.x == 0) {
+    double al, b;
+    lanczos_scalars(red[0], red[1], red[2], al, b);
+    S.alpha[s][m - 2] = al;
+    S.beta[s][m - 1] = b;
+    for (int k = 0; k < m - 1; ++k) ws.d[k] = S.alpha[s][k];
+    ws.d[m - 1] = 0.0; // Q1
+    for (int k = 0; k < m - 1; ++k) ws.e[k] = S.beta[s][k + 1];
+    for (int k = 0; k < m; ++k) { ws.phr[k] = 1.0; ws.phi[k] = 0.0; }
+  }
+  __syncthreads();
+  const double b0 = S.b0[s];
+  if (small_ql(m, ws)) {
+    if (threadIdx.x == 0) S.status = 1;
+    return;
+  }
+  const int kind = s == 0 ? F_COSM1_SQRT : a.kind1;
+  small_apply_f(m, ws, kind, T.tau, 0.0, b0, yr, nullptr);
+  __syncthreads();
+  if (threadIdx.x < m) S.y[s][threadIdx.x] = yr[threadIdx.x];
+}
+
+// ---------------------------------------------------------------- combine
+struct SnapArgs {
+  double *u; // snapshot slot base (pool, cells)
+  double *v;
+};
+
+template <int DIM>
+__device__ inline double kge_unew(const KgeArgs &a, const TrajScal &S, const double *ys0,
+                                  const double *ys1, int64_t g, double tau) {
+  const int m = a.m;
+  const int64_t cells = a.cells;
+  double d[NSEQ];
+#pragma unroll
+  for (int s = 0; s < NSEQ; ++s) {
+    const double *y = s == 0 ? ys0 : ys1;
+    const double b0 = S.b0[s];
+    double acc = b0 > 0.0 ? y[0] * ((double)seed_x(s, a, g) / b0) : 0.0;
+    for (int k = 1; k <= m - 2; ++k) acc += y[k] * (double)a.V[((size_t)s * (m - 2) + (k - 1)) * cells + g];
+    const double bl = S.beta[s][m - 1];
+    if (bl > 0.0) {
+      const float wl = a.W[((size_t)s * 2 + ((m - 2) & 1)) * cells + g];
+      const float vl = a.V[((size_t)s * (m - 2) + (m - 3)) * cells + g];
+      const float al = (float)S.alpha[s][m - 2];
+      acc += y[m - 1] * ((double)(wl - al * vl) / bl);
+    }
+    d[s] = acc;
+  }
+  return 2.0 * a.u[g] - a.up[g] + 2.0 * d[0] + (tau * tau) * d[1];
+}
+
+template <int DIM, int CPT>
+__global__ __launch_bounds__(FBS) void k_kge_combine(KgeArgs a, SnapArgs snap) {
+  const int2 cm = a.cmap[blockIdx.x];
+  if (cm.x < 0) return;
+  const TrajDev T = a.traj[cm.x];
+  const TrajScal &S = a.scal[cm.x];
+  __shared__ double ys[NSEQ][kMaxKrylov];
+  if (threadIdx.x < a.m) {
+    ys[0][threadIdx.x] = S.y[0][threadIdx.x];
+    ys[1][threadIdx.x] = S.y[1][threadIdx.x];
+  }
+  __syncthreads();
+  const int n = T.n;
+  const int64_t base = (int64_t)cm.y * (FBS * CPT);
+#pragma unroll
+  for (int kk = 0; kk < CPT; ++kk) {
+    const int64_t i = base + threadIdx.x + (int64_t)kk * FBS;
+    if (i >= T.N) break;
+    const int64_t src = clamp_src<DIM>(i, n);
+    const int64_t gi = T.off + i, gs = T.off + src;
+    const double us = kge_unew<DIM>(a, S, ys[0], ys[1], gs, T.tau);
+    a.un[gi] = us;
+    if (snap.u) {
+      const double ui = src == i ? us : kge_unew<DIM>(a, S, ys[0], ys[1], gi, T.tau);
+      snap.u[gi] = us;
+      snap.v[gi] = (ui - a.u[gi]) / T.tau;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- batch object
+struct Layout {
+  std::vector<TrajDev> traj;
+  std::vector<int2> cmap;
+  int total_chunks = 0;
+  int64_t cells = 0;
+};
+
+static Layout make_layout(const nls_traj_desc *td, int ntraj, int dim, int cpt, double dt,
+                          double mu2) {
+  Layout L;
+  const int CH = FBS * cpt;
+  L.traj.resize(ntraj);
+  std::vector<int> nch(ntraj);
+  for (int t = 0; t < ntraj; ++t) {
+    TrajDev &T = L.traj[t];
+    T.n = td[t].n;
+    T.dim = dim;
+    T.N = dim == 3 ? (int64_t)T.n * T.n * T.n : (int64_t)T.n * T.n;
+    T.off = L.cells;
+    L.cells += (T.N + 63) / 64 * 64; // keep every trajectory 256-byte aligned
+    T.nchunks = (int)((T.N + CH - 1) / CH);
+    T.chunk0 = L.total_chunks;
+    L.total_chunks += T.nchunks;
+    const double dx = 2.0 * td[t].L / (T.n - 1);
+    T.s = (float)(1.0 / (dx * dx));
+    T.mu2 = (float)mu2;
+    T.tau = dt;
+    nch[t] = T.nchunks;
+  }
+  // XCD-aware block map: workgroup b runs on XCD b % 8, so all chunks of a
+  // trajectory are placed in one XCD's list (their halo rows then meet in that
+  // XCD's L2); trajectories go largest-first to the least-loaded XCD.
+  std::vector<int> order(ntraj);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return nch[x] > nch[y]; });
+  std::array<std::vector<int2>, NXCD> bins;
+  std::array<int64_t, NXCD> load{};
+  for (int t : order) {
+    int best = 0;
+    for (int x = 1; x < NXCD; ++x)
+      if (load[x] < load[best]) best = x;
+    for (int c = 0; c < nch[t]; ++c) bins[best].push_back(int2{t, c});
+    load[best] += nch[t];
+  }
+  size_t maxlen = 0;
+  for (auto &b : bins) maxlen = std::max(maxlen, b.size());
+  L.cmap.assign(maxlen * NXCD, int2{-1, -1});
+  for (int x = 0; x < NXCD; ++x)
+    for (size_t k = 0; k < bins[x].size(); ++k) L.cmap[k * NXCD + x] = bins[x][k];
+  return L;
+}
+
+// Krylov dimension from an a-priori bound: the error of the m-step Lanczos
+// approximation of cos(tau sqrt(A)) / sinc^2 behaves like (tau^2 rho)^m / (2m)!
+// with rho >= lambda_max(A) (Gershgorin: 4 dim max(c) / dx^2 + mu2).  Smallest m
+// in [8, 16] whose bound is below 1e-14.
+static int autotune_krylov(const std::vector<double> &rho_tau2) {
+  double worst = 0.0;
+  for (double r : rho_tau2) worst = std::max(worst, r);
+  for (int m = 8; m <= 16; ++m) {
+    double lg = m * std::log(std::max(worst, 1e-300)) - std::lgamma(2.0 * m + 1.0);
+    if (lg < std::log(1e-14)) return m;
+  }
+  return 16;
+}
+
+} // namespace fused
+} // namespace nls
+
+using namespace nls;
+using namespace nls::fused;
+
+struct nls_batch {
+  nls_batch_params p{};
+  int ntraj = 0;
+  std::vector<nls_traj_desc> desc;
+  Layout lay;
+  int m = 10, cpt = 8;
+  bool tuned = false;
+  hipStream_t s = nullptr;
+  TrajDev *d_traj = nullptr;
+  int2 *d_cmap = nullptr;
+  size_t cmap_cap = 0;
+  TrajScal *d_scal = nullptr;
+  double *d_part = nullptr;
+  float *d_mf = nullptr, *d_cf = nullptr, *d_V = nullptr, *d_W = nullptr;
+  double *d_state[3] = {nullptr, nullptr, nullptr};
+  double *d_snapU = nullptr, *d_snapV = nullptr;
+  size_t bytes = 0;
+  int64_t step = 0;
+  int nsnap = 0;
+  hipGraphExec_t graphs[3] = {nullptr, nullptr, nullptr};
+  bool profiling = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  double pass_ms = 0.0;
+  int pass_launches = 0;
+
+  template <class T> T *alloc(size_t count) {
+    void *q = nullptr;
+    const size_t b = sizeof(T) * std::max<size_t>(count, 1);
+    NLS_HIP(hipMalloc(&q, b));
+    bytes += b;
+    return static_cast<T *>(q);
+  }
+
+  ~nls_batch() {
+    if (s) (void)hipStreamSynchronize(s);
+    drop_graphs();
+    for (auto e : ev_pool) (void)hipEventDestroy(e);
+    void *ptrs[] = {d_traj, d_cmap, d_scal, d_part, d_mf, d_cf, d_V, d_W,
+                    d_state[0], d_state[1], d_state[2], d_snapU, d_snapV};
+    for (void *q : ptrs)
+      if (q) (void)hipFree(q);
+    if (s) (void)hipStreamDestroy(s);
+  }
+
+  void drop_graphs() {
+    for (auto &g : graphs)
+      if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+  }
+
+  KgeArgs args(int cur) const {
+    KgeArgs a{};
+    a.traj = d_traj;
+    a.cmap = d_cmap;
+    a.scal = d_scal;
+    a.part = d_part;
+    a.total_chunks = lay.total_chunks;
+    a.mf = d_mf;
+    a.cf = d_cf;
+    a.u = d_state[cur % 3];
+    a.up = d_state[(cur + 2) % 3];
+    a.un = d_state[(cur + 1) % 3];
+    a.V = d_V;
+    a.W = d_W;
+    a.cells = lay.cells;
+    a.m = m;
+    a.q6 = p.q6;
+    a.kind1 = p.method == NLS_KGE_SEWI ? F_SINC2_HALF : F_SINC2_SQRT;
+    return a;
+  }
+
+  void set_layout(int cpt_) {
+    cpt = cpt_;
+    const double mu2 = p.mu2;
+    lay = make_layout(desc.data(), ntraj, p.dim, cpt, p.dt, mu2);
+    NLS_REQUIRE(lay.cmap.size() <= cmap_cap, "internal: chunk map larger than allocated");
+    NLS_HIP(hipMemcpyAsync(d_traj, lay.traj.data(), sizeof(TrajDev) * ntraj,
+                           hipMemcpyHostToDevice, s));
+    NLS_HIP(hipMemcpyAsync(d_cmap, lay.cmap.data(), sizeof(int2) * lay.cmap.size(),
+                           hipMemcpyHostToDevice, s));
+    NLS_HIP(hipStreamSynchronize(s));
+  }
+
+  hipEvent_t next_event() {
+    if (ev_used == ev_pool.size()) {
+      hipEvent_t e;
+      NLS_HIP(hipEventCreate(&e));
+      ev_pool.push_back(e);
+    }
+    return ev_pool[ev_used++];
+  }
+
+  template <int DIM, int CPT> void launch_step_t(int cur, bool snapshot, int slot, bool prof) {
+    const KgeArgs a = args(cur);
+    const dim3 grid((unsigned)lay.cmap.size()), blk(FBS);
+    hipLaunchKernelGGL((k_kge_pass<DIM, 0, CPT>), grid, blk, 0, s, a, 0);
+    if (m - 2 >= 1) hipLaunchKernelGGL((k_kge_pass<DIM, 1, CPT>), grid, blk, 0, s, a, 1);
+    for (int j = 2; j <= m - 2; ++j) {
+      if (prof) NLS_HIP(hipEventRecord(next_event(), s));
+      hipLaunchKernelGGL((k_kge_pass<DIM, 2, CPT>), grid, blk, 0, s, a, j);
+      if (prof) NLS_HIP(hipEventRecord(next_event(), s));
+    }
+    hipLaunchKernelGGL(k_kge_small, dim3(ntraj * NSEQ), dim3(kWave), 0, s, a, m - 2);
+    SnapArgs sn{nullptr, nullptr};
+    if (snapshot) {
+      sn.u = d_snapU + (size_t)slot * lay.cells;
+      sn.v = d_snapV + (size_t)slot * lay.cells;
+    }
+    hipLaunchKernelGGL((k_kge_combine<DIM, CPT>), grid, blk, 0, s, a, sn);
+    check_launch("fused kge step");
+  }
+
+  void launch_step(int cur, bool snapshot, int slot, bool prof) {
+    if (p.dim == 2) {
+      if (cpt == 4) launch_step_t<2, 4>(cur, snapshot, slot, prof);
+      else if (cpt == 8) launch_step_t<2, 8>(cur, snapshot, slot, prof);
+      else launch_step_t<2, 16>(cur, snapshot, slot, prof);
+    } else {
+      if (cpt == 4) launch_step_t<3, 4>(cur, snapshot, slot, prof);
+      else if (cpt == 8) launch_step_t<3, 8>(cur, snapshot, slot, prof);
+      else launch_step_t<3, 16>(cur, snapshot, slot, prof);
+    }
+  }
+
+  // Tile autotuner: times one step per cells-per-thread candidate on the live
+  // buffers (a step only writes the next-state buffer and Krylov scratch, so
+  // timing runs leave the trajectory untouched) and keeps the fastest.
+  void autotune() {
+    const int cands[3] = {4, 8, 16};
+    float best = 1e30f;
+    int best_cpt = 8;
+    hipEvent_t e0, e1;
+    NLS_HIP(hipEventCreate(&e0));
+    NLS_HIP(hipEventCreate(&e1));
+    for (int c : cands) {
+      set_layout(c);
+      launch_step((int)(step % 3), false, 0, false); // warm-up
+      NLS_HIP(hipEventRecord(e0, s));
+      for (int r = 0; r < 3; ++r) launch_step((int)(step % 3), false, 0, false);
+      NLS_HIP(hipEventRecord(e1, s));
+      NLS_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      NLS_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) { best = ms; best_cpt = c; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    set_layout(best_cpt);
+    tuned = true;
+  }
+
+  void capture_graphs() {
+    drop_graphs();
+    for (int cur = 0; cur < 3; ++cur) {
+      hipGraph_t g;
+      NLS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      launch_step(cur, false, 0, false);
+      NLS_HIP(hipStreamEndCapture(s, &g));
+      NLS_HIP(hipGraphInstantiate(&graphs[cur], g, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(g);
+    }
+  }
+
+  void check_status() {
+    std::vector<TrajScal> h(ntraj);
+    NLS_HIP(hipMemcpyAsync(h.data(), d_scal, sizeof(TrajScal) * ntraj, hipMemcpyDeviceToHost, s));
+    NLS_HIP(hipStreamSynchronize(s));
+    for (int t = 0; t < ntraj; ++t)
+      NLS_REQUIRE(h[t].status == 0, "tridiagonal eigensolver did not converge (trajectory " +
+                                        std::to_string(t) + ")");
+  }
+
+  void run(int nsteps, float *ms) {
+    if (!tuned) {
+      if (p.cells_per_thread > 0) {
+        set_layout(p.cells_per_thread);
+        tuned = true;
+      } else {
+        autotune();
+      }
+      if (p.use_graph) capture_graphs();
+    }
+    hipEvent_t e0, e1;
+    NLS_HIP(hipEventCreate(&e0));
+    NLS_HIP(hipEventCreate(&e1));
+    ev_used = 0;
+    NLS_HIP(hipEventRecord(e0, s));
+    for (int k = 0; k < nsteps; ++k) {
+      const int64_t idx = step + 1; // reference step number i (kg_driver_dev_2d.cpp:144)
+      const int cur = (int)(step % 3);
+      const bool snap = p.snapshot_every > 0 && idx % p.snapshot_every == 0 &&
+                        idx / p.snapshot_every < p.max_snapshots;
+      if (snap || !p.use_graph || profiling || !graphs[cur])
+        launch_step(cur, snap, snap ? (int)(idx / p.snapshot_every) : 0, profiling);
+      else
+        NLS_HIP(hipGraphLaunch(graphs[cur], s));
+      if (snap) nsnap = std::max(nsnap, (int)(idx / p.snapshot_every) + 1);
+      ++step;
+    }
+    NLS_HIP(hipEventRecord(e1, s));
+    NLS_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    NLS_HIP(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (profiling) {
+      pass_ms = 0.0;
+      pass_launches = 0;
+      for (size_t k = 0; k + 1 < ev_used; k += 2) {
+        float d = 0;
+        NLS_HIP(hipEventElapsedTime(&d, ev_pool[k], ev_pool[k + 1]));
+        pass_ms += d;
+        ++pass_launches;
+      }
+    }
+    check_status();
+  }
+
+  double pass_bytes() const {
+    // Algorithmic bytes of one general Lanczos pass per cell: read w'_{j-1},
+    // v_{j-1} for both sequences and c (4 * 5 B), write v_j, w'_j for both (4 * 4 B).
+    return 36.0 * (double)total_cells();
+  }
+  int64_t total_cells() const {
+    int64_t c = 0;
+    for (auto &T : lay.traj) c += T.N;
+    return c;
+  }
+};
+
+static size_t estimate(const nls_batch_params &p, int ntraj, const nls_traj_desc *td, int m) {
+  int64_t cells = 0, chunks = 0;
+  for (int t = 0; t < ntraj; ++t) {
+    const int64_t N = p.dim == 3 ? (int64_t)td[t].n * td[t].n * td[t].n : (int64_t)td[t].n * td[t].n;
+    cells += (N + 63) / 64 * 64;
+    chunks += (N + FBS * 4 - 1) / (FBS * 4);
+  }
+  size_t b = 0;
+  b += sizeof(float) * cells * 2;                         // m, c
+  b += sizeof(double) * cells * 3;                        // state ring
+  b += sizeof(float) * cells * NSEQ * (size_t)(m - 2);    // Krylov basis
+  b += sizeof(float) * cells * NSEQ * 2;                  // w' ping-pong
+  b += sizeof(double) * 2 * chunks * NSEQ * PN;           // partials
+  b += sizeof(TrajScal) * ntraj;
+  b += sizeof(double) * cells * 2 * (size_t)std::max(p.max_snapshots, 0);
+  return b;
+}
+
+static int pick_m(const nls_batch_params &p, int ntraj, const nls_traj_desc *td) {
+  if (p.krylov_dim > 0) return p.krylov_dim;
+  std::vector<double> r(ntraj);
+  for (int t = 0; t < ntraj; ++t) {
+    const double dx = 2.0 * td[t].L / (td[t].n - 1);
+    // Gershgorin bound with max(c) <= 4 assumed for an unset field; refined in set_initial
+    r[t] = (4.0 * p.dim * 4.0 / (dx * dx) + p.mu2) * p.dt * p.dt;
+  }
+  return autotune_krylov(r);
+}
+
+extern "C" {
+
+int nls_batch_estimate_bytes(const nls_batch_params *p, int ntraj, const nls_traj_desc *trajs,
+                             size_t *bytes) {
+  return guard([&] {
+    NLS_REQUIRE(p && trajs && bytes && ntraj > 0, "nls_batch_estimate_bytes: bad arguments");
+    *bytes = estimate(*p, ntraj, trajs, pick_m(*p, ntraj, trajs));
+  });
+}
+
+int nls_batch_create(const nls_batch_params *p, int ntraj, const nls_traj_desc *trajs,
+                     nls_batch **out) {
+  return guard([&] {
+    NLS_REQUIRE(p && trajs && out && ntraj > 0, "nls_batch_create: bad arguments");
+    NLS_REQUIRE(p->equation == NLS_EQ_KGE, "nls_batch_create: only the Klein-Gordon equation "
+                                           "is implemented by the fused batch so far");
+    NLS_REQUIRE(p->method == NLS_KGE_GAUTSCHI || p->method == NLS_KGE_SEWI,
+                "fused Klein-Gordon batch supports the Gautschi and sEWI methods");
+    NLS_REQUIRE(p->dim == 2 || p->dim == 3, "dim must be 2 or 3");
+    NLS_REQUIRE(p->dt > 0, "dt must be positive");
+    NLS_REQUIRE(p->cells_per_thread == 0 || p->cells_per_thread == 4 ||
+                    p->cells_per_thread == 8 || p->cells_per_thread == 16,
+                "cells_per_thread must be 0 (autotune), 4, 8 or 16");
+    for (int t = 0; t < ntraj; ++t) NLS_REQUIRE(trajs[t].n >= 3 && trajs[t].L > 0, "bad trajectory");
+    auto b = std::make_unique<nls_batch>();
+    b->p = *p;
+    b->ntraj = ntraj;
+    b->desc.assign(trajs, trajs + ntraj);
+    b->m = pick_m(*p, ntraj, trajs);
+    NLS_REQUIRE(b->m >= 3 && b->m <= kMaxKrylov, "fused path needs a Krylov dimension in [3, 32]");
+    NLS_HIP(hipStreamCreate(&b->s));
+    // layout with the smallest tile (most chunks) sizes the chunk map and partials
+    Layout l4 = make_layout(trajs, ntraj, p->dim, 4, p->dt, p->mu2);
+    b->cmap_cap = l4.cmap.size() + 8 * NXCD;
+    b->d_traj = b->alloc<TrajDev>(ntraj);
+    b->d_cmap = b->alloc<int2>(b->cmap_cap);
+    b->d_scal = b->alloc<TrajScal>(ntraj);
+    b->d_part = b->alloc<double>((size_t)2 * l4.total_chunks * NSEQ * PN);
+    const int64_t cells = l4.cells;
+    b->d_mf = b->alloc<float>(cells);
+    b->d_cf = b->alloc<float>(cells);
+    for (int k = 0; k < 3; ++k) b->d_state[k] = b->alloc<double>(cells);
+    b->d_V = b->alloc<float>((size_t)cells * NSEQ * (b->m - 2));
+    b->d_W = b->alloc<float>((size_t)cells * NSEQ * 2);
+    if (p->max_snapshots > 0) {
+      b->d_snapU = b->alloc<double>((size_t)cells * p->max_snapshots);
+      b->d_snapV = b->alloc<double>((size_t)cells * p->max_snapshots);
+    }
+    NLS_HIP(hipMemsetAsync(b->d_scal, 0, sizeof(TrajScal) * ntraj, b->s));
+    for (int k = 0; k < 3; ++k) NLS_HIP(hipMemsetAsync(b->d_state[k], 0, sizeof(double) * cells, b->s));
+    NLS_HIP(hipMemsetAsync(b->d_mf, 0, sizeof(float) * cells, b->s));
+    NLS_HIP(hipMemsetAsync(b->d_cf, 0, sizeof(float) * cells, b->s));
+    b->set_layout(8);
+    *out = b.release();
+  });
+}
+
+int nls_batch_destroy(nls_batch *b) { return guard([&] { delete b; }); }
+
+int nls_batch_set_initial(nls_batch *b, int t, const double *m, const double *c, const double *u0,
+                          const double *v0) {
+  return guard([&] {
+    NLS_REQUIRE(b && t >= 0 && t < b->ntraj, "nls_batch_set_initial: bad trajectory index");
+    NLS_REQUIRE(m && c && u0 && v0, "nls_batch_set_initial: null input");
+    NLS_REQUIRE(b->step == 0, "nls_batch_set_initial: the batch has already been advanced");
+    const TrajDev &T = b->lay.traj[t];
+    const int64_t N = T.N;
+    std::vector<float> mf(N), cf(N);
+    std::vector<double> upast(N);
+    for (int64_t i = 0; i < N; ++i) {
+      mf[i] = (float)m[i];
+      cf[i] = (float)c[i];
+      upast[i] = u0[i] - b->p.dt * v0[i]; // kg_dev.hpp:55-58
+    }
+    NLS_HIP(hipMemcpyAsync(b->d_mf + T.off, mf.data(), sizeof(float) * N, hipMemcpyHostToDevice, b->s));
+    NLS_HIP(hipMemcpyAsync(b->d_cf + T.off, cf.data(), sizeof(float) * N, hipMemcpyHostToDevice, b->s));
+    NLS_HIP(hipMemcpyAsync(b->d_state[0] + T.off, u0, sizeof(double) * N, hipMemcpyHostToDevice, b->s));
+    NLS_HIP(hipMemcpyAsync(b->d_state[2] + T.off, upast.data(), sizeof(double) * N,
+                           hipMemcpyHostToDevice, b->s));
+    if (b->p.max_snapshots > 0) {
+      NLS_HIP(hipMemcpyAsync(b->d_snapU + T.off, u0, sizeof(double) * N, hipMemcpyHostToDevice, b->s));
+      NLS_HIP(hipMemcpyAsync(b->d_snapV + T.off, v0, sizeof(double) * N, hipMemcpyHostToDevice, b->s));
+      b->nsnap = std::max(b->nsnap, 1);
+    }
+    NLS_HIP(hipStreamSynchronize(b->s));
+  });
+}
+
+int nls_batch_run(nls_batch *b, int nsteps, float *ms) {
+  return guard([&] {
+    NLS_REQUIRE(b && nsteps >= 0, "nls_batch_run: bad arguments");
+    b->run(nsteps, ms);
+  });
+}
+
+int nls_batch_get_snapshots(nls_batch *b, int t, double *u, double *v, int *count) {
+  return guard([&] {
+    NLS_REQUIRE(b && t >= 0 && t < b->ntraj, "nls_batch_get_snapshots: bad trajectory index");
+    const TrajDev &T = b->lay.traj[t];
+    if (count) *count = b->nsnap;
+    for (int k = 0; k < b->nsnap; ++k) {
+      if (u) NLS_HIP(hipMemcpyAsync(u + (size_t)k * T.N, b->d_snapU + (size_t)k * b->lay.cells + T.off,
+                                    sizeof(double) * T.N, hipMemcpyDeviceToHost, b->s));
+      if (v) NLS_HIP(hipMemcpyAsync(v + (size_t)k * T.N, b->d_snapV + (size_t)k * b->lay.cells + T.off,
+                                    sizeof(double) * T.N, hipMemcpyDeviceToHost, b->s));
+    }
+    NLS_HIP(hipStreamSynchronize(b->s));
+  });
+}
+
+int nls_batch_get_state(nls_batch *b, int t, double *u, double *v) {
+  return guard([&] {
+    NLS_REQUIRE(b && t >= 0 && t < b->ntraj, "nls_batch_get_state: bad trajectory index");
+    const TrajDev &T = b->lay.traj[t];
+    const int cur = (int)(b->step % 3), prev = (cur + 2) % 3;
+    NLS_HIP(hipMemcpyAsync(u, b->d_state[cur] + T.off, sizeof(double) * T.N, hipMemcpyDeviceToHost, b->s));
+    if (v) {
+      std::vector<double> up(T.N);
+      NLS_HIP(hipMemcpyAsync(up.data(), b->d_state[prev] + T.off, sizeof(double) * T.N,
+                             hipMemcpyDeviceToHost, b->s));
+      NLS_HIP(hipStreamSynchronize(b->s));
+      for (int64_t i = 0; i < T.N; ++i) v[i] = (u[i] - up[i]) / b->p.dt;
+    }
+    NLS_HIP(hipStreamSynchronize(b->s));
+  });
+}
+
+int nls_batch_info(nls_batch *b, int *krylov_dim, int64_t *cells, size_t *device_bytes) {
+  return guard([&] {
+    NLS_REQUIRE(b, "null batch");
+    if (krylov_dim) *krylov_dim = b->m;
+    if (cells) *cells = b->total_cells();
+    if (device_bytes) *device_bytes = b->bytes;
+  });
+}
+
+int nls_batch_set_profiling(nls_batch *b, int on) {
+  return guard([&] {
+    NLS_REQUIRE(b, "null batch");
+    b->profiling = on != 0;
+  });
+}
+
+int nls_batch_pass_timing(nls_batch *b, double *avg_ms, int *launches, double *bytes_per_launch) {
+  return guard([&] {
+    NLS_REQUIRE(b, "null batch");
+    if (avg_ms) *avg_ms = b->pass_launches ? b->pass_ms / b->pass_launches : 0.0;
+    if (launches) *launches = b->pass_launches;
+    if (bytes_per_launch) *bytes_per_launch = b->pass_bytes();
+  });
+}
+
+} // extern "C"

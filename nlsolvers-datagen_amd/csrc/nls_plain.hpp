@@ -1,0 +1,52 @@
+// Internal declarations shared by the plain (FP64 CSR) path and the fused path.
+#pragma once
+
+#include "nls_common.hpp"
+
+#include <memory>
+
+namespace nls {
+
+struct c128 {
+  double re, im;
+};
+struct c64 {
+  float re, im;
+};
+
+// Device copy of an int32 CSR operator with FP64 values.
+struct CsrDev {
+  int64_t n = 0, nnz = 0;
+  int32_t *rp = nullptr, *ci = nullptr;
+  double *val = nullptr;
+  CsrDev(int64_t n, int64_t nnz, const int32_t *rp_h, const int32_t *ci_h, const double *val_h);
+  ~CsrDev();
+  CsrDev(const CsrDev &) = delete;
+  CsrDev &operator=(const CsrDev &) = delete;
+};
+
+// Assembles the full-array operator on the host and uploads it.
+std::unique_ptr<CsrDev> build_operator(int dim, int n, double L, const double *c, int negate,
+                                       double shift, int q6);
+
+// Lanczos workspace of the plain path (KrylovInfo, device/include/lanczos.hpp:23-31,
+// plus the matrix-function buffers of matfunc_real.hpp:135-159) for S = double or c128.
+template <class S> struct PlainKrylov {
+  int64_t n;
+  int m, G;
+  hipStream_t stream;
+  S *V = nullptr, *w = nullptr, *T = nullptr, *h = nullptr, *y = nullptr, *scal = nullptr;
+  double *part = nullptr, *dscal = nullptr;
+  int *status = nullptr;
+  PlainKrylov(int64_t n, int m, hipStream_t s);
+  ~PlainKrylov();
+  void lanczos(const CsrDev &A, const S *in);
+  // out = beta0 V f(T) e1 for f(kind; t_re + i t_im)
+  void apply(const CsrDev &A, S *out, const S *in, double t_re, double t_im, int kind);
+  void check_status();
+};
+
+void launch_bc_f64(double *u, int n, int dim, hipStream_t s);
+void launch_bc_c128(c128 *u, int n, int dim, hipStream_t s);
+
+} // namespace nls

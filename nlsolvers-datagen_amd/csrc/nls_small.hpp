@@ -1,0 +1,157 @@
+// FP64 small-matrix stage of the Krylov matrix function: eigen-decomposition of
+// the m x m Lanczos tridiagonal T and y = beta0 * Q f(Lambda) Q^T e1.
+//
+// Replaces cusolverDnDsyevd/Zheevd + the f(lambda) Thrust lambdas + two GEMMs and
+// a GEMV of the reference (device/include/matfunc_real.hpp:190-313,385-392;
+// device/include/matfunc_complex.hpp:254-375,382-390).
+//
+// One 64-lane wavefront solves one problem: implicit QL with a Wilkinson-type
+// shift (Golub & Van Loan sec. 8.3) on the symmetric tridiagonal; lane k owns row
+// k of the eigenvector matrix Z (in LDS) and applies every Givens rotation to it,
+// lane 0 owns the diagonal/subdiagonal updates.  A Hermitian T with complex
+// subdiagonal t_i (the reference's complex Lanczos) is first reduced to a real
+// symmetric S = D^H T D with D = diag(ph), ph_0 = 1, ph_{i+1} = ph_i t_i/|t_i|,
+// so f(T) e1 = D f(S) e1.
+#pragma once
+
+#include "nls_common.hpp"
+
+namespace nls {
+
+constexpr int kMaxKrylov = 32;
+
+enum FuncKind : int {
+  F_COS_SQRT = 0,   // cos(t sqrt|l|)                (matfunc_real.hpp:221-228)
+  F_SINC2_SQRT = 1, // (sin x / x)^2, x = t sqrt|l|  (matfunc_real.hpp:212-219)
+  F_ID_SQRT = 2,    // t sqrt|l|                     (matfunc_real.hpp:230-237)
+  F_SINC2_HALF = 3, // (sin(x/2)/(x/2))^2: sEWI filter for Klein-Gordon
+  F_COSM1_SQRT = 4, // cos(x) - 1 = -2 sin^2(x/2)    (increment form used by the fused path)
+  F_EXP = 8,        // exp(dt l), dt complex         (matfunc_complex.hpp:281-287)
+  F_SINC = 9,       // sin(dt l)/(dt l), 1 if |.| < 1e-8  (matfunc_complex.hpp:293-300)
+};
+
+struct cplx {
+  double re, im;
+};
+
+__device__ inline cplx eval_f(int kind, double t_re, double t_im, double lam) {
+  if (kind < 8) {
+    const double x = t_re * sqrt(fabs(lam));
+    double f;
+    switch (kind) {
+    case F_COS_SQRT: f = cos(x); break;
+    case F_SINC2_SQRT: {
+      if (fabs(x) < 1e-8) f = 1.0;
+      else { double q = sin(x) / x; f = q * q; }
+      break;
+    }
+    case F_ID_SQRT: f = x; break;
+    case F_SINC2_HALF: {
+      const double y = 0.5 * x;
+      if (fabs(y) < 1e-8) f = 1.0;
+      else { double q = sin(y) / y; f = q * q; }
+      break;
+    }
+    default: { double s = sin(0.5 * x); f = -2.0 * s * s; break; }
+    }
+    return {f, 0.0};
+  }
+  const double a = t_re * lam, b = t_im * lam;
+  if (kind == F_EXP) {
+    const double ea = exp(a);
+    return {ea * cos(b), ea * sin(b)};
+  }
+  // sin(z)/z for z = a + i b
+  if (sqrt(a * a + b * b) < 1e-8) return {1.0, 0.0};
+  const double sr = sin(a) * cosh(b), si = cos(a) * sinh(b);
+  const double den = a * a + b * b;
+  return {(sr * a + si * b) / den, (si * a - sr * b) / den};
+}
+
+// Shared-memory workspace of one problem.
+struct SmallWS {
+  double d[kMaxKrylov], e[kMaxKrylov];
+  double Z[kMaxKrylov][kMaxKrylov + 1];
+  double phr[kMaxKrylov], phi[kMaxKrylov];
+  int status;
+};
+
+// QL on ws.d/ws.e (size m) with eigenvectors accumulated in ws.Z.  Must be
+// called by all 64 lanes of a single-wavefront workgroup.  Returns 0 on success.
+__device__ inline int small_ql(int m, SmallWS &ws) {
+  const int lane = threadIdx.x;
+  for (int k = lane; k < m * m; k += kWave) ws.Z[k / m][k % m] = (k / m == k % m) ? 1.0 : 0.0;
+  if (lane == 0) ws.e[m - 1] = 0.0;
+  __syncthreads();
+  for (int l = 0; l < m; ++l) {
+    int iter = 0;
+    for (;;) {
+      int mm;
+      for (mm = l; mm < m - 1; ++mm) {
+        const double dd = fabs(ws.d[mm]) + fabs(ws.d[mm + 1]);
+        if (fabs(ws.e[mm]) <= 2.220446049250313e-16 * dd) break;
+      }
+      if (mm == l) break;
+      if (++iter > 60) return 1;
+      double g = (ws.d[l + 1] - ws.d[l]) / (2.0 * ws.e[l]);
+      double r = hypot(g, 1.0);
+      g = ws.d[mm] - ws.d[l] + ws.e[l] / (g + (g >= 0.0 ? r : -r));
+      double s = 1.0, c = 1.0, p = 0.0;
+      bool deflated = false;
+      for (int i = mm - 1; i >= l; --i) {
+        const double f = s * ws.e[i], b = c * ws.e[i];
+        r = hypot(f, g);
+        __syncthreads();
+        if (lane == 0) ws.e[i + 1] = r;
+        if (r == 0.0) {
+          if (lane == 0) { ws.d[i + 1] -= p; ws.e[mm] = 0.0; }
+          __syncthreads();
+          deflated = true;
+          break;
+        }
+        const double ir = 1.0 / r;
+        s = f * ir;
+        c = g * ir;
+        g = ws.d[i + 1] - p;
+        r = (ws.d[i] - g) * s + 2.0 * c * b;
+        p = s * r;
+        const double dnew = g + p;
+        g = c * r - b;
+        if (lane < m) {
+          const double zf = ws.Z[lane][i + 1], zi = ws.Z[lane][i];
+          ws.Z[lane][i + 1] = s * zi + c * zf;
+          ws.Z[lane][i] = c * zi - s * zf;
+        }
+        __syncthreads();
+        if (lane == 0) ws.d[i + 1] = dnew;
+        __syncthreads();
+      }
+      if (deflated) continue;
+      __syncthreads();
+      if (lane == 0) { ws.d[l] -= p; ws.e[l] = g; ws.e[mm] = 0.0; }
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  return 0;
+}
+
+// After small_ql: y_k = beta0 * ph_k * sum_i Z[k][i] f(l_i) Z[0][i], lanes k < m.
+__device__ inline void small_apply_f(int m, SmallWS &ws, int kind, double t_re, double t_im,
+                                     double beta0, double *yr, double *yi) {
+  const int lane = threadIdx.x;
+  if (lane < m) {
+    double ar = 0.0, ai = 0.0;
+    for (int i = 0; i < m; ++i) {
+      const cplx f = eval_f(kind, t_re, t_im, ws.d[i]);
+      const double w = ws.Z[lane][i] * ws.Z[0][i];
+      ar += w * f.re;
+      ai += w * f.im;
+    }
+    const double pr = ws.phr[lane], pi = ws.phi[lane];
+    yr[lane] = beta0 * (pr * ar - pi * ai);
+    if (yi) yi[lane] = beta0 * (pr * ai + pi * ar);
+  }
+}
+
+} // namespace nls

@@ -1,0 +1,904 @@
+/*
+ * nls_oracle.c -- FP64 CPU restatement of the nlsolvers-datagen time steppers.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP library
+ * (libnls.so) is compared against.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it; the product never links or calls it.
+ *
+ * Every routine names the reference file:line whose semantics it restates
+ * (paths relative to the reference repository root).  Reference quirks that
+ * decide parity are reproduced on purpose and flagged "Qn":
+ *   Q1  Lanczos runs j = 0..m-2 only; T[m-1,m-1] stays 0     (device/include/lanczos.hpp:234-263)
+ *   Q2  lower subdiagonal of T holds the recomputed <A v_j, v_{j-1}>,
+ *       T[m-1,m-2] holds beta_{m-2}; the eigensolver reads LOWER
+ *                                                            (lanczos.hpp:237-255, matfunc_real.hpp:385-392)
+ *   Q3  one classical Gram-Schmidt pass per iteration        (lanczos.hpp:249-250)
+ *   Q4  Gautschi uses cos(t*sqrt|l|) and sinc^2(t*sqrt|l|)   (kg_single.cuh:101-126, matfunc_real.hpp:212-238)
+ *   Q6  3D "y" couplings cross plane boundaries               (common/include/laplacians.hpp:192-198)
+ *   Q7  operator spans the full array incl. the ghost ring; Neumann copy after each step
+ *   Q8  nt-1 steps, snapshot every nt/ns steps               (device/drivers/kg_driver_dev_2d.cpp:144-153)
+ *   Q9  u_{-1} = u0 - dt*v0, v = (u+ - u)/dt                 (device/include/kg_dev.hpp:55-58)
+ *   Q14 zero input vector: the reference divides by 0; here the result is 0 (guard)
+ *
+ * Build-defined extensions (no reference counterpart, used by the north-star
+ * configurations): a spectral shift mu2 (operator A = -L + mu2*I), the sEWI
+ * filter sinc^2(t*sqrt(l)/2), a Q6-free 3D operator, Philox4x32-10 noise with a
+ * fixed-operation Box-Muller transform, and a damped stochastic phi^4 stepper.
+ *
+ * Compile with -ffp-contract=off so that a*b+c is never fused behind our back
+ * (the noise transform relies on a fixed sequence of IEEE operations).
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_API __attribute__((visibility("default")))
+
+/* ------------------------------------------------------------------------- */
+/* Operator assembly                                                          */
+/* ------------------------------------------------------------------------- */
+
+/* nnz of the 2D full-array operator: n^2 diagonal + 4 n (n-1) couplings. */
+ORC_API int64_t orc_lap2d_nnz(int n) { return (int64_t)n * n + 4LL * n * (n - 1); }
+
+/* nnz of the 3D operator.  q6 != 0 keeps the reference's cross-plane "y"
+ * couplings (laplacians.hpp:192-198): n(n-1) extra pairs. */
+ORC_API int64_t orc_lap3d_nnz(int n, int q6) {
+  int64_t N = (int64_t)n * n * n;
+  int64_t x = (int64_t)n * n * (n - 1);          /* pairs along the fast axis */
+  int64_t y = q6 ? (N - n) : (int64_t)n * n * (n - 1);
+  int64_t z = N - (int64_t)n * n;
+  return N + 2 * (x + y + z);
+}
+
+/*
+ * 2D anisotropic no-flux operator, common/include/laplacians.hpp:54-103.
+ * n is the FULL side (the reference calls the builder with n-2 and adds 2 back,
+ * kg_driver_dev_2d.cpp:105-108).  Cell (i,j) has flat index i*n+j.
+ *   w(a,b)   = (c[a] + c[b]) / 2.0                     (:77, :86)
+ *   dsum[r]  = ((w_left + w_right) + w_up) + w_down    (loop order :75-91)
+ *   L[r][r]  = -dsum[r];  every value then times 1.0/(dx*dy)  (:93-100)
+ * negate != 0 returns -L (the Gautschi drivers, kg_driver_dev_2d.cpp:105);
+ * shift is added to the diagonal afterwards (build-defined mass term mu2).
+ * Output is CSR with ascending columns (Eigen's compressed storage of a
+ * symmetric matrix, passed as CSR by the drivers, kg_driver_dev_2d.cpp:132-137).
+ */
+ORC_API int orc_build_lap2d(int n, double dx, double dy, const double *c, int negate,
+                            double shift, int32_t *row_ptr, int32_t *col, double *val) {
+  const int64_t N = (int64_t)n * n;
+  const double s = 1.0 / (dx * dy);
+  int64_t k = 0;
+  for (int64_t r = 0; r < N; ++r) {
+    const int64_t i = r / n, j = r % n;
+    double wl = 0, wr = 0, wu = 0, wd = 0;
+    int hl = j > 0, hr = j < n - 1, hu = i > 0, hd = i < n - 1;
+    if (hl) wl = (c[r - 1] + c[r]) / 2.0;
+    if (hr) wr = (c[r] + c[r + 1]) / 2.0;
+    if (hu) wu = (c[r - n] + c[r]) / 2.0;
+    if (hd) wd = (c[r] + c[r + n]) / 2.0;
+    double dsum = 0.0;
+    if (hl) dsum += wl;
+    if (hr) dsum += wr;
+    if (hu) dsum += wu;
+    if (hd) dsum += wd;
+    const double sg = negate ? -1.0 : 1.0;
+    row_ptr[r] = (int32_t)k;
+    if (hu) { col[k] = (int32_t)(r - n); val[k++] = sg * (wu * s); }
+    if (hl) { col[k] = (int32_t)(r - 1); val[k++] = sg * (wl * s); }
+    col[k] = (int32_t)r; val[k++] = sg * (-dsum * s) + shift;
+    if (hr) { col[k] = (int32_t)(r + 1); val[k++] = sg * (wr * s); }
+    if (hd) { col[k] = (int32_t)(r + n); val[k++] = sg * (wd * s); }
+  }
+  row_ptr[N] = (int32_t)k;
+  return 0;
+}
+
+/*
+ * 3D anisotropic operator, laplacians.hpp:158-218.  Flat index ((k*n)+j)*n+i.
+ * Pairs: fast axis (r, r+1) when (r+1)%n != 0 (:182-190); "y" (r, r+n) for every
+ * r < N-n (:192-198, Q6: crosses plane boundaries when q6 != 0); z (r, r+n^2)
+ * (:200-206).  Diagonal accumulation order x-, x+, y-, y+, z-, z+.  Scale
+ * 1/(dx*dx) on every axis (:215).
+ */
+ORC_API int orc_build_lap3d(int n, double dx, const double *c, int negate, double shift,
+                            int q6, int32_t *row_ptr, int32_t *col, double *val) {
+  const int64_t n2 = (int64_t)n * n, N = n2 * n;
+  const double s = 1.0 / (dx * dx);
+  const double sg = negate ? -1.0 : 1.0;
+  int64_t k = 0;
+  for (int64_t r = 0; r < N; ++r) {
+    const int64_t i = r % n, jplane = r % n2;
+    int hxm = i > 0, hxp = i < n - 1;
+    int hym = r >= n, hyp = r + n < N;
+    if (!q6) { hym = jplane >= n; hyp = jplane < n2 - n; }
+    int hzm = r >= n2, hzp = r + n2 < N;
+    double wxm = 0, wxp = 0, wym = 0, wyp = 0, wzm = 0, wzp = 0;
+    if (hxm) wxm = (c[r - 1] + c[r]) / 2.0;
+    if (hxp) wxp = (c[r] + c[r + 1]) / 2.0;
+    if (hym) wym = (c[r - n] + c[r]) / 2.0;
+    if (hyp) wyp = (c[r] + c[r + n]) / 2.0;
+    if (hzm) wzm = (c[r - n2] + c[r]) / 2.0;
+    if (hzp) wzp = (c[r] + c[r + n2]) / 2.0;
+    double dsum = 0.0;
+    if (hxm) dsum += wxm;
+    if (hxp) dsum += wxp;
+    if (hym) dsum += wym;
+    if (hyp) dsum += wyp;
+    if (hzm) dsum += wzm;
+    if (hzp) dsum += wzp;
+    row_ptr[r] = (int32_t)k;
+    if (hzm) { col[k] = (int32_t)(r - n2); val[k++] = sg * (wzm * s); }
+    if (hym) { col[k] = (int32_t)(r - n); val[k++] = sg * (wym * s); }
+    if (hxm) { col[k] = (int32_t)(r - 1); val[k++] = sg * (wxm * s); }
+    col[k] = (int32_t)r; val[k++] = sg * (-dsum * s) + shift;
+    if (hxp) { col[k] = (int32_t)(r + 1); val[k++] = sg * (wxp * s); }
+    if (hyp) { col[k] = (int32_t)(r + n); val[k++] = sg * (wyp * s); }
+    if (hzp) { col[k] = (int32_t)(r + n2); val[k++] = sg * (wzp * s); }
+  }
+  row_ptr[N] = (int32_t)k;
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* CSR operator container and SpMV (device/include/spmv.hpp:65-72: y = 1*A*x)  */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+  int64_t N;
+  const int32_t *rp, *ci;
+  const double *v;
+} orc_csr;
+
+ORC_API void orc_csr_spmv(int64_t N, const int32_t *rp, const int32_t *ci, const double *v,
+                          const double *x, double *y) {
+  for (int64_t r = 0; r < N; ++r) {
+    double acc = 0.0;
+    for (int32_t k = rp[r]; k < rp[r + 1]; ++k) acc += v[k] * x[ci[k]];
+    y[r] = acc;
+  }
+}
+
+ORC_API void orc_csr_spmv_c(int64_t N, const int32_t *rp, const int32_t *ci, const double *v,
+                            const double complex *x, double complex *y) {
+  for (int64_t r = 0; r < N; ++r) {
+    double ar = 0.0, ai = 0.0;
+    for (int32_t k = rp[r]; k < rp[r + 1]; ++k) {
+      ar += v[k] * creal(x[ci[k]]);
+      ai += v[k] * cimag(x[ci[k]]);
+    }
+    y[r] = ar + ai * I;
+  }
+}
+
+static double dot_r(int64_t n, const double *a, const double *b) {
+  double s = 0.0;
+  for (int64_t i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Symmetric tridiagonal eigensolver: implicit QL with Wilkinson-type shift    */
+/* (Golub & Van Loan, "Matrix Computations", sec. 8.3; Wilkinson & Reinsch).    */
+/* Replaces cusolverDnDsyevd on the m x m Lanczos matrix (matfunc_real.hpp:388).*/
+/* d[0..m-1] diagonal (out: eigenvalues), e[0..m-2] subdiagonal (destroyed),   */
+/* Z (m x m row-major, out: Z[k*m+i] = component k of eigenvector i).          */
+/* ------------------------------------------------------------------------- */
+ORC_API int orc_tridiag_eig(int m, double *d, const double *e_in, double *Z) {
+  double e[64];
+  if (m <= 0 || m > 64) return -1;
+  for (int i = 0; i < m - 1; ++i) e[i] = e_in[i];
+  e[m - 1] = 0.0;
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j) Z[i * m + j] = (i == j) ? 1.0 : 0.0;
+  for (int l = 0; l < m; ++l) {
+    int iter = 0;
+    for (;;) {
+      int mm;
+      for (mm = l; mm < m - 1; ++mm) {
+        double dd = fabs(d[mm]) + fabs(d[mm + 1]);
+        if (fabs(e[mm]) <= 2.220446049250313e-16 * dd) break;
+      }
+      if (mm == l) break;
+      if (++iter > 60) return -2;
+      /* shift: eigenvalue of the leading 2x2 block closer to d[l] */
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = hypot(g, 1.0);
+      g = d[mm] - d[l] + e[l] / (g + (g >= 0 ? r : -r));
+      double s = 1.0, c = 1.0, p = 0.0;
+      int i;
+      int deflated = 0;
+      for (i = mm - 1; i >= l; --i) {
+        double f = s * e[i], b = c * e[i];
+        r = hypot(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) { d[i + 1] -= p; e[mm] = 0.0; deflated = 1; break; }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        p = s * r;
+        d[i + 1] = g + p;
+        g = c * r - b;
+        for (int k = 0; k < m; ++k) {
+          double zf = Z[k * m + i + 1];
+          Z[k * m + i + 1] = s * Z[k * m + i] + c * zf;
+          Z[k * m + i] = c * Z[k * m + i] - s * zf;
+        }
+      }
+      if (deflated) continue;
+      d[l] -= p;
+      e[l] = g;
+      e[mm] = 0.0;
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Real Lanczos, reference variant (device/include/lanczos.hpp:208-267)        */
+/* V column-major n x m (V[j*n+i]); T column-major m x m.                       */
+/* variant 0 = reference (Q1, Q2, Q3); variant 1 = textbook (no CGS, symmetric  */
+/* T from beta, alpha_{m-1} computed).                                          */
+/* Returns beta0 = ||x|| (lanczos.hpp:220).                                     */
+/* ------------------------------------------------------------------------- */
+ORC_API double orc_lanczos_real(const orc_csr *A, const double *x, int m, int variant,
+                                double *V, double *T) {
+  const int64_t n = A->N;
+  double *w = (double *)malloc(sizeof(double) * n);
+  double h[64];
+  memset(T, 0, sizeof(double) * m * m);
+  double beta = sqrt(dot_r(n, x, x));
+  if (beta == 0.0) { /* Q14 guard */
+    memset(V, 0, sizeof(double) * n * m);
+    free(w);
+    return 0.0;
+  }
+  const double sc = 1.0 / beta;
+  for (int64_t i = 0; i < n; ++i) V[i] = x[i] * sc;
+  double beta_prev = 0.0;
+  for (int j = 0; j < m - 1; ++j) {
+    double *vj = V + (int64_t)j * n;
+    orc_csr_spmv(n, A->rp, A->ci, A->v, vj, w);
+    if (j > 0) {
+      double *vp = V + (int64_t)(j - 1) * n;
+      double hd = dot_r(n, w, vp);
+      if (variant == 0) {
+        T[m * (j - 1) + j] = hd; /* Q2: lower subdiagonal overwritten */
+        for (int64_t i = 0; i < n; ++i) w[i] += -hd * vp[i];
+      } else {
+        for (int64_t i = 0; i < n; ++i) w[i] += -beta_prev * vp[i];
+      }
+    }
+    double a = dot_r(n, w, vj);
+    T[m * j + j] = a;
+    for (int64_t i = 0; i < n; ++i) w[i] += -a * vj[i];
+    if (variant == 0) { /* Q3: h = V^T w ; w -= V h over j+1 columns */
+      for (int k = 0; k <= j; ++k) h[k] = dot_r(n, V + (int64_t)k * n, w);
+      for (int64_t i = 0; i < n; ++i) {
+        double acc = 0.0;
+        for (int k = 0; k <= j; ++k) acc += V[(int64_t)k * n + i] * h[k];
+        w[i] -= acc;
+      }
+    }
+    double b = sqrt(dot_r(n, w, w));
+    T[m * j + j + 1] = b;
+    T[m * (j + 1) + j] = b;
+    if (b > 0.0) {
+      const double ib = 1.0 / b;
+      for (int64_t i = 0; i < n; ++i) w[i] *= ib;
+    }
+    memcpy(V + (int64_t)(j + 1) * n, w, sizeof(double) * n);
+    beta_prev = b;
+  }
+  if (variant == 1) { /* textbook: alpha_{m-1} */
+    double *vl = V + (int64_t)(m - 1) * n;
+    orc_csr_spmv(n, A->rp, A->ci, A->v, vl, w);
+    T[m * (m - 1) + m - 1] = dot_r(n, w, vl);
+  }
+  free(w);
+  return beta;
+}
+
+/* matrix-function kinds (matfunc_real.hpp:102 plus build-defined ones) */
+enum { F_COS_SQRT = 0, F_SINC2_SQRT = 1, F_ID_SQRT = 2, F_SINC2_HALF = 3, F_COSM1_SQRT = 4 };
+
+static double real_f(int kind, double t, double lam) {
+  double x = t * sqrt(fabs(lam));
+  switch (kind) {
+  case F_COS_SQRT: return cos(x);                                          /* :221-228 */
+  case F_SINC2_SQRT: return (fabs(x) < 1e-8) ? 1.0 : pow(sin(x) / x, 2);   /* :212-219 */
+  case F_ID_SQRT: return x;                                                /* :230-237 */
+  case F_SINC2_HALF: { double y = 0.5 * x; return (fabs(y) < 1e-8) ? 1.0 : pow(sin(y) / y, 2); }
+  case F_COSM1_SQRT: { double y = sin(0.5 * x); return -2.0 * y * y; }
+  }
+  return 0.0;
+}
+
+/* y_k = sum_i Z[k,i] f(lambda_i) Z[0,i]  (= (Q f(L) Q^T e1)_k, matfunc_real.hpp:252-292) */
+ORC_API int orc_small_matfunc_real(int m, const double *Tcm, int kind, double t, double *y) {
+  double d[64], e[64], Z[64 * 64];
+  for (int i = 0; i < m; ++i) d[i] = Tcm[m * i + i];
+  for (int i = 0; i < m - 1; ++i) e[i] = Tcm[m * i + i + 1]; /* lower: row i+1, col i */
+  int rc = orc_tridiag_eig(m, d, e, Z);
+  if (rc) return rc;
+  for (int k = 0; k < m; ++k) {
+    double acc = 0.0;
+    for (int i = 0; i < m; ++i) acc += Z[k * m + i] * real_f(kind, t, d[i]) * Z[i];
+    y[k] = acc;
+  }
+  return 0;
+}
+
+/* result = beta0 * V * f(T) e1  (MatrixFunctionApplicatorReal::apply, matfunc_real.hpp:190-313).
+ * in may alias out (kg_single.cuh:111). */
+ORC_API int orc_matfunc_real(const orc_csr *A, double *out, const double *in, double t, int kind,
+                             int m, int variant) {
+  const int64_t n = A->N;
+  double *V = (double *)malloc(sizeof(double) * n * m);
+  double T[64 * 64], y[64];
+  double beta = orc_lanczos_real(A, in, m, variant, V, T);
+  int rc = 0;
+  if (beta == 0.0) {
+    memset(out, 0, sizeof(double) * n);
+  } else {
+    rc = orc_small_matfunc_real(m, T, kind, t, y);
+    for (int64_t i = 0; i < n; ++i) {
+      double acc = 0.0;
+      for (int k = 0; k < m; ++k) acc += V[(int64_t)k * n + i] * y[k];
+      out[i] = acc * beta;
+    }
+  }
+  free(V);
+  return rc;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Neumann ghost-ring copy, device/include/boundaries.cuh:9-81                  */
+/* ------------------------------------------------------------------------- */
+ORC_API void orc_bc2d(double *u, int n) { /* :9-19, index i*n+j */
+  for (int j = 1; j < n - 1; ++j) u[j] = u[n + j];
+  for (int j = 1; j < n - 1; ++j) u[(int64_t)(n - 1) * n + j] = u[(int64_t)(n - 2) * n + j];
+  for (int i = 0; i < n; ++i) u[(int64_t)i * n] = u[(int64_t)i * n + 1];
+  for (int i = 0; i < n; ++i) u[(int64_t)i * n + n - 1] = u[(int64_t)i * n + n - 2];
+}
+
+ORC_API void orc_bc2d_c(double complex *u, int n) {
+  for (int j = 1; j < n - 1; ++j) u[j] = u[n + j];
+  for (int j = 1; j < n - 1; ++j) u[(int64_t)(n - 1) * n + j] = u[(int64_t)(n - 2) * n + j];
+  for (int i = 0; i < n; ++i) u[(int64_t)i * n] = u[(int64_t)i * n + 1];
+  for (int i = 0; i < n; ++i) u[(int64_t)i * n + n - 1] = u[(int64_t)i * n + n - 2];
+}
+
+/* 3D: index (a*n + b)*n + k with k fastest (boundaries.cuh:23-81):
+ * slow-axis faces over interior (b,k); middle-axis faces over all a, interior k;
+ * fast-axis faces over all (a,b). */
+#define BC3D_BODY(U)                                                                       \
+  const int64_t n2 = (int64_t)n * n;                                                       \
+  for (int b = 1; b < n - 1; ++b)                                                          \
+    for (int k = 1; k < n - 1; ++k) {                                                      \
+      U[(int64_t)b * n + k] = U[n2 + (int64_t)b * n + k];                                  \
+      U[(n - 1) * n2 + (int64_t)b * n + k] = U[(n - 2) * n2 + (int64_t)b * n + k];         \
+    }                                                                                      \
+  for (int a = 0; a < n; ++a)                                                              \
+    for (int k = 1; k < n - 1; ++k) {                                                      \
+      U[a * n2 + k] = U[a * n2 + n + k];                                                   \
+      U[a * n2 + (int64_t)(n - 1) * n + k] = U[a * n2 + (int64_t)(n - 2) * n + k];         \
+    }                                                                                      \
+  for (int a = 0; a < n; ++a)                                                              \
+    for (int b = 0; b < n; ++b) {                                                          \
+      U[a * n2 + (int64_t)b * n] = U[a * n2 + (int64_t)b * n + 1];                         \
+      U[a * n2 + (int64_t)b * n + n - 1] = U[a * n2 + (int64_t)b * n + n - 2];             \
+    }
+
+ORC_API void orc_bc3d(double *u, int n) { BC3D_BODY(u) }
+ORC_API void orc_bc3d_c(double complex *u, int n) { BC3D_BODY(u) }
+
+/* ------------------------------------------------------------------------- */
+/* Klein-Gordon driver (kg_driver_dev_2d.cpp:51-158, kg_dev.hpp:32-131,         */
+/* kg_single.cuh:57-179).                                                       */
+/*  method 0: Gautschi-ref  u+ = 2cos(tW)u - u- + t^2 sinc^2(tW)(-m u^3)  (Q4)  */
+/*  method 1: Stormer-Verlet u+ = 2u - u- + t^2 (L u - m u^3), L = -A           */
+/*  method 2: sEWI (build-defined, Q18): filter sinc^2(tW/2)                     */
+/* Operator A = -L_c + mu2 I, assembled here from c (reference-exact values).    */
+/* out_u/out_v: ns x N snapshots.  Snapshot 0 = (u0, v0).                         */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  int dim, n;
+  double L, T;
+  int nt, ns, krylov, method, variant, q6;
+  double mu2;
+} orc_kge_cfg;
+
+static int64_t cells_of(int dim, int n) { return dim == 3 ? (int64_t)n * n * n : (int64_t)n * n; }
+
+static void build_op(int dim, int n, double L, int negate, double shift, int q6, const double *c,
+                     int32_t **rp, int32_t **ci, double **v, orc_csr *A) {
+  const int64_t N = cells_of(dim, n);
+  const double dx = 2.0 * L / (n - 1);
+  int64_t nnz = dim == 3 ? orc_lap3d_nnz(n, q6) : orc_lap2d_nnz(n);
+  *rp = (int32_t *)malloc(sizeof(int32_t) * (N + 1));
+  *ci = (int32_t *)malloc(sizeof(int32_t) * nnz);
+  *v = (double *)malloc(sizeof(double) * nnz);
+  if (dim == 3) orc_build_lap3d(n, dx, c, negate, shift, q6, *rp, *ci, *v);
+  else orc_build_lap2d(n, dx, dx, c, negate, shift, *rp, *ci, *v);
+  A->N = N; A->rp = *rp; A->ci = *ci; A->v = *v;
+}
+
+ORC_API int orc_kge_run(const orc_kge_cfg *cfg, const double *mfield, const double *c,
+                        const double *u0, const double *v0, double *out_u, double *out_v) {
+  const int n = cfg->n;
+  const int64_t N = cells_of(cfg->dim, n);
+  const double dt = cfg->T / cfg->nt;
+  const int freq = cfg->nt / cfg->ns;
+  int32_t *rp, *ci;
+  double *vals;
+  orc_csr A;
+  if (cfg->method == 1)
+    build_op(cfg->dim, n, cfg->L, 0, -cfg->mu2, cfg->q6, c, &rp, &ci, &vals, &A);
+  else
+    build_op(cfg->dim, n, cfg->L, 1, cfg->mu2, cfg->q6, c, &rp, &ci, &vals, &A);
+  double *u = (double *)malloc(sizeof(double) * N), *up = (double *)malloc(sizeof(double) * N);
+  double *v = (double *)malloc(sizeof(double) * N), *buf = (double *)malloc(sizeof(double) * N);
+  double *b2 = (double *)malloc(sizeof(double) * N), *b3 = (double *)malloc(sizeof(double) * N);
+  memcpy(u, u0, sizeof(double) * N);
+  memcpy(v, v0, sizeof(double) * N);
+  for (int64_t i = 0; i < N; ++i) up[i] = u0[i] - dt * v0[i]; /* Q9, kg_dev.hpp:55-58 */
+  memcpy(out_u, u, sizeof(double) * N);
+  memcpy(out_v, v, sizeof(double) * N);
+  int rc = 0;
+  for (int it = 1; it < cfg->nt && !rc; ++it) {
+    memcpy(buf, u, sizeof(double) * N);
+    if (cfg->method == 1) { /* kg_single.cuh:145-178 */
+      orc_csr_spmv(N, A.rp, A.ci, A.v, u, b2);
+      for (int64_t i = 0; i < N; ++i) b3[i] = -mfield[i] * u[i] * u[i] * u[i];
+      const double t2 = dt * dt;
+      for (int64_t i = 0; i < N; ++i) {
+        double x = 2.0 * u[i] - up[i];
+        x = x + t2 * b2[i];
+        u[i] = x + t2 * b3[i];
+      }
+    } else { /* kg_single.cuh:94-126 */
+      rc |= orc_matfunc_real(&A, b2, u, dt, F_COS_SQRT, cfg->krylov, cfg->variant);
+      for (int64_t i = 0; i < N; ++i) b2[i] = b2[i] * 2.0;
+      for (int64_t i = 0; i < N; ++i) b3[i] = -mfield[i] * u[i] * u[i] * u[i];
+      rc |= orc_matfunc_real(&A, b3, b3, dt,
+                             cfg->method == 2 ? F_SINC2_HALF : F_SINC2_SQRT, cfg->krylov,
+                             cfg->variant);
+      const double t2 = dt * dt;
+      for (int64_t i = 0; i < N; ++i) b3[i] = b3[i] * t2;
+      for (int64_t i = 0; i < N; ++i) u[i] = (b2[i] - up[i]) + b3[i];
+    }
+    memcpy(up, buf, sizeof(double) * N);
+    for (int64_t i = 0; i < N; ++i) v[i] = (u[i] - up[i]) / dt;
+    if (cfg->dim == 3) orc_bc3d(u, n); else orc_bc2d(u, n);
+    if (freq > 0 && it % freq == 0 && it / freq < cfg->ns) {
+      int64_t s = it / freq;
+      memcpy(out_u + s * N, u, sizeof(double) * N);
+      memcpy(out_v + s * N, v, sizeof(double) * N);
+    }
+  }
+  free(u); free(up); free(v); free(buf); free(b2); free(b3);
+  free(rp); free(ci); free(vals);
+  return rc;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Complex Lanczos (lanczos_complex.hpp:320-519) and matrix function            */
+/* (matfunc_complex.hpp:254-375).  The operator is real-valued (built from a    */
+/* real c), vectors complex.  cublasZdotc(x=w, y=v) = sum conj(w_i) v_i (Q15).  */
+/* The Hermitian tridiagonal T (lower triangle read, Q2) is reduced to a real   */
+/* symmetric S = D^H T D by a diagonal unitary D (d_0 = 1), so                  */
+/* f(T) e1 = D f(S) e1.                                                        */
+/* ------------------------------------------------------------------------- */
+static double complex zdotc(int64_t n, const double complex *x, const double complex *y) {
+  double re = 0.0, im = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    double complex p = conj(x[i]) * y[i];
+    re += creal(p);
+    im += cimag(p);
+  }
+  return re + im * I;
+}
+
+ORC_API double orc_lanczos_complex(const orc_csr *A, const double complex *x, int m, int variant,
+                                   double complex *V, double complex *T) {
+  const int64_t n = A->N;
+  double complex *w = (double complex *)malloc(sizeof(double complex) * n);
+  double complex h[64];
+  memset(T, 0, sizeof(double complex) * m * m);
+  double nr = 0.0;
+  for (int64_t i = 0; i < n; ++i) nr += creal(x[i]) * creal(x[i]) + cimag(x[i]) * cimag(x[i]);
+  double beta = sqrt(nr);
+  if (beta == 0.0) {
+    memset(V, 0, sizeof(double complex) * n * m);
+    free(w);
+    return 0.0;
+  }
+  for (int64_t i = 0; i < n; ++i) V[i] = x[i] * (1.0 / beta);
+  double beta_prev = 0.0;
+  for (int j = 0; j < m - 1; ++j) {
+    double complex *vj = V + (int64_t)j * n;
+    orc_csr_spmv_c(n, A->rp, A->ci, A->v, vj, w);
+    if (j > 0) {
+      double complex *vp = V + (int64_t)(j - 1) * n;
+      double complex hd = variant == 0 ? zdotc(n, w, vp) : beta_prev;
+      if (variant == 0) T[m * (j - 1) + j] = hd;
+      for (int64_t i = 0; i < n; ++i) w[i] += -hd * vp[i];
+    }
+    double complex a = zdotc(n, w, vj);
+    if (variant == 1) a = creal(zdotc(n, vj, w));
+    T[m * j + j] = a;
+    for (int64_t i = 0; i < n; ++i) w[i] += -a * vj[i];
+    if (variant == 0) { /* Zgemv C then N (lanczos_complex.hpp:466-472) */
+      for (int k = 0; k <= j; ++k) h[k] = zdotc(n, V + (int64_t)k * n, w);
+      for (int64_t i = 0; i < n; ++i) {
+        double complex acc = 0.0;
+        for (int k = 0; k <= j; ++k) acc += V[(int64_t)k * n + i] * h[k];
+        w[i] -= acc;
+      }
+    }
+    double nb = 0.0;
+    for (int64_t i = 0; i < n; ++i) nb += creal(w[i]) * creal(w[i]) + cimag(w[i]) * cimag(w[i]);
+    double b = sqrt(nb);
+    T[m * j + j + 1] = b;
+    T[m * (j + 1) + j] = b;
+    if (b > 0.0)
+      for (int64_t i = 0; i < n; ++i) w[i] *= (1.0 / b);
+    memcpy(V + (int64_t)(j + 1) * n, w, sizeof(double complex) * n);
+    beta_prev = b;
+  }
+  if (variant == 1) {
+    double complex *vl = V + (int64_t)(m - 1) * n;
+    orc_csr_spmv_c(n, A->rp, A->ci, A->v, vl, w);
+    T[m * (m - 1) + m - 1] = creal(zdotc(n, vl, w));
+  }
+  free(w);
+  return beta;
+}
+
+/* kind 0: exp(dt*lambda) ; kind 1: sin(dt*lambda)/(dt*lambda), 1 if |.| < 1e-8
+ * (matfunc_complex.hpp:281-300). dt = dt_re + i dt_im. */
+ORC_API int orc_small_matfunc_complex(int m, const double complex *Tcm, int kind, double dt_re,
+                                      double dt_im, double complex *y) {
+  double d[64], e[64], Z[64 * 64];
+  double complex ph[64];
+  ph[0] = 1.0;
+  for (int i = 0; i < m; ++i) d[i] = creal(Tcm[m * i + i]);
+  for (int i = 0; i < m - 1; ++i) {
+    double complex t = Tcm[m * i + i + 1]; /* lower entry T[i+1,i] */
+    double a = cabs(t);
+    e[i] = a;
+    ph[i + 1] = a > 0.0 ? ph[i] * (t / a) : ph[i];
+  }
+  int rc = orc_tridiag_eig(m, d, e, Z);
+  if (rc) return rc;
+  const double complex dt = dt_re + dt_im * I;
+  for (int k = 0; k < m; ++k) {
+    double complex acc = 0.0;
+    for (int i = 0; i < m; ++i) {
+      double complex f;
+      if (kind == 0) f = cexp(dt * d[i]);
+      else {
+        double complex val = dt * d[i];
+        f = cabs(val) < 1e-8 ? 1.0 : csin(val) / val;
+      }
+      acc += Z[k * m + i] * f * Z[i];
+    }
+    y[k] = ph[k] * acc;
+  }
+  return 0;
+}
+
+ORC_API int orc_matfunc_complex(const orc_csr *A, double complex *out, const double complex *in,
+                                double dt_re, double dt_im, int kind, int m, int variant) {
+  const int64_t n = A->N;
+  double complex *V = (double complex *)malloc(sizeof(double complex) * n * m);
+  double complex T[64 * 64], y[64];
+  double beta = orc_lanczos_complex(A, in, m, variant, V, T);
+  int rc = 0;
+  if (beta == 0.0) {
+    memset(out, 0, sizeof(double complex) * n);
+  } else {
+    rc = orc_small_matfunc_complex(m, T, kind, dt_re, dt_im, y);
+    for (int64_t i = 0; i < n; ++i) {
+      double complex acc = 0.0;
+      for (int k = 0; k < m; ++k) acc += V[(int64_t)k * n + i] * y[k];
+      out[i] = acc * beta;
+    }
+  }
+  free(V);
+  return rc;
+}
+
+/* ------------------------------------------------------------------------- */
+/* NLSE driver (nlse_cubic_driver_2d.cpp:43-113, nlse_dev.hpp:20-64,187-239,   */
+/* 324-335).  method 0 = Strang SS2, method 1 = SEWI (first step Strang, Q16).  */
+/* Operator L (not negated).  Snapshots are taken inside step(), i.e. BEFORE    */
+/* the ghost-ring copy, into consecutive slots; slot 0 = psi0.                  */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  int dim, n;
+  double L, T;
+  int nt, ns, krylov, method, variant, q6;
+} orc_nlse_cfg;
+
+static void nl_phase(int64_t N, double complex *out, const double complex *in, const double *mf,
+                     double complex tau_half) {
+  for (int64_t i = 0; i < N; ++i) { /* compute_density + apply_nonlinear_part */
+    double rho = mf[i] * (creal(in[i]) * creal(in[i]) + cimag(in[i]) * cimag(in[i]));
+    out[i] = in[i] * cexp(tau_half * rho);
+  }
+}
+
+ORC_API int orc_nlse_run(const orc_nlse_cfg *cfg, const double *mfield, const double *c,
+                         const double complex *u0, double complex *out) {
+  const int n = cfg->n;
+  const int64_t N = cells_of(cfg->dim, n);
+  const double dt = cfg->T / cfg->nt;
+  const int freq = cfg->nt / cfg->ns;
+  int32_t *rp, *ci;
+  double *vals;
+  orc_csr A;
+  build_op(cfg->dim, n, cfg->L, 0, 0.0, cfg->q6, c, &rp, &ci, &vals, &A);
+  double complex *u = malloc(sizeof(double complex) * N), *buf = malloc(sizeof(double complex) * N);
+  double complex *b2 = malloc(sizeof(double complex) * N), *b3 = malloc(sizeof(double complex) * N);
+  double complex *uprev = malloc(sizeof(double complex) * N);
+  memcpy(u, u0, sizeof(double complex) * N);
+  int slot = 0;
+  memcpy(out, u, sizeof(double complex) * N);
+  slot = 1;
+  const double complex tau = dt * I;
+  int rc = 0;
+  for (int it = 1; it < cfg->nt && !rc; ++it) {
+    int strang = (cfg->method == 0) || (it == 1);
+    if (cfg->method == 1 && it == 1) memcpy(uprev, u, sizeof(double complex) * N);
+    if (strang) { /* nlse_dev.hpp:187-197 */
+      nl_phase(N, buf, u, mfield, 0.5 * tau);
+      rc |= orc_matfunc_complex(&A, u, buf, creal(tau), cimag(tau), 0, cfg->krylov, cfg->variant);
+      nl_phase(N, u, u, mfield, 0.5 * tau);
+      if (it % freq == 0 && slot < cfg->ns) memcpy(out + (int64_t)(slot++) * N, u, sizeof(double complex) * N);
+    } else { /* nlse_dev.hpp:211-233 */
+      memcpy(buf, u, sizeof(double complex) * N);
+      for (int64_t i = 0; i < N; ++i) /* compute_B */
+        b2[i] = -mfield[i] * (creal(u[i]) * creal(u[i]) + cimag(u[i]) * cimag(u[i])) * u[i];
+      rc |= orc_matfunc_complex(&A, b3, b2, cimag(tau), 0.0, 1, cfg->krylov, cfg->variant);
+      rc |= orc_matfunc_complex(&A, b2, b3, creal(tau), cimag(tau), 0, cfg->krylov, cfg->variant);
+      rc |= orc_matfunc_complex(&A, b3, uprev, 2.0 * creal(tau), 2.0 * cimag(tau), 0, cfg->krylov,
+                                cfg->variant);
+      /* Q20: the reference calls apply_sewi(d_u, d_buf2, d_buf3, tau) whose body is
+       * out = exp_prev - 2 tau exp_psi_B, but passes buf2 = e^{tau L} sinc B as
+       * exp_prev and buf3 = e^{2 tau L} u_prev as exp_psi_B (nlse_dev.hpp:54-64,
+       * 219-225).  variant 0 reproduces that; variant 1 uses the intended
+       * psi+ = e^{2 tau L} psi- - 2 tau e^{tau L} sinc(tau L) B(psi). */
+      if (cfg->variant == 0)
+        for (int64_t i = 0; i < N; ++i) u[i] = b2[i] - 2.0 * tau * b3[i];
+      else
+        for (int64_t i = 0; i < N; ++i) u[i] = b3[i] - 2.0 * tau * b2[i];
+      memcpy(uprev, buf, sizeof(double complex) * N);
+      if (it > 1 && it % freq == 0 && slot < cfg->ns) memcpy(out + (int64_t)(slot++) * N, u, sizeof(double complex) * N);
+    }
+    if (cfg->dim == 3) orc_bc3d_c(u, n); else orc_bc2d_c(u, n);
+  }
+  free(u); free(buf); free(b2); free(b3); free(uprev);
+  free(rp); free(ci); free(vals);
+  return rc;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy */
+/* as 1, 2, 3", SC'11): multipliers 0xD2511F53, 0xCD9E8D57, Weyl increments     */
+/* 0x9E3779B9, 0xBB67AE85, ten rounds.                                          */
+/* ------------------------------------------------------------------------- */
+ORC_API void orc_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+  uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+  uint32_t k0 = key_in[0], k1 = key_in[1];
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    uint32_t n1 = (uint32_t)p1;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    uint32_t n3 = (uint32_t)p0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* ln(u) for u in (0,1]: u = 2^e * x with x in [sqrt(1/2), sqrt(2)),
+ * ln x = 2 atanh(s), s = (x-1)/(x+1), odd series to s^23.  Only +,-,*,/,fma. */
+ORC_API double orc_det_log(double u) {
+  uint64_t bits;
+  memcpy(&bits, &u, 8);
+  int e = (int)((bits >> 52) & 0x7ff) - 1023;
+  bits = (bits & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL;
+  double x;
+  memcpy(&x, &bits, 8);
+  if (x > 1.4142135623730951) { x = x * 0.5; e += 1; }
+  const double s = (x - 1.0) / (x + 1.0);
+  const double s2 = s * s;
+  double p = 1.0 / 23.0;
+  p = fma(p, s2, 1.0 / 21.0); p = fma(p, s2, 1.0 / 19.0); p = fma(p, s2, 1.0 / 17.0);
+  p = fma(p, s2, 1.0 / 15.0); p = fma(p, s2, 1.0 / 13.0); p = fma(p, s2, 1.0 / 11.0);
+  p = fma(p, s2, 1.0 / 9.0);  p = fma(p, s2, 1.0 / 7.0);  p = fma(p, s2, 1.0 / 5.0);
+  p = fma(p, s2, 1.0 / 3.0);
+  const double lx = 2.0 * fma(s * s2, p, s);
+  const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+  return fma((double)e, ln2_hi, fma((double)e, ln2_lo, lx));
+}
+
+/* (cos 2 pi w, sin 2 pi w) for w in [0,1): quadrant from 4w (exact), reduced
+ * angle phi in [0, pi/4], Taylor series to phi^17 / phi^18. */
+ORC_API void orc_det_sincos2pi(double w, double *cs, double *sn) {
+  const double t = 4.0 * w;
+  const int q = (int)t;
+  double r = t - (double)q;
+  int swap = 0;
+  if (r > 0.5) { r = 1.0 - r; swap = 1; }
+  const double phi = r * 1.5707963267948966;
+  const double p2 = phi * phi;
+  /* sin: phi (1 - p2/3! + p2^2/5! - ...), coefficients 1/(2k+1)! alternating */
+  double s_ = 1.0 / 355687428096000.0;                   /*  1/17! */
+  s_ = fma(s_, -p2, 1.0 / 1307674368000.0);              /*  1/15! */
+  s_ = fma(s_, -p2, 1.0 / 6227020800.0);                 /*  1/13! */
+  s_ = fma(s_, -p2, 1.0 / 39916800.0);                   /*  1/11! */
+  s_ = fma(s_, -p2, 1.0 / 362880.0);                     /*  1/9!  */
+  s_ = fma(s_, -p2, 1.0 / 5040.0);                       /*  1/7!  */
+  s_ = fma(s_, -p2, 1.0 / 120.0);                        /*  1/5!  */
+  s_ = fma(s_, -p2, 1.0 / 6.0);                          /*  1/3!  */
+  s_ = fma(s_, -p2, 1.0);
+  const double sphi = phi * s_;
+  double c_ = 1.0 / 6402373705728000.0;                  /*  1/18! */
+  c_ = fma(c_, -p2, 1.0 / 20922789888000.0);             /*  1/16! */
+  c_ = fma(c_, -p2, 1.0 / 87178291200.0);                /*  1/14! */
+  c_ = fma(c_, -p2, 1.0 / 479001600.0);                  /*  1/12! */
+  c_ = fma(c_, -p2, 1.0 / 3628800.0);                    /*  1/10! */
+  c_ = fma(c_, -p2, 1.0 / 40320.0);                      /*  1/8!  */
+  c_ = fma(c_, -p2, 1.0 / 720.0);                        /*  1/6!  */
+  c_ = fma(c_, -p2, 1.0 / 24.0);                         /*  1/4!  */
+  c_ = fma(c_, -p2, 1.0 / 2.0);                          /*  1/2!  */
+  c_ = fma(c_, -p2, 1.0);
+  double cr = swap ? sphi : c_, sr = swap ? c_ : sphi;   /* (cos, sin) of r*pi/2 */
+  switch (q & 3) {
+  case 0: *cs = cr; *sn = sr; break;
+  case 1: *cs = -sr; *sn = cr; break;
+  case 2: *cs = -cr; *sn = -sr; break;
+  default: *cs = sr; *sn = -cr; break;
+  }
+}
+
+/* Four standard normals for counter (group, step, stream, 0) and key = seed.
+ * Box-Muller on (x0,x1) and (x2,x3); uniforms u1 = (x + 0.5) 2^-32 in (0,1),
+ * u2 = x 2^-32 in [0,1).  Cell c uses group c/4, lane c%4. */
+ORC_API void orc_noise4(uint64_t seed, uint32_t stream, uint32_t step, uint32_t group, double z[4]) {
+  uint32_t ctr[4] = {group, step, stream, 0u};
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t x[4];
+  orc_philox4x32_10(ctr, key, x);
+  for (int h = 0; h < 2; ++h) {
+    const double u1 = ((double)x[2 * h] + 0.5) * 2.3283064365386963e-10;
+    const double u2 = (double)x[2 * h + 1] * 2.3283064365386963e-10;
+    const double r = sqrt(-2.0 * orc_det_log(u1));
+    double cs, sn;
+    orc_det_sincos2pi(u2, &cs, &sn);
+    z[2 * h] = r * cs;
+    z[2 * h + 1] = r * sn;
+  }
+}
+
+ORC_API void orc_noise_field(uint64_t seed, uint32_t stream, uint32_t step, int64_t N, double *z) {
+  for (int64_t g = 0; g * 4 < N; ++g) {
+    double q[4];
+    orc_noise4(seed, stream, step, (uint32_t)g, q);
+    for (int l = 0; l < 4 && g * 4 + l < N; ++l) z[g * 4 + l] = q[l];
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Stochastic phi^4 (stochastic_phi4.cuh:19-77, stochastic_phi4_dev.hpp:46-51)  */
+/* mode 0 (reference form, Q13 with Philox noise in place of time(nullptr)):    */
+/*   u+ = 2u - u- + t^2 L u + t^2 (-m (u - u^3 + s_eff z)),                    */
+/*   s_eff = 0.01 sqrt(2 dx) / t                                                */
+/* mode 1 (damped, build-defined):                                              */
+/*   (1 + g t/2) u+ = 2u - (1 - g t/2) u- + t^2 (L u + m (u - u^3))            */
+/*                    + sigma t^{3/2} dx^{-d/2} z                               */
+/* Noise for step it (1-based) uses counter step = it.                          */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  int dim, n;
+  double L, T;
+  int nt, ns, mode, q6;
+  double gamma, sigma;
+  uint64_t seed;
+  uint32_t stream;
+} orc_sp4_cfg;
+
+ORC_API int orc_sp4_run(const orc_sp4_cfg *cfg, const double *mfield, const double *c,
+                        const double *u0, const double *v0, double *out_u, double *out_v) {
+  const int n = cfg->n;
+  const int64_t N = cells_of(cfg->dim, n);
+  const double dt = cfg->T / cfg->nt;
+  const int freq = cfg->nt / cfg->ns;
+  const double dx = 2.0 * cfg->L / (n - 1);
+  int32_t *rp, *ci;
+  double *vals;
+  orc_csr A;
+  build_op(cfg->dim, n, cfg->L, 0, 0.0, cfg->q6, c, &rp, &ci, &vals, &A);
+  double *u = malloc(sizeof(double) * N), *up = malloc(sizeof(double) * N);
+  double *v = malloc(sizeof(double) * N), *buf = malloc(sizeof(double) * N);
+  double *Lu = malloc(sizeof(double) * N), *z = malloc(sizeof(double) * N);
+  memcpy(u, u0, sizeof(double) * N);
+  memcpy(v, v0, sizeof(double) * N);
+  for (int64_t i = 0; i < N; ++i) up[i] = u0[i] - dt * v0[i];
+  memcpy(out_u, u, sizeof(double) * N);
+  memcpy(out_v, v, sizeof(double) * N);
+  const double t2 = dt * dt;
+  const double s_eff = 0.01 * sqrt(2.0 * dx) / dt;
+  const double dxd = cfg->dim == 3 ? dx * sqrt(dx) : dx;
+  const double namp = cfg->sigma * dt * sqrt(dt) / dxd;
+  const double gp = 1.0 + 0.5 * cfg->gamma * dt, gm = 1.0 - 0.5 * cfg->gamma * dt;
+  for (int it = 1; it < cfg->nt; ++it) {
+    memcpy(buf, u, sizeof(double) * N);
+    orc_csr_spmv(N, A.rp, A.ci, A.v, u, Lu);
+    orc_noise_field(cfg->seed, cfg->stream, (uint32_t)it, N, z);
+    if (cfg->mode == 0) {
+      for (int64_t i = 0; i < N; ++i) {
+        double uu = u[i];
+        double nl = -mfield[i] * (uu - uu * uu * uu + s_eff * z[i]);
+        double x = 2.0 * uu - up[i];
+        x = x + t2 * Lu[i];
+        u[i] = x + t2 * nl;
+      }
+    } else {
+      for (int64_t i = 0; i < N; ++i) {
+        double uu = u[i];
+        double f = Lu[i] + mfield[i] * (uu - uu * uu * uu);
+        double x = 2.0 * uu - gm * up[i] + t2 * f + namp * z[i];
+        u[i] = x / gp;
+      }
+    }
+    memcpy(up, buf, sizeof(double) * N);
+    for (int64_t i = 0; i < N; ++i) v[i] = (u[i] - up[i]) / dt;
+    if (cfg->dim == 3) orc_bc3d(u, n); else orc_bc2d(u, n);
+    if (freq > 0 && it % freq == 0 && it / freq < cfg->ns) {
+      int64_t s = it / freq;
+      memcpy(out_u + s * N, u, sizeof(double) * N);
+      memcpy(out_v + s * N, v, sizeof(double) * N);
+    }
+  }
+  free(u); free(up); free(v); free(buf); free(Lu); free(z);
+  free(rp); free(ci); free(vals);
+  return 0;
+}
+
+/* Batch entry for the CPU baseline: runs independent KGE trajectories with one
+ * trajectory per OpenMP thread.  u0/v0/m/c/out are laid out trajectory-major. */
+ORC_API int orc_kge_run_batch(int ntraj, const orc_kge_cfg *cfg, const double *mfield,
+                              const double *c, const double *u0, const double *v0,
+                              double *out_u, double *out_v) {
+  const int64_t N = cells_of(cfg->dim, cfg->n);
+  int rc = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : rc)
+  for (int t = 0; t < ntraj; ++t)
+    rc |= orc_kge_run(cfg, mfield + t * N, c + t * N, u0 + t * N, v0 + t * N,
+                      out_u + (int64_t)t * cfg->ns * N, out_v + (int64_t)t * cfg->ns * N);
+  return rc;
+}
+
+ORC_API int orc_omp_threads(void) {
+  int n = 1;
+#pragma omp parallel
+  {
+#pragma omp single
+    {
+#ifdef _OPENMP
+      extern int omp_get_num_threads(void);
+      n = omp_get_num_threads();
+#endif
+    }
+  }
+  return n;
+}

@@ -1,0 +1,129 @@
+"""GPU parity of the fused batched FP32/FP64 Klein-Gordon path against the FP64
+oracle (reference Lanczos variant), plus determinism properties: a trajectory's
+result does not depend on its batch-mates, graphs replay the eager result."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# relative-L2 bar of the fused FP32 Krylov / FP64 state path vs the FP64 oracle
+TOL = 1e-5
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def make_case(n, seed, traj, L=10.0, ring=False):
+    from nls.samplers import gaussian_sum_2d, kink_ring_2d
+    rng = np.random.default_rng(seed * 100 + traj)
+    if ring:
+        u0, v0, _ = kink_ring_2d(n, L, seed, traj)
+    else:
+        u0, v0, _ = gaussian_sum_2d(n, L, seed, traj)
+    m = 0.8 + 0.4 * rng.random((n, n))
+    c = 0.8 + 0.4 * rng.random((n, n))
+    return m, c, u0, v0
+
+
+def test_fused_kge_uneven_batch_vs_oracle(orc):
+    from nls.batch import FusedBatch
+    ns_list = [48, 64, 33]
+    L, dt, nt, every = 10.0, 0.005, 201, 50
+    ns = 5
+    b = FusedBatch(ns_list, L, dt, krylov_dim=10, mu2=1.0, snapshot_every=every,
+                   max_snapshots=ns, cells_per_thread=8)
+    cases = [make_case(n, 2, t, ring=(t == 1)) for t, n in enumerate(ns_list)]
+    for t, (m, c, u0, v0) in enumerate(cases):
+        b.set_initial(t, m, c, u0, v0)
+    b.run(nt - 1)
+    for t, n in enumerate(ns_list):
+        m, c, u0, v0 = cases[t]
+        ur, vr = orc.kge_run(n, L, dt * nt, nt, ns, m, c, u0, v0, krylov=10, mu2=1.0)
+        u, v = b.snapshots(t)
+        assert u.shape[0] == ns
+        for k in range(ns):
+            assert rel_l2(u[k], ur[k]) < TOL, (t, k)
+        for k in range(1, ns):
+            assert rel_l2(v[k], vr[k]) < 1e-3, (t, k)  # first-order velocity, FP32 Krylov
+
+
+@pytest.mark.parametrize("method", ["gautschi", "sewi"])
+def test_fused_kge_3d_vs_oracle(orc, method):
+    from nls.batch import FusedBatch
+    from nls.samplers import bubbles_3d
+    n, L, dt, nt, ns = 20, 8.0, 0.004, 61, 3
+    for q6 in (True, False):
+        u0, v0 = bubbles_3d(n, L, 3, 1)
+        m = np.ones((n, n, n))
+        c = np.ones((n, n, n))
+        b = FusedBatch([n], L, dt, method=method, dim=3, krylov_dim=12, mu2=1.0, q6=q6,
+                       snapshot_every=30, max_snapshots=ns, cells_per_thread=8)
+        b.set_initial(0, m, c, u0, v0)
+        b.run(nt - 1)
+        u, _ = b.snapshots(0)
+        ur, _ = orc.kge_run(n, L, dt * nt, nt, ns, m, c, u0, v0, krylov=12, mu2=1.0, dim=3,
+                            q6=q6, method={"gautschi": 0, "sewi": 2}[method])
+        for k in range(ns):
+            assert rel_l2(u[k], ur[k]) < TOL, (q6, k)
+
+
+def test_fused_alone_equals_in_batch_bitwise():
+    from nls.batch import FusedBatch
+    L, dt, steps = 10.0, 0.005, 40
+    mix = [64, 40, 96, 64]
+    target = 2
+    cases = [make_case(n, 4, t) for t, n in enumerate(mix)]
+    b = FusedBatch(mix, L, dt, krylov_dim=8, mu2=1.0, cells_per_thread=8)
+    for t, cs in enumerate(cases):
+        b.set_initial(t, *cs)
+    b.run(steps)
+    u_mix, _ = b.state(target)
+    a = FusedBatch([mix[target]], L, dt, krylov_dim=8, mu2=1.0, cells_per_thread=8)
+    a.set_initial(0, *cases[target])
+    a.run(steps)
+    u_alone, _ = a.state(0)
+    assert np.array_equal(u_mix, u_alone)
+
+
+def test_fused_graph_replay_equals_eager():
+    from nls.batch import FusedBatch
+    L, dt, steps = 10.0, 0.005, 30
+    ns_list = [50, 70]
+    cases = [make_case(n, 5, t) for t, n in enumerate(ns_list)]
+    out = []
+    for graph in (True, False):
+        b = FusedBatch(ns_list, L, dt, krylov_dim=9, mu2=1.0, use_graph=graph, cells_per_thread=4)
+        for t, cs in enumerate(cases):
+            b.set_initial(t, *cs)
+        for _ in range(3):  # several replays with host synchronisation between them
+            b.run(steps // 3)
+        out.append([b.state(t)[0] for t in range(len(ns_list))])
+    for x, y in zip(*out):
+        assert np.array_equal(x, y)
+
+
+def test_fused_autotuned_tile_matches_oracle(orc):
+    from nls.batch import FusedBatch
+    n, L, dt, nt = 80, 10.0, 0.005, 41
+    m, c, u0, v0 = make_case(n, 6, 0)
+    b = FusedBatch([n], L, dt, krylov_dim=0, mu2=1.0, snapshot_every=40, max_snapshots=2)
+    b.set_initial(0, m, c, u0, v0)
+    b.run(nt - 1)
+    info = b.info()
+    assert 8 <= info["krylov_dim"] <= 16
+    u, _ = b.snapshots(0)
+    ur, _ = orc.kge_run(n, L, dt * nt, nt, 2, m, c, u0, v0, krylov=info["krylov_dim"], mu2=1.0)
+    assert rel_l2(u[1], ur[1]) < TOL
+
+
+def test_fused_zero_field_stays_zero():
+    """Q14 edge: a zero field (zero Krylov seed) must give 0, not NaN."""
+    from nls.batch import FusedBatch
+    n = 32
+    b = FusedBatch([n], 10.0, 0.005, krylov_dim=8, cells_per_thread=8)
+    z = np.zeros((n, n))
+    b.set_initial(0, np.ones((n, n)), np.ones((n, n)), z, z)
+    b.run(5)
+    u, _ = b.state(0)
+    assert np.all(u == 0.0)

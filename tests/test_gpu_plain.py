@@ -1,0 +1,205 @@
+"""GPU parity of the plain FP64 path (CSR SpMV + reference-granularity Lanczos)
+against the FP64 CPU oracle on the same seeded inputs."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def field(n, seed, L=10.0):
+    from nls.samplers import gaussian_sum_2d
+    u0, v0, _ = gaussian_sum_2d(n, L, seed, 0)
+    return u0, v0
+
+
+@pytest.mark.parametrize("n", [128, 300])
+def test_csr_spmv_f64_vs_oracle(orc, n):
+    from nls.device import DeviceArray, DeviceSpMV
+    rng = np.random.default_rng(n)
+    dx = 20.0 / (n - 1)
+    op = orc.laplacian_2d(n, dx, dx, 0.5 + rng.random(n * n), negate=True, shift=1.0)
+    A = DeviceSpMV(op.rp, op.ci, op.v)
+    x = rng.standard_normal(n * n)
+    dx_, dy_ = DeviceArray.from_host(x), DeviceArray(n * n, np.float64)
+    A.multiply(dx_, dy_)
+    y = dy_.to_host()
+    ref = op.matvec(x)
+    scale = np.abs(op.v).max() * np.abs(x).max()
+    assert np.abs(y - ref).max() <= 8e-16 * scale
+
+
+def test_csr_spmv_other_types(orc):
+    from nls.device import DeviceArray, DeviceSpMV
+    rng = np.random.default_rng(7)
+    n = 64
+    op = orc.laplacian_2d(n, 0.3, 0.3, 1 + rng.random(n * n))
+    A = op.dense()
+    for vt in (np.float32, np.complex128, np.complex64):
+        vals = op.v.astype(vt)
+        if np.iscomplexobj(vals):
+            vals = vals * np.exp(0.3j)  # genuinely complex values
+        S = DeviceSpMV(op.rp, op.ci, vals)
+        x = rng.standard_normal(n * n).astype(vt)
+        if np.iscomplexobj(x):
+            x = x + 1j * rng.standard_normal(n * n).astype(vt)
+        dxx, dyy = DeviceArray.from_host(x), DeviceArray(n * n, vt)
+        S.multiply(dxx, dyy)
+        y = dyy.to_host()
+        Ad = A.astype(np.complex128) * (np.exp(0.3j) if np.iscomplexobj(vals) else 1)
+        ref = Ad @ x.astype(np.complex128)
+        tol = 1e-5 if vt in (np.float32, np.complex64) else 1e-14
+        assert np.abs(y - ref).max() <= tol * np.abs(ref).max()
+
+
+def test_small_matfunc_gpu_vs_oracle(orc):
+    from nls.solvers import small_matfunc
+    rng = np.random.default_rng(11)
+    for m in (2, 3, 8, 10, 12, 16, 25, 32):
+        d = rng.random(m) * 100
+        e = rng.random(m - 1) * 20
+        if m > 4:
+            e[m // 3] = 0.0
+        T = np.diag(d) + np.diag(e, -1)  # lower triangle holds e
+        Tcm = np.ascontiguousarray(T.T)
+        for kind, t in ((orc.F_COS_SQRT, 0.05), (orc.F_SINC2_SQRT, 0.05), (orc.F_ID_SQRT, 0.05),
+                        (orc.F_SINC2_HALF, 0.1), (orc.F_COSM1_SQRT, 0.01)):
+            y = small_matfunc(d, e, kind, t).real
+            ref = orc.small_matfunc_real(Tcm, kind, t)
+            assert np.abs(y - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
+        y = small_matfunc(d, e, 8, 0.005j)
+        Tc = Tcm.astype(np.complex128)
+        ref = orc.small_matfunc_complex(Tc, 0, 0.005j)
+        assert np.abs(y - ref).max() <= 1e-13
+
+
+def test_matfunc_real_plain_vs_oracle(orc):
+    from nls.solvers import MatrixFunctionApplicatorReal
+    n = 64
+    dx = 20.0 / (n - 1)
+    rng = np.random.default_rng(3)
+    op = orc.laplacian_2d(n, dx, dx, 0.5 + rng.random(n * n), negate=True, shift=1.0)
+    u0, _ = field(n, 5)
+    x = u0.ravel() + 0.01 * rng.standard_normal(n * n)
+    mf = MatrixFunctionApplicatorReal(op.rp, op.ci, op.v, 10)
+    for kind in (0, 1, 2, 3, 4):
+        y = mf.apply(x, 0.01, kind)
+        ref = orc.matfunc_real(op, x, 0.01, kind, 10, variant=0)
+        assert rel_l2(y, ref) < 1e-12, kind
+    T, beta = mf.last_krylov()
+    _, _, Tref = orc.lanczos_real(op, x, 10, variant=0)
+    assert beta == pytest.approx(np.linalg.norm(x), rel=1e-14)
+    assert T[9, 9] == 0.0  # Q1 (column-major [col][row])
+    np.testing.assert_allclose(T, Tref, rtol=0, atol=1e-10 * np.abs(Tref).max())
+
+
+def test_matfunc_complex_plain_vs_oracle(orc):
+    from nls.solvers import MatrixFunctionApplicatorComplex
+    from nls.samplers import nlse_packets_2d
+    n = 48
+    dx = 20.0 / (n - 1)
+    op = orc.laplacian_2d(n, dx, dx, np.ones(n * n))
+    psi, _ = nlse_packets_2d(n, 10.0, 1, 0)
+    x = psi.ravel()
+    mf = MatrixFunctionApplicatorComplex(op.rp, op.ci, op.v, 12)
+    for dt, f, kind in ((0.005j, "exp", 0), (0.005, "sinc", 1), (0.01j, "exp", 0)):
+        y = mf.apply(x, dt, f)
+        ref = orc.matfunc_complex(op, x, dt, kind, 12, variant=0)
+        assert rel_l2(y, ref) < 1e-12
+
+
+def test_kge_gautschi_config1_vs_oracle(orc):
+    """Config 1: 2D KGE 128^2 on [-10,10]^2, Gautschi m=10, dt=0.01, nt=500,
+    snapshots every 50 steps (ns=10), Gaussians from seed 0; reference
+    equation (mu2 = 0, m = c = 1)."""
+    from nls.solvers import KGESolverDevice
+    n, L, T, nt, ns = 128, 10.0, 5.0, 500, 10
+    u0, v0 = field(n, 0)
+    m = np.ones((n, n))
+    c = np.ones((n, n))
+    s = KGESolverDevice(n, L, T, nt, ns, m, c, u0, v0, krylov_dim=10)
+    s.run()
+    u, v = s.transfer_snapshots()
+    ur, vr = orc.kge_run(n, L, T, nt, ns, m, c, u0, v0, krylov=10, method=0)
+    for k in range(ns):
+        assert rel_l2(u[k], ur[k]) < 1e-10, k
+        assert rel_l2(v[k], vr[k]) < 1e-8, k
+
+
+@pytest.mark.parametrize("method", ["sv", "sewi"])
+def test_kge_other_methods_vs_oracle(orc, method):
+    from nls.solvers import KGESolverDevice
+    n, L, T, nt, ns = 64, 10.0, 0.5, 100, 5
+    rng = np.random.default_rng(2)
+    u0, v0 = field(n, 3)
+    m = 0.5 + rng.random((n, n))
+    c = 0.5 + rng.random((n, n))
+    meth = {"sv": 1, "sewi": 2}[method]
+    s = KGESolverDevice(n, L, T, nt, ns, m, c, u0, v0, krylov_dim=10, method=method, mu2=1.0)
+    s.run()
+    u, _ = s.transfer_snapshots()
+    ur, _ = orc.kge_run(n, L, T, nt, ns, m, c, u0, v0, krylov=10, method=meth, mu2=1.0)
+    for k in range(ns):
+        assert rel_l2(u[k], ur[k]) < 1e-10
+
+
+@pytest.mark.parametrize("q6", [True, False])
+def test_kge_3d_plain_vs_oracle(orc, q6):
+    from nls.samplers import bubbles_3d
+    from nls.solvers import KGESolverDevice
+    n, L, T, nt, ns = 14, 8.0, 0.2, 50, 5
+    u0, v0 = bubbles_3d(n, L, 3, 0)
+    m = np.ones((n, n, n))
+    c = np.ones((n, n, n))
+    s = KGESolverDevice(n, L, T, nt, ns, m, c, u0, v0, krylov_dim=10, dim=3, q6=q6)
+    s.run()
+    u, _ = s.transfer_snapshots()
+    ur, _ = orc.kge_run(n, L, T, nt, ns, m, c, u0, v0, krylov=10, dim=3, q6=q6)
+    for k in range(ns):
+        assert rel_l2(u[k], ur[k]) < 1e-10
+
+
+@pytest.mark.parametrize("method", ["strang", "sewi"])
+def test_nlse_plain_vs_oracle(orc, method):
+    from nls.samplers import nlse_packets_2d
+    from nls.solvers import NLSESolverDevice
+    n, L, T, nt, ns = 48, 10.0, 0.25, 50, 5
+    psi, _ = nlse_packets_2d(n, L, 1, 2)
+    m = np.ones((n, n))
+    c = np.ones((n, n))
+    s = NLSESolverDevice(n, L, T, nt, ns, m, c, psi, krylov_dim=12, method=method)
+    s.run()
+    out = s.transfer_snapshots()
+    ref = orc.nlse_run(n, L, T, nt, ns, m, c, psi, krylov=12,
+                       method={"strang": 0, "sewi": 1}[method], variant=0)
+    for k in range(ns):
+        assert rel_l2(out[k], ref[k]) < 1e-10
+
+
+@pytest.mark.parametrize("mode", ["reference", "damped"])
+def test_sp4_plain_vs_oracle(orc, mode):
+    from nls.solvers import SP4SolverDevice
+    n, L, T, nt, ns = 64, 10.0, 1.0, 200, 4
+    u0, v0 = field(n, 9)
+    m = np.ones((n, n))
+    c = np.ones((n, n))
+    md = {"reference": 0, "damped": 1}[mode]
+    s = SP4SolverDevice(n, L, T, nt, ns, m, c, u0, v0, mode=mode, gamma=0.1, sigma=0.3, seed=77,
+                        stream_id=5)
+    s.run()
+    u, v = s.transfer_snapshots()
+    ur, vr = orc.sp4_run(n, L, T, nt, ns, m, c, u0, v0, mode=md, gamma=0.1, sigma=0.3, seed=77,
+                         stream=5)
+    for k in range(ns):
+        assert rel_l2(u[k], ur[k]) < 1e-12
+
+
+def test_philox_normals_bitexact(orc):
+    from nls.solvers import philox_normals
+    for seed, stream, step in ((0, 0, 0), (12345, 3, 17), (2 ** 40 + 5, 1, 9999)):
+        z = philox_normals(seed, stream, step, 10007)
+        ref = orc.noise_field(seed, stream, step, 10007)
+        assert np.array_equal(z, ref)
