@@ -276,6 +276,60 @@ NLS_API int nls_batch_pass_timing(nls_batch *b, double *avg_ms, int *launches,
 NLS_API int nls_batch_estimate_bytes(const nls_batch_params *p, int ntraj,
                                      const nls_traj_desc *trajs, size_t *bytes);
 
+/* ---------------------------------------------------------------- serving loop */
+/* Continuous batching of stochastic phi^4 trajectory requests of different
+ * sizes on one GPU.  No reference counterpart as a service: it replaces the
+ * SLURM job arrays that run one executable per trajectory
+ * (scripts/kg_2d_launch.sh, scripts/real_launcher_2d.py:260-306) and the
+ * per-step noise of SP4Solver::step (device/include/stochastic_phi4.cuh:19-77).
+ * Requests are admitted first-in first-out while they fit the HBM budget (one
+ * pre-allocated slab) and the slot limit; every launch advances every active
+ * request by one Stormer-Verlet step with Philox noise keyed by the request's
+ * seed, so a request's result does not depend on its batch-mates. */
+typedef struct {
+  size_t hbm_budget_bytes; /* device slab holding all admitted requests */
+  int max_active;          /* concurrent requests */
+  double L;                /* domain [-L, L]^2 */
+  int mode;                /* NLS_SP4_DAMPED or NLS_SP4_REFERENCE */
+  double gamma;            /* damping */
+  int max_steps_per_group; /* steps launched between admission checks (0 = 64) */
+} nls_server_params;
+typedef struct {
+  int n;             /* grid side */
+  int nt;            /* nt-1 steps are taken */
+  double dt;
+  double sigma;      /* noise amplitude */
+  uint64_t seed;     /* Philox key */
+  int num_snapshots; /* snapshot k at step k*(nt/num_snapshots); 0 = final state only */
+} nls_request;
+typedef struct {
+  int64_t completed, failed, active, queued;
+  double p50_ms, p99_ms, mean_ms; /* completion - submission */
+  double cell_steps;              /* total cell-steps executed */
+  double busy_s;                  /* time spent in launched step groups */
+  size_t peak_bytes;
+  int peak_active;
+} nls_server_stats;
+typedef struct nls_server nls_server;
+NLS_API int nls_server_create(const nls_server_params *p, nls_server **out);
+NLS_API int nls_server_destroy(nls_server *s);
+/* m, c, u0, v0: host n*n doubles or NULL (m = c = 1, u0 = v0 = 0). */
+NLS_API int nls_server_submit(nls_server *s, const nls_request *q, const double *m, const double *c,
+                              const double *u0, const double *v0, int64_t *id);
+/* state: 0 queued, 1 running, 2 done, 3 failed */
+NLS_API int nls_server_status(nls_server *s, int64_t id, int *state);
+NLS_API int nls_server_wait(nls_server *s, int64_t id, double timeout_s);
+NLS_API int nls_server_drain(nls_server *s, double timeout_s);
+/* snapshots (count*n*n doubles), final state (n*n), latency; any output may be NULL. */
+NLS_API int nls_server_fetch(nls_server *s, int64_t id, double *snapshots, int *count,
+                             double *u_final, double *latency_ms);
+NLS_API int nls_server_release(nls_server *s, int64_t id);
+NLS_API int nls_server_get_stats(nls_server *s, nls_server_stats *st);
+/* Host-only check of the admission allocator (first-fit over one slab):
+ * ops[2k] = 0 allocates ops[2k+1] bytes, 1 frees the block of op ops[2k+1];
+ * results[k] = offset (-1 if it does not fit) or bytes in use after a free. */
+NLS_API int nls_pool_simulate(size_t capacity, int nops, const int64_t *ops, int64_t *results);
+
 #ifdef __cplusplus
 }
 #endif

@@ -460,6 +460,270 @@ static int autotune_krylov(const std::vector<double> &rho_tau2) {
   return 16;
 }
 
+// ================================================================ cubic NLSE (Strang)
+// Reference step (nlse_dev.hpp:187-197):
+//   psi1 = psi exp(i dt/2 m|psi|^2);  psi2 = exp(i dt L) psi1;  psi3 = psi2 exp(i dt/2 m|psi2|^2)
+// with the snapshot taken before the ghost copy (nlse_dev.hpp:195-197).  One
+// complex64 Krylov sequence; L is real symmetric, so alpha and b are real and
+// T is real (the reference's Zdotc-based T is Hermitian with rounding-level
+// imaginary parts).  The first pass forms psi1 at each cell and its neighbours
+// and stores it (seed X); the combine pass forms psi2 = sum_k y_k v_k, the
+// second half phase, the snapshot and the ghost copy.
+struct c32 {
+  float re, im;
+};
+__device__ inline c32 operator+(c32 a, c32 b) { return {a.re + b.re, a.im + b.im}; }
+__device__ inline c32 operator-(c32 a, c32 b) { return {a.re - b.re, a.im - b.im}; }
+__device__ inline c32 operator*(float s, c32 a) { return {s * a.re, s * a.im}; }
+
+struct NlseArgs {
+  const TrajDev *traj;
+  const int2 *cmap;
+  TrajScal *scal; // sequence 0 arrays; y[1] holds Im(y)
+  double *part;   // [2 parity][total_chunks][PN]
+  int total_chunks;
+  const float *mf, *cf;
+  const c32 *psi;
+  c32 *psi_next;
+  c32 *X; // seed psi1
+  c32 *V; // [m-2][cells]
+  c32 *W; // [2][cells]
+  int64_t cells;
+  int m, q6;
+  double dt;
+};
+
+__device__ inline c32 nl_half_phase(c32 p, float mval, double dt) {
+  const double rho = (double)mval * ((double)p.re * p.re + (double)p.im * p.im);
+  const double ang = 0.5 * dt * rho;
+  double sn, cs;
+  sincos(ang, &sn, &cs);
+  return {(float)(p.re * cs - p.im * sn), (float)(p.re * sn + p.im * cs)};
+}
+
+__device__ inline void reduce_partials3(const NlseArgs &a, const TrajDev &T, int parity,
+                                        double *red) {
+  if (threadIdx.x < kWave) {
+    const int lane = threadIdx.x;
+    double acc[PN] = {0.0, 0.0, 0.0};
+    const double *p = a.part + ((size_t)parity * a.total_chunks + T.chunk0) * PN;
+    for (int g = lane; g < T.nchunks; g += kWave)
+#pragma unroll
+      for (int k = 0; k < PN; ++k) acc[k] += p[(size_t)g * PN + k];
+#pragma unroll
+    for (int k = 0; k < PN; ++k) {
+      const double v = wave_sum(acc[k]);
+      if (lane == 0) red[k] = v;
+    }
+  }
+  __syncthreads();
+}
+
+template <int DIM, int STAGE, int CPT>
+__global__ __launch_bounds__(FBS) void k_nlse_pass(NlseArgs a, int j) {
+  const int2 cm = a.cmap[blockIdx.x];
+  if (cm.x < 0) return;
+  const TrajDev T = a.traj[cm.x];
+  __shared__ double red[PN];
+  __shared__ double sh[FBS / kWave];
+  float cA = 0.f, cB = 0.f, cC = 0.f;
+  if (STAGE >= 1) {
+    reduce_partials3(a, T, (j - 1) & 1, red);
+    const double q0 = red[0], q1 = red[1], q2 = red[2];
+    if (STAGE == 1) {
+      const double b0 = sqrt(q0);
+      const double al = q0 > 0.0 ? q1 / q0 : 0.0;
+      const double b1sq = q0 > 0.0 ? (q2 - q1 * q1 / q0) / q0 : 0.0;
+      const double b1 = (b1sq > 1e-14 * (q0 > 0.0 ? q2 / q0 : 0.0) && b1sq > 0.0) ? sqrt(b1sq) : 0.0;
+      cA = (float)al;
+      cB = (b0 > 0.0 && b1 > 0.0) ? (float)(1.0 / (b0 * b1)) : 0.0f;
+      cC = b0 > 0.0 ? (float)(b1 / b0) : 0.0f;
+      if (cm.y == 0 && threadIdx.x == 0) {
+        a.scal[cm.x].b0[0] = b0;
+        a.scal[cm.x].alpha[0][0] = al;
+        a.scal[cm.x].beta[0][1] = b1;
+      }
+    } else {
+      double al, b;
+      lanczos_scalars(q0, q1, q2, al, b);
+      cA = (float)al;
+      cB = b > 0.0 ? (float)(1.0 / b) : 0.0f;
+      cC = (float)b;
+      if (cm.y == 0 && threadIdx.x == 0) {
+        a.scal[cm.x].alpha[0][j - 1] = al;
+        a.scal[cm.x].beta[0][j] = b;
+      }
+    }
+  }
+  const int n = T.n;
+  const float sc = T.s;
+  const int64_t cells = a.cells;
+  double acc[PN] = {0.0, 0.0, 0.0};
+  const int64_t base = (int64_t)cm.y * (FBS * CPT);
+#pragma unroll
+  for (int kk = 0; kk < CPT; ++kk) {
+    const int64_t i = base + threadIdx.x + (int64_t)kk * FBS;
+    if (i >= T.N) break;
+    const int64_t gi = T.off + i;
+    Nbr nb;
+    neighbours<DIM>(i, n, T.N, a.q6, nb);
+    const float ci = a.cf[gi];
+    float w[6];
+    float wsum = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 2 * DIM; ++q) {
+      w[q] = nb.has[q] ? 0.5f * (ci + a.cf[gi + nb.off[q]]) : 0.0f;
+      wsum += w[q];
+    }
+    auto val = [&](int64_t g) -> c32 {
+      if (STAGE == 0) return nl_half_phase(a.psi[g], a.mf[g], a.dt);
+      if (STAGE == 1) return cB * (a.W[g] - cA * a.X[g]);
+      return cB * (a.W[(size_t)((j - 1) & 1) * cells + g] - cA * a.V[(size_t)(j - 2) * cells + g]);
+    };
+    const c32 vi = val(gi);
+    c32 nbsum{0.f, 0.f};
+#pragma unroll
+    for (int q = 0; q < 2 * DIM; ++q)
+      if (nb.has[q]) nbsum = nbsum + w[q] * val(gi + nb.off[q]);
+    // L v = s * (sum_nb w v_nb - wsum v_i): div(c grad), not negated (nlse_cubic_driver_2d.cpp:94-97)
+    const c32 Lv = sc * (nbsum - wsum * vi);
+    if (STAGE == 0) {
+      a.X[gi] = vi;
+      a.W[gi] = Lv;
+      acc[0] += (double)vi.re * vi.re + (double)vi.im * vi.im;
+      acc[1] += (double)vi.re * Lv.re + (double)vi.im * Lv.im;
+      acc[2] += (double)Lv.re * Lv.re + (double)Lv.im * Lv.im;
+    } else {
+      const c32 vprev = STAGE == 1 ? a.X[gi] : a.V[(size_t)(j - 2) * cells + gi];
+      const c32 wp = Lv - cC * vprev;
+      a.V[(size_t)(j - 1) * cells + gi] = vi;
+      a.W[(size_t)(j & 1) * cells + gi] = wp;
+      acc[0] += (double)wp.re * vi.re + (double)wp.im * vi.im;
+      acc[1] += (double)wp.re * wp.re + (double)wp.im * wp.im;
+      acc[2] += (double)vi.re * vi.re + (double)vi.im * vi.im;
+    }
+  }
+  double *dst = a.part + ((size_t)(j & 1) * a.total_chunks + T.chunk0 + cm.y) * PN;
+#pragma unroll
+  for (int k = 0; k < PN; ++k) {
+    const double t = block_sum<FBS>(acc[k], sh);
+    if (threadIdx.x == 0) dst[k] = t;
+  }
+}
+
+__global__ __launch_bounds__(kWave) void k_nlse_small(NlseArgs a, int last_pass) {
+  const int t = blockIdx.x;
+  const TrajDev T = a.traj[t];
+  __shared__ SmallWS ws;
+  __shared__ double red[PN];
+  __shared__ double yr[kMaxKrylov], yi[kMaxKrylov];
+  const int m = a.m;
+  {
+    const int lane = threadIdx.x;
+    double acc[PN] = {0.0, 0.0, 0.0};
+    const double *p = a.part + ((size_t)(last_pass & 1) * a.total_chunks + T.chunk0) * PN;
+    for (int g = lane; g < T.nchunks; g += kWave)
+      for (int k = 0; k < PN; ++k) acc[k] += p[(size_t)g * PN + k];
+    for (int k = 0; k < PN; ++k) {
+      const double v = wave_sum(acc[k]);
+      if (lane == 0) red[k] = v;
+    }
+  }
+  __syncthreads();
+  TrajScal &S = a.scal[t];
+  if (threadIdx.x == 0) {
+    double al, b;
+    lanczos_scalars(red[0], red[1], red[2], al, b);
+    S.alpha[0][m - 2] = al;
+    S.beta[0][m - 1] = b;
+    for (int k = 0; k < m - 1; ++k) ws.d[k] = S.alpha[0][k];
+    ws.d[m - 1] = 0.0; // Q1
+    for (int k = 0; k < m - 1; ++k) ws.e[k] = S.beta[0][k + 1];
+    for (int k = 0; k < m; ++k) { ws.phr[k] = 1.0; ws.phi[k] = 0.0; }
+  }
+  __syncthreads();
+  if (small_ql(m, ws)) {
+    if (threadIdx.x == 0) S.status = 1;
+    return;
+  }
+  small_apply_f(m, ws, F_EXP, 0.0, a.dt, S.b0[0], yr, yi);
+  __syncthreads();
+  if (threadIdx.x < m) {
+    S.y[0][threadIdx.x] = yr[threadIdx.x];
+    S.y[1][threadIdx.x] = yi[threadIdx.x];
+  }
+}
+
+__device__ inline void nlse_psi3(const NlseArgs &a, const TrajScal &S, const double *yr,
+                                 const double *yi, int64_t g, double &ore, double &oim) {
+  const int m = a.m;
+  const int64_t cells = a.cells;
+  const double b0 = S.b0[0];
+  double re = 0.0, im = 0.0;
+  if (b0 > 0.0) {
+    const c32 x = a.X[g];
+    const double xr = x.re / b0, xi = x.im / b0;
+    re += yr[0] * xr - yi[0] * xi;
+    im += yr[0] * xi + yi[0] * xr;
+  }
+  for (int k = 1; k <= m - 2; ++k) {
+    const c32 v = a.V[(size_t)(k - 1) * cells + g];
+    re += yr[k] * v.re - yi[k] * v.im;
+    im += yr[k] * v.im + yi[k] * v.re;
+  }
+  const double bl = S.beta[0][m - 1];
+  if (bl > 0.0) {
+    const c32 wl = a.W[(size_t)((m - 2) & 1) * cells + g];
+    const c32 vl = a.V[(size_t)(m - 3) * cells + g];
+    const float al = (float)S.alpha[0][m - 2];
+    const double vr = (double)(wl.re - al * vl.re) / bl, vim = (double)(wl.im - al * vl.im) / bl;
+    re += yr[m - 1] * vr - yi[m - 1] * vim;
+    im += yr[m - 1] * vim + yi[m - 1] * vr;
+  }
+  // second half phase with |psi2|^2 (nlse_dev.hpp:192-193)
+  const double rho = (double)a.mf[g] * (re * re + im * im);
+  double sn, cs;
+  sincos(0.5 * a.dt * rho, &sn, &cs);
+  ore = re * cs - im * sn;
+  oim = re * sn + im * cs;
+}
+
+struct SnapC {
+  double *psi; // complex128 interleaved, slot base
+};
+
+template <int DIM, int CPT>
+__global__ __launch_bounds__(FBS) void k_nlse_combine(NlseArgs a, SnapC snap) {
+  const int2 cm = a.cmap[blockIdx.x];
+  if (cm.x < 0) return;
+  const TrajDev T = a.traj[cm.x];
+  const TrajScal &S = a.scal[cm.x];
+  __shared__ double ys[2][kMaxKrylov];
+  if (threadIdx.x < a.m) {
+    ys[0][threadIdx.x] = S.y[0][threadIdx.x];
+    ys[1][threadIdx.x] = S.y[1][threadIdx.x];
+  }
+  __syncthreads();
+  const int n = T.n;
+  const int64_t base = (int64_t)cm.y * (FBS * CPT);
+#pragma unroll
+  for (int kk = 0; kk < CPT; ++kk) {
+    const int64_t i = base + threadIdx.x + (int64_t)kk * FBS;
+    if (i >= T.N) break;
+    const int64_t src = clamp_src<DIM>(i, n);
+    const int64_t gi = T.off + i, gs = T.off + src;
+    double sr, si;
+    nlse_psi3(a, S, ys[0], ys[1], gs, sr, si);
+    a.psi_next[gi] = c32{(float)sr, (float)si};
+    if (snap.psi) { // snapshot before the ghost copy (nlse_dev.hpp:195-197)
+      double ir = sr, ii = si;
+      if (src != i) nlse_psi3(a, S, ys[0], ys[1], gi, ir, ii);
+      snap.psi[2 * gi] = ir;
+      snap.psi[2 * gi + 1] = ii;
+    }
+  }
+}
+
 } // namespace fused
 } // namespace nls
 
@@ -479,8 +743,8 @@ struct nls_batch {
   size_t cmap_cap = 0;
   TrajScal *d_scal = nullptr;
   double *d_part = nullptr;
-  float *d_mf = nullptr, *d_cf = nullptr, *d_V = nullptr, *d_W = nullptr;
-  double *d_state[3] = {nullptr, nullptr, nullptr};
+  float *d_mf = nullptr, *d_cf = nullptr, *d_V = nullptr, *d_W = nullptr, *d_X = nullptr;
+  double *d_state[3] = {nullptr, nullptr, nullptr}; // KGE: FP64 ring of 3; NLSE: complex64 pair
   double *d_snapU = nullptr, *d_snapV = nullptr;
   size_t bytes = 0;
   int64_t step = 0;
@@ -504,7 +768,7 @@ struct nls_batch {
     if (s) (void)hipStreamSynchronize(s);
     drop_graphs();
     for (auto e : ev_pool) (void)hipEventDestroy(e);
-    void *ptrs[] = {d_traj, d_cmap, d_scal, d_part, d_mf, d_cf, d_V, d_W,
+    void *ptrs[] = {d_traj, d_cmap, d_scal, d_part, d_mf, d_cf, d_V, d_W, d_X,
                     d_state[0], d_state[1], d_state[2], d_snapU, d_snapV};
     for (void *q : ptrs)
       if (q) (void)hipFree(q);
@@ -535,6 +799,47 @@ struct nls_batch {
     a.q6 = p.q6;
     a.kind1 = p.method == NLS_KGE_SEWI ? F_SINC2_HALF : F_SINC2_SQRT;
     return a;
+  }
+
+  bool nlse() const { return p.equation == NLS_EQ_NLSE; }
+  int nstates() const { return nlse() ? 2 : 3; }
+
+  NlseArgs nargs(int cur) const {
+    NlseArgs a{};
+    a.traj = d_traj;
+    a.cmap = d_cmap;
+    a.scal = d_scal;
+    a.part = d_part;
+    a.total_chunks = lay.total_chunks;
+    a.mf = d_mf;
+    a.cf = d_cf;
+    a.psi = reinterpret_cast<const c32 *>(d_state[cur & 1]);
+    a.psi_next = reinterpret_cast<c32 *>(d_state[(cur + 1) & 1]);
+    a.X = reinterpret_cast<c32 *>(d_X);
+    a.V = reinterpret_cast<c32 *>(d_V);
+    a.W = reinterpret_cast<c32 *>(d_W);
+    a.cells = lay.cells;
+    a.m = m;
+    a.q6 = p.q6;
+    a.dt = p.dt;
+    return a;
+  }
+
+  template <int DIM, int CPT> void launch_nlse_t(int cur, bool snapshot, int slot, bool prof) {
+    const NlseArgs a = nargs(cur);
+    const dim3 grid((unsigned)lay.cmap.size()), blk(FBS);
+    hipLaunchKernelGGL((k_nlse_pass<DIM, 0, CPT>), grid, blk, 0, s, a, 0);
+    hipLaunchKernelGGL((k_nlse_pass<DIM, 1, CPT>), grid, blk, 0, s, a, 1);
+    for (int j = 2; j <= m - 2; ++j) {
+      if (prof) NLS_HIP(hipEventRecord(next_event(), s));
+      hipLaunchKernelGGL((k_nlse_pass<DIM, 2, CPT>), grid, blk, 0, s, a, j);
+      if (prof) NLS_HIP(hipEventRecord(next_event(), s));
+    }
+    hipLaunchKernelGGL(k_nlse_small, dim3(ntraj), dim3(kWave), 0, s, a, m - 2);
+    SnapC sn{nullptr};
+    if (snapshot) sn.psi = d_snapU + (size_t)slot * lay.cells * 2;
+    hipLaunchKernelGGL((k_nlse_combine<DIM, CPT>), grid, blk, 0, s, a, sn);
+    check_launch("fused nlse step");
   }
 
   void set_layout(int cpt_) {
@@ -579,6 +884,18 @@ struct nls_batch {
   }
 
   void launch_step(int cur, bool snapshot, int slot, bool prof) {
+    if (nlse()) {
+      if (p.dim == 2) {
+        if (cpt == 4) launch_nlse_t<2, 4>(cur, snapshot, slot, prof);
+        else if (cpt == 8) launch_nlse_t<2, 8>(cur, snapshot, slot, prof);
+        else launch_nlse_t<2, 16>(cur, snapshot, slot, prof);
+      } else {
+        if (cpt == 4) launch_nlse_t<3, 4>(cur, snapshot, slot, prof);
+        else if (cpt == 8) launch_nlse_t<3, 8>(cur, snapshot, slot, prof);
+        else launch_nlse_t<3, 16>(cur, snapshot, slot, prof);
+      }
+      return;
+    }
     if (p.dim == 2) {
       if (cpt == 4) launch_step_t<2, 4>(cur, snapshot, slot, prof);
       else if (cpt == 8) launch_step_t<2, 8>(cur, snapshot, slot, prof);
@@ -602,9 +919,9 @@ struct nls_batch {
     NLS_HIP(hipEventCreate(&e1));
     for (int c : cands) {
       set_layout(c);
-      launch_step((int)(step % 3), false, 0, false); // warm-up
+      launch_step((int)(step % nstates()), false, 0, false); // warm-up
       NLS_HIP(hipEventRecord(e0, s));
-      for (int r = 0; r < 3; ++r) launch_step((int)(step % 3), false, 0, false);
+      for (int r = 0; r < 3; ++r) launch_step((int)(step % nstates()), false, 0, false);
       NLS_HIP(hipEventRecord(e1, s));
       NLS_HIP(hipEventSynchronize(e1));
       float ms = 0;
@@ -619,7 +936,7 @@ struct nls_batch {
 
   void capture_graphs() {
     drop_graphs();
-    for (int cur = 0; cur < 3; ++cur) {
+    for (int cur = 0; cur < nstates(); ++cur) {
       hipGraph_t g;
       NLS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       launch_step(cur, false, 0, false);
@@ -655,7 +972,7 @@ struct nls_batch {
     NLS_HIP(hipEventRecord(e0, s));
     for (int k = 0; k < nsteps; ++k) {
       const int64_t idx = step + 1; // reference step number i (kg_driver_dev_2d.cpp:144)
-      const int cur = (int)(step % 3);
+      const int cur = (int)(step % nstates());
       const bool snap = p.snapshot_every > 0 && idx % p.snapshot_every == 0 &&
                         idx / p.snapshot_every < p.max_snapshots;
       if (snap || !p.use_graph || profiling || !graphs[cur])
@@ -705,13 +1022,14 @@ static size_t estimate(const nls_batch_params &p, int ntraj, const nls_traj_desc
     chunks += (N + FBS * 4 - 1) / (FBS * 4);
   }
   size_t b = 0;
+  const bool nl = p.equation == NLS_EQ_NLSE;
   b += sizeof(float) * cells * 2;                         // m, c
-  b += sizeof(double) * cells * 3;                        // state ring
-  b += sizeof(float) * cells * NSEQ * (size_t)(m - 2);    // Krylov basis
+  b += sizeof(double) * cells * 3;                        // state ring (NLSE: 2 complex64 + seed)
+  b += sizeof(float) * cells * NSEQ * (size_t)(m - 2);    // Krylov basis (NLSE: complex64)
   b += sizeof(float) * cells * NSEQ * 2;                  // w' ping-pong
   b += sizeof(double) * 2 * chunks * NSEQ * PN;           // partials
   b += sizeof(TrajScal) * ntraj;
-  b += sizeof(double) * cells * 2 * (size_t)std::max(p.max_snapshots, 0);
+  b += sizeof(double) * cells * 2 * (size_t)std::max(p.max_snapshots, 0) * (nl ? 1 : 1);
   return b;
 }
 
@@ -740,10 +1058,14 @@ int nls_batch_create(const nls_batch_params *p, int ntraj, const nls_traj_desc *
                      nls_batch **out) {
   return guard([&] {
     NLS_REQUIRE(p && trajs && out && ntraj > 0, "nls_batch_create: bad arguments");
-    NLS_REQUIRE(p->equation == NLS_EQ_KGE, "nls_batch_create: only the Klein-Gordon equation "
-                                           "is implemented by the fused batch so far");
-    NLS_REQUIRE(p->method == NLS_KGE_GAUTSCHI || p->method == NLS_KGE_SEWI,
-                "fused Klein-Gordon batch supports the Gautschi and sEWI methods");
+    NLS_REQUIRE(p->equation == NLS_EQ_KGE || p->equation == NLS_EQ_NLSE,
+                "nls_batch_create: the fused batch runs Klein-Gordon and NLSE; stochastic phi^4 "
+                "is served by nls_server");
+    if (p->equation == NLS_EQ_KGE)
+      NLS_REQUIRE(p->method == NLS_KGE_GAUTSCHI || p->method == NLS_KGE_SEWI,
+                  "fused Klein-Gordon batch supports the Gautschi and sEWI methods");
+    else
+      NLS_REQUIRE(p->method == NLS_NLSE_STRANG, "fused NLSE batch supports the Strang method");
     NLS_REQUIRE(p->dim == 2 || p->dim == 3, "dim must be 2 or 3");
     NLS_REQUIRE(p->dt > 0, "dt must be positive");
     NLS_REQUIRE(p->cells_per_thread == 0 || p->cells_per_thread == 4 ||
@@ -767,15 +1089,18 @@ int nls_batch_create(const nls_batch_params *p, int ntraj, const nls_traj_desc *
     const int64_t cells = l4.cells;
     b->d_mf = b->alloc<float>(cells);
     b->d_cf = b->alloc<float>(cells);
-    for (int k = 0; k < 3; ++k) b->d_state[k] = b->alloc<double>(cells);
+    const int nst = b->nstates();
+    for (int k = 0; k < nst; ++k) b->d_state[k] = b->alloc<double>(cells);
     b->d_V = b->alloc<float>((size_t)cells * NSEQ * (b->m - 2));
     b->d_W = b->alloc<float>((size_t)cells * NSEQ * 2);
+    if (b->nlse()) b->d_X = b->alloc<float>((size_t)cells * 2);
     if (p->max_snapshots > 0) {
-      b->d_snapU = b->alloc<double>((size_t)cells * p->max_snapshots);
-      b->d_snapV = b->alloc<double>((size_t)cells * p->max_snapshots);
+      // KGE: u and v (FP64); NLSE: psi (complex128) in d_snapU
+      b->d_snapU = b->alloc<double>((size_t)cells * p->max_snapshots * (b->nlse() ? 2 : 1));
+      if (!b->nlse()) b->d_snapV = b->alloc<double>((size_t)cells * p->max_snapshots);
     }
     NLS_HIP(hipMemsetAsync(b->d_scal, 0, sizeof(TrajScal) * ntraj, b->s));
-    for (int k = 0; k < 3; ++k) NLS_HIP(hipMemsetAsync(b->d_state[k], 0, sizeof(double) * cells, b->s));
+    for (int k = 0; k < nst; ++k) NLS_HIP(hipMemsetAsync(b->d_state[k], 0, sizeof(double) * cells, b->s));
     NLS_HIP(hipMemsetAsync(b->d_mf, 0, sizeof(float) * cells, b->s));
     NLS_HIP(hipMemsetAsync(b->d_cf, 0, sizeof(float) * cells, b->s));
     b->set_layout(8);
@@ -793,6 +1118,26 @@ int nls_batch_set_initial(nls_batch *b, int t, const double *m, const double *c,
     NLS_REQUIRE(b->step == 0, "nls_batch_set_initial: the batch has already been advanced");
     const TrajDev &T = b->lay.traj[t];
     const int64_t N = T.N;
+    if (b->nlse()) {
+      std::vector<float> mf(N), cf(N), psi(2 * N);
+      for (int64_t i = 0; i < N; ++i) {
+        mf[i] = (float)m[i];
+        cf[i] = (float)c[i];
+        psi[2 * i] = (float)u0[2 * i];
+        psi[2 * i + 1] = (float)u0[2 * i + 1];
+      }
+      NLS_HIP(hipMemcpyAsync(b->d_mf + T.off, mf.data(), sizeof(float) * N, hipMemcpyHostToDevice, b->s));
+      NLS_HIP(hipMemcpyAsync(b->d_cf + T.off, cf.data(), sizeof(float) * N, hipMemcpyHostToDevice, b->s));
+      NLS_HIP(hipMemcpyAsync(b->d_state[0] + T.off, psi.data(), sizeof(float) * 2 * N,
+                             hipMemcpyHostToDevice, b->s));
+      if (b->p.max_snapshots > 0) {
+        NLS_HIP(hipMemcpyAsync(b->d_snapU + 2 * T.off, u0, sizeof(double) * 2 * N,
+                               hipMemcpyHostToDevice, b->s));
+        b->nsnap = std::max(b->nsnap, 1);
+      }
+      NLS_HIP(hipStreamSynchronize(b->s));
+      return;
+    }
     std::vector<float> mf(N), cf(N);
     std::vector<double> upast(N);
     for (int64_t i = 0; i < N; ++i) {
@@ -826,6 +1171,13 @@ int nls_batch_get_snapshots(nls_batch *b, int t, double *u, double *v, int *coun
     NLS_REQUIRE(b && t >= 0 && t < b->ntraj, "nls_batch_get_snapshots: bad trajectory index");
     const TrajDev &T = b->lay.traj[t];
     if (count) *count = b->nsnap;
+    if (b->nlse()) {
+      for (int k = 0; k < b->nsnap && u; ++k)
+        NLS_HIP(hipMemcpyAsync(u + (size_t)k * 2 * T.N, b->d_snapU + ((size_t)k * b->lay.cells + T.off) * 2,
+                               sizeof(double) * 2 * T.N, hipMemcpyDeviceToHost, b->s));
+      NLS_HIP(hipStreamSynchronize(b->s));
+      return;
+    }
     for (int k = 0; k < b->nsnap; ++k) {
       if (u) NLS_HIP(hipMemcpyAsync(u + (size_t)k * T.N, b->d_snapU + (size_t)k * b->lay.cells + T.off,
                                     sizeof(double) * T.N, hipMemcpyDeviceToHost, b->s));
@@ -840,6 +1192,14 @@ int nls_batch_get_state(nls_batch *b, int t, double *u, double *v) {
   return guard([&] {
     NLS_REQUIRE(b && t >= 0 && t < b->ntraj, "nls_batch_get_state: bad trajectory index");
     const TrajDev &T = b->lay.traj[t];
+    if (b->nlse()) {
+      std::vector<float> psi(2 * T.N);
+      NLS_HIP(hipMemcpyAsync(psi.data(), b->d_state[b->step & 1] + T.off, sizeof(float) * 2 * T.N,
+                             hipMemcpyDeviceToHost, b->s));
+      NLS_HIP(hipStreamSynchronize(b->s));
+      for (int64_t i = 0; i < 2 * T.N; ++i) u[i] = psi[i];
+      return;
+    }
     const int cur = (int)(b->step % 3), prev = (cur + 2) % 3;
     NLS_HIP(hipMemcpyAsync(u, b->d_state[cur] + T.off, sizeof(double) * T.N, hipMemcpyDeviceToHost, b->s));
     if (v) {

@@ -43,6 +43,23 @@ class BatchParams(C.Structure):
                 ("max_snapshots", C.c_int), ("cells_per_thread", C.c_int)]
 
 
+class ServerParams(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_size_t), ("max_active", C.c_int), ("L", C.c_double),
+                ("mode", C.c_int), ("gamma", C.c_double), ("max_steps_per_group", C.c_int)]
+
+
+class Request(C.Structure):
+    _fields_ = [("n", C.c_int), ("nt", C.c_int), ("dt", C.c_double), ("sigma", C.c_double),
+                ("seed", C.c_uint64), ("num_snapshots", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("completed", C.c_int64), ("failed", C.c_int64), ("active", C.c_int64),
+                ("queued", C.c_int64), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
+                ("mean_ms", C.c_double), ("cell_steps", C.c_double), ("busy_s", C.c_double),
+                ("peak_bytes", C.c_size_t), ("peak_active", C.c_int)]
+
+
 class TrajDesc(C.Structure):
     _fields_ = [("n", C.c_int), ("L", C.c_double), ("sigma", C.c_double), ("seed", C.c_uint64)]
 
@@ -126,6 +143,16 @@ _SIGS = {
     "nls_batch_pass_timing": [_P, _D, C.POINTER(C.c_int), _D],
     "nls_batch_estimate_bytes": [C.POINTER(BatchParams), C.c_int, C.POINTER(TrajDesc),
                                  C.POINTER(C.c_size_t)],
+    "nls_server_create": [C.POINTER(ServerParams), C.POINTER(C.c_void_p)],
+    "nls_server_destroy": [_P],
+    "nls_server_submit": [_P, C.POINTER(Request), _D, _D, _D, _D, _I64],
+    "nls_server_status": [_P, C.c_int64, C.POINTER(C.c_int)],
+    "nls_server_wait": [_P, C.c_int64, C.c_double],
+    "nls_server_drain": [_P, C.c_double],
+    "nls_server_fetch": [_P, C.c_int64, _D, C.POINTER(C.c_int), _D, _D],
+    "nls_server_release": [_P, C.c_int64],
+    "nls_server_get_stats": [_P, C.POINTER(ServerStats)],
+    "nls_pool_simulate": [C.c_size_t, C.c_int, _I64, _I64],
 }
 _RESTYPE = {"nls_last_error": C.c_char_p, "nls_laplacian_nnz": C.c_int64}
 

@@ -29,8 +29,8 @@ def make_case(n, seed, traj, L=10.0, ring=False):
 def test_fused_kge_uneven_batch_vs_oracle(orc):
     from nls.batch import FusedBatch
     ns_list = [48, 64, 33]
-    L, dt, nt, every = 10.0, 0.005, 201, 50
-    ns = 5
+    L, dt, nt, ns = 10.0, 0.005, 201, 5
+    every = nt // ns  # the reference cadence: freq = nt / num_snapshots
     b = FusedBatch(ns_list, L, dt, krylov_dim=10, mu2=1.0, snapshot_every=every,
                    max_snapshots=ns, cells_per_thread=8)
     cases = [make_case(n, 2, t, ring=(t == 1)) for t, n in enumerate(ns_list)]
@@ -58,7 +58,7 @@ def test_fused_kge_3d_vs_oracle(orc, method):
         m = np.ones((n, n, n))
         c = np.ones((n, n, n))
         b = FusedBatch([n], L, dt, method=method, dim=3, krylov_dim=12, mu2=1.0, q6=q6,
-                       snapshot_every=30, max_snapshots=ns, cells_per_thread=8)
+                       snapshot_every=nt // ns, max_snapshots=ns, cells_per_thread=8)
         b.set_initial(0, m, c, u0, v0)
         b.run(nt - 1)
         u, _ = b.snapshots(0)
@@ -107,7 +107,7 @@ def test_fused_autotuned_tile_matches_oracle(orc):
     from nls.batch import FusedBatch
     n, L, dt, nt = 80, 10.0, 0.005, 41
     m, c, u0, v0 = make_case(n, 6, 0)
-    b = FusedBatch([n], L, dt, krylov_dim=0, mu2=1.0, snapshot_every=40, max_snapshots=2)
+    b = FusedBatch([n], L, dt, krylov_dim=0, mu2=1.0, snapshot_every=nt // 2, max_snapshots=2)
     b.set_initial(0, m, c, u0, v0)
     b.run(nt - 1)
     info = b.info()
@@ -127,3 +127,28 @@ def test_fused_zero_field_stays_zero():
     b.run(5)
     u, _ = b.state(0)
     assert np.all(u == 0.0)
+
+
+def test_fused_nlse_vs_oracle(orc):
+    """Config-2 shape at test size: complex64 fused Strang vs the complex128
+    oracle (reference Lanczos variant), rel-L2 and mass."""
+    from nls.batch import FusedBatch
+    from nls.samplers import nlse_packets_2d
+    ns_list = [48, 40]
+    L, dt, nt, ns = 10.0, 0.005, 101, 5
+    b = FusedBatch(ns_list, L, dt, equation="nlse", method="strang", krylov_dim=12,
+                   snapshot_every=nt // ns, max_snapshots=ns, cells_per_thread=8)
+    psis = []
+    for t, n in enumerate(ns_list):
+        psi, _ = nlse_packets_2d(n, L, 1, t)
+        psis.append(psi)
+        b.set_initial(t, np.ones((n, n)), np.ones((n, n)), psi)
+    b.run(nt - 1)
+    for t, n in enumerate(ns_list):
+        out, _ = b.snapshots(t)
+        ref = orc.nlse_run(n, L, dt * nt, nt, ns, np.ones((n, n)), np.ones((n, n)), psis[t],
+                           krylov=12, method=0, variant=0)
+        for k in range(ns):
+            assert rel_l2(out[k], ref[k]) < 1e-4, (t, k)
+        m0 = np.sum(np.abs(ref[0]) ** 2)
+        assert abs(np.sum(np.abs(out[-1]) ** 2) - np.sum(np.abs(ref[-1]) ** 2)) / m0 < 1e-5

@@ -43,9 +43,9 @@ def test_csr_spmv_other_types(orc):
         if np.iscomplexobj(vals):
             vals = vals * np.exp(0.3j)  # genuinely complex values
         S = DeviceSpMV(op.rp, op.ci, vals)
-        x = rng.standard_normal(n * n).astype(vt)
-        if np.iscomplexobj(x):
-            x = x + 1j * rng.standard_normal(n * n).astype(vt)
+        x = rng.standard_normal(n * n) + (1j * rng.standard_normal(n * n)
+                                          if np.iscomplexobj(vals) else 0.0)
+        x = x.astype(vt)
         dxx, dyy = DeviceArray.from_host(x), DeviceArray(n * n, vt)
         S.multiply(dxx, dyy)
         y = dyy.to_host()
@@ -126,6 +126,8 @@ def test_kge_gautschi_config1_vs_oracle(orc):
     ur, vr = orc.kge_run(n, L, T, nt, ns, m, c, u0, v0, krylov=10, method=0)
     for k in range(ns):
         assert rel_l2(u[k], ur[k]) < 1e-10, k
+    assert np.array_equal(v[0], vr[0])  # v0 = 0
+    for k in range(1, ns):
         assert rel_l2(v[k], vr[k]) < 1e-8, k
 
 

@@ -1,0 +1,63 @@
+"""GPU tests of the serving loop: each request's result is the same alone and in
+the mix (bitwise), matches the FP64 oracle, and latency statistics are kept."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def test_server_mix_equals_alone_and_oracle(orc):
+    from nls.samplers import gaussian_sum_2d
+    from nls.server import TrajectoryServer
+    reqs = [(64, 120, 11, 0.2), (32, 200, 12, 0.05), (48, 90, 13, 0.5), (64, 150, 14, 0.3),
+            (40, 60, 15, 0.1)]
+    dt, L = 0.005, 10.0
+    ics = {}
+    for n, nt, seed, sig in reqs:
+        u0, v0, _ = gaussian_sum_2d(n, L, seed, 0)
+        ics[seed] = (u0, 0.1 * u0)
+    mix = TrajectoryServer(hbm_budget_bytes=64 << 20, max_active=8, L_=L, gamma=0.1,
+                           max_steps_per_group=16)
+    ids = {}
+    for n, nt, seed, sig in reqs:
+        u0, v0 = ics[seed]
+        ids[seed] = mix.submit(n, nt, dt, sig, seed, num_snapshots=4, u0=u0, v0=v0)
+    mix.drain(300)
+    for n, nt, seed, sig in reqs:
+        got = mix.fetch(ids[seed])
+        u0, v0 = ics[seed]
+        alone = TrajectoryServer(hbm_budget_bytes=16 << 20, max_active=1, L_=L, gamma=0.1,
+                                 max_steps_per_group=7)
+        rid = alone.submit(n, nt, dt, sig, seed, num_snapshots=4, u0=u0, v0=v0)
+        alone.wait(rid, 300)
+        ref = alone.fetch(rid)
+        alone.close()
+        assert np.array_equal(got["u_final"], ref["u_final"])
+        assert np.array_equal(got["snapshots"], ref["snapshots"])
+        ou, _ = orc.sp4_run(n, L, dt * nt, nt, 4, np.ones((n, n)), np.ones((n, n)), u0, v0,
+                            mode=1, gamma=0.1, sigma=sig, seed=seed, stream=0)
+        for k in range(4):
+            assert rel_l2(got["snapshots"][k], ou[k]) < 1e-10, (seed, k)
+    st = mix.stats()
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    assert 0 < st["p50_ms"] <= st["p99_ms"]
+    assert st["peak_active"] >= 2
+
+
+def test_server_budget_limits_concurrency():
+    from nls.server import TrajectoryServer
+    # each 64^2 request needs 3*32 KiB + 2*16 KiB = 128 KiB; a 300 KiB budget admits two
+    s = TrajectoryServer(hbm_budget_bytes=(1 << 20), max_active=64, L_=10.0,
+                         max_steps_per_group=8)
+    big = [s.submit(160, 40, 0.005, 0.1, k) for k in range(6)]  # 160^2: 800 KiB each
+    s.drain(300)
+    st = s.stats()
+    assert st["completed"] == 6 and st["peak_active"] == 1
+    assert st["peak_bytes"] <= (1 << 20)
+    with pytest.raises(Exception):
+        s.submit(400, 10, 0.005, 0.1, 1)  # larger than the whole budget
+    s.close()
