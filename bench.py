@@ -55,12 +55,15 @@ def fused_run(args, inputs):
     call("nls_dev_synchronize")
     wall = time.perf_counter() - t0
     info = b.info()
+    info.update(b.tile())
     # dominant-kernel timing: a few eager steps with HIP events around each
     # Lanczos pass launch, on the batch stream
-    b.set_profiling(True)
-    b.run(args.prof_steps)
-    pt = b.pass_timing()
-    b.set_profiling(False)
+    pt = {"avg_ms": 0.0, "launches": 0, "bytes_per_launch": 0.0}
+    if args.prof_steps > 0:
+        b.set_profiling(True)
+        b.run(args.prof_steps)
+        pt = b.pass_timing()
+        b.set_profiling(False)
     return b, ms, wall, info, pt
 
 
@@ -141,6 +144,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-parity", action="store_true")
+    ap.add_argument("--skip-plain", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not "
@@ -158,11 +162,20 @@ def main():
     args.krylov_used = info["krylov_dim"]
     achieved = pt["bytes_per_launch"] / (pt["avg_ms"] / 1e3) / 1e9 if pt["avg_ms"] > 0 else None
     log(f"[bench] fused: {value:.3e} cell-steps/s ({ms / args.steps:.3f} ms/step), "
-        f"m={info['krylov_dim']}, pass avg {pt['avg_ms'] * 1e3:.1f} us -> {achieved:.0f} GB/s")
+        f"m={info['krylov_dim']}, tile {info['cells_per_thread']} kind={info['pass_kind']}, "
+        f"pass avg {pt['avg_ms'] * 1e3:.1f} us -> {achieved or 0:.0f} GB/s")
     del b
-    log("[bench] plain FP64 CSR drop-in")
-    plain_rate, plain_wall, plain_steps = plain_run(args, inputs)
-    log(f"[bench] plain: {plain_rate:.3e} cell-steps/s")
+    import ctypes as C
+    from nls._lib import call as _call
+    gb, cms = C.c_double(), C.c_double()
+    _call("nls_stream_copy_bench", 1 << 30, 20, C.byref(gb), C.byref(cms))
+    stream_gbs = gb.value
+    log(f"[bench] STREAM copy (1 GiB, float4): {stream_gbs:.0f} GB/s")
+    plain_rate, plain_steps = None, 0
+    if not args.skip_plain:
+        log("[bench] plain FP64 CSR drop-in")
+        plain_rate, plain_wall, plain_steps = plain_run(args, inputs)
+        log(f"[bench] plain: {plain_rate:.3e} cell-steps/s")
     err = None
     if not args.skip_parity:
         log("[bench] parity vs FP64 oracle")
@@ -190,17 +203,23 @@ def main():
         "config": {"workload": f"KGE-2D batched, {args.ntraj} x {args.n}^2, side 20, Gautschi, "
                                f"mu2=1, dt={args.dt}, Krylov m={m}, HIP graph step capture",
                    "ntraj": args.ntraj, "n": args.n, "dt": args.dt, "krylov_dim": m,
+                   "cells_per_thread": info["cells_per_thread"], "pass_kind": info["pass_kind"],
                    "cells": cells, "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": None,
-                     "kernel": "k_kge_pass<2,2,CPT> (fused stencil+Lanczos pass)",
+                     "kernel": {2: f"k_kge_march2d<2, {info['cells_per_thread']}> (row-march fused "
+                                   "stencil+Lanczos pass)",
+                                1: "k_kge_pass2d<2,CPT> (LDS-tiled fused stencil+Lanczos pass)",
+                                0: "k_kge_pass<2,2,CPT> (fused stencil+Lanczos pass)"}[
+                                    info["pass_kind"]],
                      "bytes_per_launch": pt["bytes_per_launch"],
-                     "avg_launch_ms": pt["avg_ms"], "launches_timed": pt["launches"]},
+                     "avg_launch_ms": pt["avg_ms"], "launches_timed": pt["launches"],
+                     "stream_copy_gbs": stream_gbs},
         "plain": {"value": plain_rate, "unit": "cell-steps/s",
                   "path": "FP64 CSR SpMV + reference-granularity Lanczos (CGS), one trajectory "
                           "at a time", "steps": plain_steps},
-        "speedup_vs_plain": value / plain_rate,
+        "speedup_vs_plain": (value / plain_rate) if plain_rate else None,
         "rel_l2_vs_fp64_oracle": err,
         "latency": None,
         "cpu_baseline": cpu,

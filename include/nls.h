@@ -220,6 +220,10 @@ NLS_API int nls_sp4_run(nls_sp4 *h);
 NLS_API int nls_philox_normals(uint64_t seed, uint32_t stream_id, uint32_t step, int64_t count,
                                double *out_host);
 
+/* STREAM copy of `bytes` (read + write counted) repeated `reps` times: achievable
+ * HBM bandwidth on this device, reported beside the roofline figures. */
+NLS_API int nls_stream_copy_bench(size_t bytes, int reps, double *gbps, double *avg_ms);
+
 /* ---------------------------------------------------------------- fused batched path */
 /* Many trajectories (uneven grids allowed) advanced together by the fused
  * FP32 matrix-free stencil+Lanczos kernels with FP64 state and FP64 small
@@ -239,9 +243,10 @@ typedef struct {
   int use_graph;     /* capture steps into HIP graphs */
   int snapshot_every;/* steps between snapshots (0 = none kept on device) */
   int max_snapshots; /* snapshot slots per trajectory */
-  int cells_per_thread; /* tile: 4, 8 or 16 cells per thread; 0 = autotune.  A fixed
-                           tile makes every trajectory's reduction order, hence its
-                           result, independent of the other trajectories in the batch */
+  int cells_per_thread; /* tile: cells per workgroup / 256 in {4, 8, 16, 32, 64}; 0 =
+                           autotune.  A fixed tile makes every trajectory's reduction
+                           order, hence its result, independent of the other
+                           trajectories in the batch */
 } nls_batch_params;
 typedef struct {
   int n;             /* full side */
@@ -272,6 +277,9 @@ NLS_API int nls_batch_info(nls_batch *b, int *krylov_dim, int64_t *cells, size_t
 NLS_API int nls_batch_set_profiling(nls_batch *b, int on);
 NLS_API int nls_batch_pass_timing(nls_batch *b, double *avg_ms, int *launches,
                                   double *bytes_per_launch);
+/* Tile in use after the first nls_batch_run: cells per thread and the pass
+ * kernel (0 direct stencil, 1 LDS-tiled chunk, 2 row march with LDS window). */
+NLS_API int nls_batch_tile(nls_batch *b, int *cells_per_thread, int *pass_kind);
 /* Device bytes a batch with these trajectories would hold (admission control). */
 NLS_API int nls_batch_estimate_bytes(const nls_batch_params *p, int ntraj,
                                      const nls_traj_desc *trajs, size_t *bytes);

@@ -36,6 +36,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -339,57 +340,504 @@ struct SnapArgs {
   double *v;
 };
 
-template <int DIM>
-__device__ inline double kge_unew(const KgeArgs &a, const TrajScal &S, const double *ys0,
-                                  const double *ys1, int64_t g, double tau) {
-  const int m = a.m;
-  const int64_t cells = a.cells;
-  double d[NSEQ];
+// ---------------------------------------------------------------- 2D LDS-tiled Lanczos pass
+// Same arithmetic as k_kge_pass<2, STAGE, CPT>, restructured for memory-level
+// parallelism: the block first streams its chunk plus one grid row of halo on
+// each side ([base-n, base+CH+n)) with independent coalesced loads, forms
+// v_j (both sequences) once per element into LDS, then evaluates the stencil
+// from LDS.  Halo rows are shared with the neighbouring chunks, which run on
+// the same XCD (block map), so they come from L2.
+template <int STAGE, int CPT>
+__global__ __launch_bounds__(FBS) void k_kge_pass2d(KgeArgs a, int j, int emax) {
+  extern __shared__ float lds[];
+  const int2 cm = a.cmap[blockIdx.x];
+  if (cm.x < 0) return;
+  const TrajDev T = a.traj[cm.x];
+  __shared__ double red[NSEQ * PN];
+  __shared__ double sh[FBS / kWave];
+  float cA[NSEQ] = {0.f, 0.f}, cB[NSEQ] = {0.f, 0.f}, cC[NSEQ] = {0.f, 0.f};
+  if (STAGE >= 1) {
+    reduce_partials(a, T, (j - 1) & 1, red);
 #pragma unroll
-  for (int s = 0; s < NSEQ; ++s) {
-    const double *y = s == 0 ? ys0 : ys1;
-    const double b0 = S.b0[s];
-    double acc = b0 > 0.0 ? y[0] * ((double)seed_x(s, a, g) / b0) : 0.0;
-    for (int k = 1; k <= m - 2; ++k) acc += y[k] * (double)a.V[((size_t)s * (m - 2) + (k - 1)) * cells + g];
-    const double bl = S.beta[s][m - 1];
-    if (bl > 0.0) {
-      const float wl = a.W[((size_t)s * 2 + ((m - 2) & 1)) * cells + g];
-      const float vl = a.V[((size_t)s * (m - 2) + (m - 3)) * cells + g];
-      const float al = (float)S.alpha[s][m - 2];
-      acc += y[m - 1] * ((double)(wl - al * vl) / bl);
+    for (int s = 0; s < NSEQ; ++s) {
+      const double q0 = red[s * PN], q1 = red[s * PN + 1], q2 = red[s * PN + 2];
+      if (STAGE == 1) {
+        const double b0 = sqrt(q0);
+        const double al = q0 > 0.0 ? q1 / q0 : 0.0;
+        const double b1sq = q0 > 0.0 ? (q2 - q1 * q1 / q0) / q0 : 0.0;
+        const double b1 = (b1sq > 1e-14 * (q0 > 0.0 ? q2 / q0 : 0.0) && b1sq > 0.0) ? sqrt(b1sq) : 0.0;
+        cA[s] = (float)al;
+        cB[s] = (b0 > 0.0 && b1 > 0.0) ? (float)(1.0 / (b0 * b1)) : 0.0f;
+        cC[s] = b0 > 0.0 ? (float)(b1 / b0) : 0.0f;
+        if (cm.y == 0 && threadIdx.x == 0) {
+          a.scal[cm.x].b0[s] = b0;
+          a.scal[cm.x].alpha[s][0] = al;
+          a.scal[cm.x].beta[s][1] = b1;
+        }
+      } else {
+        double al, b;
+        lanczos_scalars(q0, q1, q2, al, b);
+        cA[s] = (float)al;
+        cB[s] = b > 0.0 ? (float)(1.0 / b) : 0.0f;
+        cC[s] = (float)b;
+        if (cm.y == 0 && threadIdx.x == 0) {
+          a.scal[cm.x].alpha[s][j - 1] = al;
+          a.scal[cm.x].beta[s][j] = b;
+        }
+      }
     }
-    d[s] = acc;
   }
-  return 2.0 * a.u[g] - a.up[g] + 2.0 * d[0] + (tau * tau) * d[1];
+  const int n = T.n;
+  const int64_t N = T.N, cells = a.cells;
+  constexpr int CH = FBS * CPT;
+  const int64_t base = (int64_t)cm.y * CH;
+  const int64_t cend = base + CH < N ? base + CH : N;
+  const int64_t lo = base - n > 0 ? base - n : 0;
+  const int64_t hi = cend + n < N ? cend + n : N;
+  const int E = (int)(hi - lo);
+  float *sv0 = lds, *sv1 = lds + emax, *sc = lds + 2 * emax;
+  const float *W0 = a.W + (size_t)(0 * 2 + ((j - 1) & 1)) * cells;
+  const float *W1 = a.W + (size_t)(1 * 2 + ((j - 1) & 1)) * cells;
+  const float *V0 = a.V + (size_t)(0 * (a.m - 2) + (j - 2)) * cells;
+  const float *V1 = a.V + (size_t)(1 * (a.m - 2) + (j - 2)) * cells;
+  const float *Z0 = a.W + (size_t)(0 * 2) * cells, *Z1 = a.W + (size_t)(1 * 2) * cells;
+  // phase 1: stream the extended range, UNR elements per thread in flight
+  constexpr int UNR = 4;
+  const int64_t gbase = T.off + lo;
+  for (int e0 = threadIdx.x; e0 < E; e0 += UNR * FBS) {
+    float x0[UNR], x1[UNR], y0[UNR], y1[UNR], cc[UNR];
+#pragma unroll
+    for (int q = 0; q < UNR; ++q) {
+      const int e = e0 + q * FBS;
+      const int64_t g = gbase + (e < E ? e : 0);
+      cc[q] = a.cf[g];
+      if (STAGE == 0) {
+        x0[q] = seed_x(0, a, g);
+        x1[q] = seed_x(1, a, g);
+      } else if (STAGE == 1) {
+        x0[q] = Z0[g];
+        x1[q] = Z1[g];
+        y0[q] = seed_x(0, a, g);
+        y1[q] = seed_x(1, a, g);
+      } else {
+        x0[q] = W0[g];
+        x1[q] = W1[g];
+        y0[q] = V0[g];
+        y1[q] = V1[g];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < UNR; ++q) {
+      const int e = e0 + q * FBS;
+      if (e < E) {
+        sc[e] = cc[q];
+        if (STAGE == 0) {
+          sv0[e] = x0[q];
+          sv1[e] = x1[q];
+        } else {
+          sv0[e] = (x0[q] - cA[0] * y0[q]) * cB[0];
+          sv1[e] = (x1[q] - cA[1] * y1[q]) * cB[1];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const float scl = T.s, mu2 = T.mu2;
+  double acc[NSEQ][PN];
+#pragma unroll
+  for (int s = 0; s < NSEQ; ++s)
+#pragma unroll
+    for (int k = 0; k < PN; ++k) acc[s][k] = 0.0;
+#pragma unroll
+  for (int kk = 0; kk < CPT; ++kk) {
+    const int64_t i = base + threadIdx.x + (int64_t)kk * FBS;
+    if (i < N) {
+      const int e = (int)(i - lo);
+      const int64_t r = i / n, col = i - r * n;
+      const float ci = sc[e];
+      const float wu = r > 0 ? 0.5f * (ci + sc[e - n]) : 0.0f;
+      const float wl = col > 0 ? 0.5f * (ci + sc[e - 1]) : 0.0f;
+      const float wr = col < n - 1 ? 0.5f * (ci + sc[e + 1]) : 0.0f;
+      const float wd = r < n - 1 ? 0.5f * (ci + sc[e + n]) : 0.0f;
+      const float wsum = ((wu + wl) + wr) + wd;
+      const int64_t gi = T.off + i;
+#pragma unroll
+      for (int s = 0; s < NSEQ; ++s) {
+        const float *sv = s == 0 ? sv0 : sv1;
+        const float vi = sv[e];
+        float nbsum = 0.0f;
+        if (r > 0) nbsum += wu * sv[e - n];
+        if (col > 0) nbsum += wl * sv[e - 1];
+        if (col < n - 1) nbsum += wr * sv[e + 1];
+        if (r < n - 1) nbsum += wd * sv[e + n];
+        const float Av = scl * (wsum * vi - nbsum) + mu2 * vi;
+        if (STAGE == 0) {
+          a.W[(size_t)(s * 2) * cells + gi] = Av;
+          acc[s][0] += (double)vi * vi;
+          acc[s][1] += (double)vi * Av;
+          acc[s][2] += (double)Av * Av;
+        } else {
+          // v_{j-1} at the cell (just streamed by this block: an L2 hit)
+          const float vprev = STAGE == 1 ? seed_x(s, a, gi) : (s == 0 ? V0 : V1)[gi];
+          const float wp = Av - cC[s] * vprev;
+          a.V[((size_t)s * (a.m - 2) + (j - 1)) * cells + gi] = vi;
+          a.W[((size_t)s * 2 + (j & 1)) * cells + gi] = wp;
+          acc[s][0] += (double)wp * vi;
+          acc[s][1] += (double)wp * wp;
+          acc[s][2] += (double)vi * vi;
+        }
+      }
+    }
+  }
+  double *dst = a.part + ((size_t)(j & 1) * a.total_chunks + T.chunk0 + cm.y) * (NSEQ * PN);
+#pragma unroll
+  for (int s = 0; s < NSEQ; ++s)
+#pragma unroll
+    for (int k = 0; k < PN; ++k) {
+      const double t = block_sum<FBS>(acc[s][k], sh);
+      if (threadIdx.x == 0) dst[s * PN + k] = t;
+    }
 }
 
-template <int DIM, int CPT>
-__global__ __launch_bounds__(FBS) void k_kge_combine(KgeArgs a, SnapArgs snap) {
+inline size_t pass2d_lds_bytes(int cpt, int nmax) {
+  const int CH = FBS * cpt, emax = CH + 2 * nmax;
+  return sizeof(float) * 3 * (size_t)emax;
+}
+
+// ---------------------------------------------------------------- 2D row-march Lanczos pass
+// Each block owns one contiguous range of a trajectory's cells (a few grid rows
+// to tens of rows) and walks it in sub-chunks of SUB cells.  The LDS window
+// holds [cb - n, cb + SUB + n) of v_j (both sequences), c and the raw v_{j-1}
+// (or seed); moving to the next sub-chunk, the 2n cells both windows share are
+// shifted inside LDS and only SUB new cells are read from memory, so every
+// input element is read about once per pass.  The per-trajectory Lanczos
+// scalars are formed once per block and the block writes one set of partial
+// sums for its range.
+template <int NV>
+__device__ inline void block_sum_n(double (&v)[NV], double *sh /* NV * (FBS/64) */) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sh[k * (FBS / kWave) + w] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double t = sh[k * (FBS / kWave)];
+#pragma unroll
+    for (int q = 1; q < FBS / kWave; ++q) t += sh[k * (FBS / kWave) + q];
+    v[k] = t;
+  }
+}
+
+constexpr int MSPT = 4;               // cells per thread per sub-chunk
+constexpr int MSUB = FBS * MSPT;      // sub-chunk cells
+
+template <int STAGE, int RCPT>
+__global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax) {
+  constexpr int range = FBS * RCPT; // cells of the block's row range
+  extern __shared__ float lds[];
+  const int2 cm = a.cmap[blockIdx.x];
+  if (cm.x < 0) return;
+  const TrajDev T = a.traj[cm.x];
+  __shared__ double red[NSEQ * PN];
+  __shared__ double shn[NSEQ * PN * (FBS / kWave)];
+  float cA[NSEQ] = {0.f, 0.f}, cB[NSEQ] = {0.f, 0.f}, cC[NSEQ] = {0.f, 0.f};
+  if (STAGE >= 1) {
+    reduce_partials(a, T, (j - 1) & 1, red);
+#pragma unroll
+    for (int s = 0; s < NSEQ; ++s) {
+      const double q0 = red[s * PN], q1 = red[s * PN + 1], q2 = red[s * PN + 2];
+      if (STAGE == 1) {
+        const double b0 = sqrt(q0);
+        const double al = q0 > 0.0 ? q1 / q0 : 0.0;
+        const double b1sq = q0 > 0.0 ? (q2 - q1 * q1 / q0) / q0 : 0.0;
+        const double b1 = (b1sq > 1e-14 * (q0 > 0.0 ? q2 / q0 : 0.0) && b1sq > 0.0) ? sqrt(b1sq) : 0.0;
+        cA[s] = (float)al;
+        cB[s] = (b0 > 0.0 && b1 > 0.0) ? (float)(1.0 / (b0 * b1)) : 0.0f;
+        cC[s] = b0 > 0.0 ? (float)(b1 / b0) : 0.0f;
+        if (cm.y == 0 && threadIdx.x == 0) {
+          a.scal[cm.x].b0[s] = b0;
+          a.scal[cm.x].alpha[s][0] = al;
+          a.scal[cm.x].beta[s][1] = b1;
+        }
+      } else {
+        double al, b;
+        lanczos_scalars(q0, q1, q2, al, b);
+        cA[s] = (float)al;
+        cB[s] = b > 0.0 ? (float)(1.0 / b) : 0.0f;
+        cC[s] = (float)b;
+        if (cm.y == 0 && threadIdx.x == 0) {
+          a.scal[cm.x].alpha[s][j - 1] = al;
+          a.scal[cm.x].beta[s][j] = b;
+        }
+      }
+    }
+  }
+  const int n = T.n;
+  const int64_t N = T.N, cells = a.cells;
+  float *sv0 = lds, *sv1 = lds + emax, *sc = lds + 2 * emax;
+  float *sp0 = lds + 3 * emax, *sp1 = lds + 4 * emax;
+  const float *W0 = a.W + (size_t)(0 * 2 + ((j - 1) & 1)) * cells;
+  const float *W1 = a.W + (size_t)(1 * 2 + ((j - 1) & 1)) * cells;
+  const float *V0 = a.V + (size_t)(0 * (a.m - 2) + (j - 2)) * cells;
+  const float *V1 = a.V + (size_t)(1 * (a.m - 2) + (j - 2)) * cells;
+  const float *Z0 = a.W + (size_t)(0 * 2) * cells, *Z1 = a.W + (size_t)(1 * 2) * cells;
+  float *Vout0 = a.V + ((size_t)0 * (a.m - 2) + (j - 1)) * cells;
+  float *Vout1 = a.V + ((size_t)1 * (a.m - 2) + (j - 1)) * cells;
+  float *Wout0 = a.W + ((size_t)0 * 2 + (j & 1)) * cells;
+  float *Wout1 = a.W + ((size_t)1 * 2 + (j & 1)) * cells;
+  const float scl = T.s, mu2 = T.mu2;
+  double acc[NSEQ * PN];
+#pragma unroll
+  for (int k = 0; k < NSEQ * PN; ++k) acc[k] = 0.0;
+  const int64_t rbeg = (int64_t)cm.y * range;
+  const int64_t rend = rbeg + range < N ? rbeg + range : N;
+  // Register double-buffering: the loads of sub-chunk k+1 are issued before the
+  // stencil of sub-chunk k runs from LDS, so memory latency overlaps compute.
+  float px0[MSPT], px1[MSPT], py0[MSPT], py1[MSPT], pcc[MSPT];
+  auto fetch = [&](int64_t flo, int cnt) {
+#pragma unroll
+    for (int q = 0; q < MSPT; ++q) {
+      const int e = threadIdx.x + q * FBS;
+      const int64_t g = T.off + flo + (e < cnt ? e : 0);
+      pcc[q] = a.cf[g];
+      if (STAGE == 0) {
+        px0[q] = seed_x(0, a, g);
+        px1[q] = seed_x(1, a, g);
+        py0[q] = py1[q] = 0.f;
+      } else if (STAGE == 1) {
+        px0[q] = Z0[g];
+        px1[q] = Z1[g];
+        py0[q] = seed_x(0, a, g);
+        py1[q] = seed_x(1, a, g);
+      } else {
+        px0[q] = W0[g];
+        px1[q] = W1[g];
+        py0[q] = V0[g];
+        py1[q] = V1[g];
+      }
+    }
+  };
+  auto deposit = [&](int64_t flo, int cnt, int64_t wb) {
+#pragma unroll
+    for (int q = 0; q < MSPT; ++q) {
+      const int e = threadIdx.x + q * FBS;
+      if (e < cnt) {
+        const int w = (int)(flo + e - wb);
+        sc[w] = pcc[q];
+        if (STAGE == 0) {
+          sv0[w] = px0[q];
+          sv1[w] = px1[q];
+        } else {
+          sv0[w] = (px0[q] - cA[0] * py0[q]) * cB[0];
+          sv1[w] = (px1[q] - cA[1] * py1[q]) * cB[1];
+          sp0[w] = py0[q];
+          sp1[w] = py1[q];
+        }
+      }
+    }
+  };
+  // first window: [max(rbeg - n, 0), min(rbeg + SUB + n, N)) in slices of SUB
+  {
+    const int64_t wb = rbeg - n;
+    const int64_t flo0 = wb > 0 ? wb : 0;
+    const int64_t ce0 = rbeg + MSUB < rend ? rbeg + MSUB : rend;
+    const int64_t hi0 = ce0 + n < N ? ce0 + n : N;
+    for (int64_t f = flo0; f < hi0; f += MSUB) {
+      const int cnt = (int)((hi0 - f) < MSUB ? (hi0 - f) : MSUB);
+      fetch(f, cnt);
+      deposit(f, cnt, wb);
+    }
+  }
+  for (int64_t cb = rbeg; cb < rend; cb += MSUB) {
+    const int64_t ce = cb + MSUB < rend ? cb + MSUB : rend;
+    const int64_t wb = cb - n;
+    // prefetch the next window's new cells [ce + n, min(ce + SUB + n, N))
+    const int64_t nce = ce + MSUB < rend ? ce + MSUB : rend;
+    const int64_t nflo = ce + n;
+    const int64_t nhi = nce + n < N ? nce + n : N;
+    const int ncnt = ce < rend && nhi > nflo ? (int)(nhi - nflo) : 0;
+    if (ncnt > 0) fetch(nflo, ncnt);
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < MSPT; ++kk) {
+      const int64_t i = cb + threadIdx.x + (int64_t)kk * FBS;
+      if (i < ce) {
+        const int e = (int)(i - wb);
+        const int64_t r = i / n, col = i - r * n;
+        const float ci = sc[e];
+        const float wu = r > 0 ? 0.5f * (ci + sc[e - n]) : 0.0f;
+        const float wl = col > 0 ? 0.5f * (ci + sc[e - 1]) : 0.0f;
+        const float wr = col < n - 1 ? 0.5f * (ci + sc[e + 1]) : 0.0f;
+        const float wd = r < n - 1 ? 0.5f * (ci + sc[e + n]) : 0.0f;
+        const float wsum = ((wu + wl) + wr) + wd;
+        const int64_t gi = T.off + i;
+#pragma unroll
+        for (int s = 0; s < NSEQ; ++s) {
+          const float *sv = s == 0 ? sv0 : sv1;
+          const float vi = sv[e];
+          float nbsum = 0.0f;
+          if (r > 0) nbsum += wu * sv[e - n];
+          if (col > 0) nbsum += wl * sv[e - 1];
+          if (col < n - 1) nbsum += wr * sv[e + 1];
+          if (r < n - 1) nbsum += wd * sv[e + n];
+          const float Av = scl * (wsum * vi - nbsum) + mu2 * vi;
+          if (STAGE == 0) {
+            (s == 0 ? Wout0 : Wout1)[gi] = Av;
+            acc[s * PN + 0] += (double)vi * vi;
+            acc[s * PN + 1] += (double)vi * Av;
+            acc[s * PN + 2] += (double)Av * Av;
+          } else {
+            const float vprev = (s == 0 ? sp0 : sp1)[e];
+            const float wp = Av - cC[s] * vprev;
+            (s == 0 ? Vout0 : Vout1)[gi] = vi;
+            (s == 0 ? Wout0 : Wout1)[gi] = wp;
+            acc[s * PN + 0] += (double)wp * vi;
+            acc[s * PN + 1] += (double)wp * wp;
+            acc[s * PN + 2] += (double)vi * vi;
+          }
+        }
+      }
+    }
+    if (ce >= rend) break;
+    __syncthreads(); // all stencil reads of this window are done
+    // shift the shared 2n cells to the front, then deposit the prefetched cells
+    for (int e = threadIdx.x; e < 2 * n; e += FBS) {
+      sv0[e] = sv0[MSUB + e];
+      sv1[e] = sv1[MSUB + e];
+      sc[e] = sc[MSUB + e];
+      sp0[e] = sp0[MSUB + e];
+      sp1[e] = sp1[MSUB + e];
+    }
+    __syncthreads();
+    if (ncnt > 0) deposit(nflo, ncnt, wb + MSUB);
+  }
+  block_sum_n<NSEQ * PN>(acc, shn);
+  if (threadIdx.x == 0) {
+    double *dst = a.part + ((size_t)(j & 1) * a.total_chunks + T.chunk0 + cm.y) * (NSEQ * PN);
+#pragma unroll
+    for (int k = 0; k < NSEQ * PN; ++k) dst[k] = acc[k];
+  }
+}
+
+inline size_t march2d_lds_bytes(int nmax) {
+  return sizeof(float) * 5 * (size_t)(MSUB + 2 * nmax);
+}
+
+// Combine over a runtime chunk in groups of 4 cells per thread (independent loads).
+// v_{m-1} = (w'_{m-2} - alpha_{m-2} v_{m-2}) / b_{m-1} is folded into the
+// coefficients: d = y0 x/b0 + sum_{k<m-2} y_k v_k + (y_{m-2} - y_{m-1} alpha/b) v_{m-2}
+// + (y_{m-1}/b) w'_{m-2}, so each stored vector is read exactly once.
+template <int DIM>
+__global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a, int chunk) {
   const int2 cm = a.cmap[blockIdx.x];
   if (cm.x < 0) return;
   const TrajDev T = a.traj[cm.x];
   const TrajScal &S = a.scal[cm.x];
-  __shared__ double ys[NSEQ][kMaxKrylov];
-  if (threadIdx.x < a.m) {
-    ys[0][threadIdx.x] = S.y[0][threadIdx.x];
-    ys[1][threadIdx.x] = S.y[1][threadIdx.x];
+  __shared__ double cv[NSEQ][kMaxKrylov]; // coefficient of V slot k-1 (k = 1..m-2); [0]: seed
+  __shared__ double cw[NSEQ];             // coefficient of w'_{m-2}
+  const int m = a.m;
+  if (threadIdx.x < NSEQ) {
+    const int s = threadIdx.x;
+    const double ib0 = S.b0[s] > 0.0 ? 1.0 / S.b0[s] : 0.0;
+    const double ibl = S.beta[s][m - 1] > 0.0 ? 1.0 / S.beta[s][m - 1] : 0.0;
+    const double al = (double)(float)S.alpha[s][m - 2];
+    cv[s][0] = S.y[s][0] * ib0;
+    for (int k = 1; k <= m - 2; ++k) cv[s][k] = S.y[s][k];
+    cv[s][m - 2] -= S.y[s][m - 1] * al * ibl;
+    cw[s] = S.y[s][m - 1] * ibl;
+  }
+  __syncthreads();
+  constexpr int G = 4;
+  const int n = T.n;
+  const int64_t cells = a.cells;
+  const int64_t cbeg = (int64_t)cm.y * chunk;
+  const int64_t cend = cbeg + chunk < T.N ? cbeg + chunk : T.N;
+  const double t2 = T.tau * T.tau;
+  const float *Wl0 = a.W + ((size_t)0 * 2 + ((m - 2) & 1)) * cells;
+  const float *Wl1 = a.W + ((size_t)1 * 2 + ((m - 2) & 1)) * cells;
+  for (int64_t gb = cbeg; gb < cend; gb += G * FBS) {
+    int64_t gs[G];
+    bool ok[G];
+    double d0[G], d1[G];
+#pragma unroll
+    for (int kk = 0; kk < G; ++kk) {
+      const int64_t i = gb + threadIdx.x + (int64_t)kk * FBS;
+      ok[kk] = i < cend;
+      gs[kk] = T.off + (ok[kk] ? clamp_src<DIM>(i, n) : 0);
+      const double uu = a.u[gs[kk]];
+      const double g1 = -(double)a.mf[gs[kk]] * uu * uu * uu;
+      d0[kk] = cv[0][0] * (double)(float)uu + cw[0] * (double)Wl0[gs[kk]];
+      d1[kk] = cv[1][0] * (double)(float)g1 + cw[1] * (double)Wl1[gs[kk]];
+    }
+#pragma unroll 2
+    for (int k = 1; k <= m - 2; ++k) {
+      const float *Vk0 = a.V + ((size_t)0 * (m - 2) + (k - 1)) * cells;
+      const float *Vk1 = a.V + ((size_t)1 * (m - 2) + (k - 1)) * cells;
+      const double y0 = cv[0][k], y1 = cv[1][k];
+#pragma unroll
+      for (int kk = 0; kk < G; ++kk) {
+        d0[kk] += y0 * (double)Vk0[gs[kk]];
+        d1[kk] += y1 * (double)Vk1[gs[kk]];
+      }
+    }
+#pragma unroll
+    for (int kk = 0; kk < G; ++kk) {
+      const int64_t i = gb + threadIdx.x + (int64_t)kk * FBS;
+      if (ok[kk]) a.un[T.off + i] = 2.0 * a.u[gs[kk]] - a.up[gs[kk]] + 2.0 * d0[kk] + t2 * d1[kk];
+    }
+  }
+}
+
+// Snapshot combine over a runtime chunk: also the velocity (u+ - u)/tau before
+// the ghost copy and both copies of u+ (same folded coefficients as above).
+template <int DIM>
+__global__ __launch_bounds__(FBS) void k_kge_combine_snap(KgeArgs a, SnapArgs snap, int chunk) {
+  const int2 cm = a.cmap[blockIdx.x];
+  if (cm.x < 0) return;
+  const TrajDev T = a.traj[cm.x];
+  const TrajScal &S = a.scal[cm.x];
+  __shared__ double cv[NSEQ][kMaxKrylov];
+  __shared__ double cw[NSEQ];
+  const int m = a.m;
+  if (threadIdx.x < NSEQ) {
+    const int s = threadIdx.x;
+    const double ib0 = S.b0[s] > 0.0 ? 1.0 / S.b0[s] : 0.0;
+    const double ibl = S.beta[s][m - 1] > 0.0 ? 1.0 / S.beta[s][m - 1] : 0.0;
+    const double al = (double)(float)S.alpha[s][m - 2];
+    cv[s][0] = S.y[s][0] * ib0;
+    for (int k = 1; k <= m - 2; ++k) cv[s][k] = S.y[s][k];
+    cv[s][m - 2] -= S.y[s][m - 1] * al * ibl;
+    cw[s] = S.y[s][m - 1] * ibl;
   }
   __syncthreads();
   const int n = T.n;
-  const int64_t base = (int64_t)cm.y * (FBS * CPT);
-#pragma unroll
-  for (int kk = 0; kk < CPT; ++kk) {
-    const int64_t i = base + threadIdx.x + (int64_t)kk * FBS;
-    if (i >= T.N) break;
+  const int64_t cells = a.cells;
+  const double t2 = T.tau * T.tau;
+  auto unew = [&](int64_t g) {
+    const double uu = a.u[g];
+    const double g1 = -(double)a.mf[g] * uu * uu * uu;
+    double d0 = cv[0][0] * (double)(float)uu + cw[0] * (double)a.W[((size_t)0 * 2 + ((m - 2) & 1)) * cells + g];
+    double d1 = cv[1][0] * (double)(float)g1 + cw[1] * (double)a.W[((size_t)1 * 2 + ((m - 2) & 1)) * cells + g];
+    for (int k = 1; k <= m - 2; ++k) {
+      d0 += cv[0][k] * (double)a.V[((size_t)0 * (m - 2) + (k - 1)) * cells + g];
+      d1 += cv[1][k] * (double)a.V[((size_t)1 * (m - 2) + (k - 1)) * cells + g];
+    }
+    return 2.0 * uu - a.up[g] + 2.0 * d0 + t2 * d1;
+  };
+  const int64_t cbeg = (int64_t)cm.y * chunk;
+  const int64_t cend = cbeg + chunk < T.N ? cbeg + chunk : T.N;
+  for (int64_t i = cbeg + threadIdx.x; i < cend; i += FBS) {
     const int64_t src = clamp_src<DIM>(i, n);
     const int64_t gi = T.off + i, gs = T.off + src;
-    const double us = kge_unew<DIM>(a, S, ys[0], ys[1], gs, T.tau);
+    const double us = unew(gs);
     a.un[gi] = us;
-    if (snap.u) {
-      const double ui = src == i ? us : kge_unew<DIM>(a, S, ys[0], ys[1], gi, T.tau);
-      snap.u[gi] = us;
-      snap.v[gi] = (ui - a.u[gi]) / T.tau;
-    }
+    const double ui = src == i ? us : unew(gi);
+    snap.u[gi] = us;
+    snap.v[gi] = (ui - a.u[gi]) / T.tau;
   }
 }
 
@@ -751,6 +1199,8 @@ struct nls_batch {
   int nsnap = 0;
   hipGraphExec_t graphs[3] = {nullptr, nullptr, nullptr};
   bool profiling = false;
+  int pass_kind = 0; // 0 direct, 1 LDS-tiled chunk, 2 row march
+  int nmax = 0;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   double pass_ms = 0.0;
@@ -846,12 +1296,20 @@ struct nls_batch {
     cpt = cpt_;
     const double mu2 = p.mu2;
     lay = make_layout(desc.data(), ntraj, p.dim, cpt, p.dt, mu2);
+    nmax = 0;
+    for (auto &T : lay.traj) nmax = std::max(nmax, T.n);
     NLS_REQUIRE(lay.cmap.size() <= cmap_cap, "internal: chunk map larger than allocated");
     NLS_HIP(hipMemcpyAsync(d_traj, lay.traj.data(), sizeof(TrajDev) * ntraj,
                            hipMemcpyHostToDevice, s));
     NLS_HIP(hipMemcpyAsync(d_cmap, lay.cmap.data(), sizeof(int2) * lay.cmap.size(),
                            hipMemcpyHostToDevice, s));
     NLS_HIP(hipStreamSynchronize(s));
+  }
+
+  int nmax_of() const {
+    int x = 0;
+    for (auto &d : desc) x = std::max(x, d.n);
+    return x;
   }
 
   hipEvent_t next_event() {
@@ -863,23 +1321,45 @@ struct nls_batch {
     return ev_pool[ev_used++];
   }
 
-  template <int DIM, int CPT> void launch_step_t(int cur, bool snapshot, int slot, bool prof) {
+  // Pass kernels: kind 0 = direct global-memory stencil (any dim), 1 = LDS-tiled
+  // chunk + halo rows (2D), 2 = row march with an LDS window (2D).
+  template <int DIM, int STAGE> void launch_pass(const KgeArgs &a, int j) {
+    const dim3 grid((unsigned)lay.cmap.size()), blk(FBS);
+    if (pass_kind == 2) {
+      const size_t lds = march2d_lds_bytes(nmax);
+      const int emax = MSUB + 2 * nmax;
+      if (cpt == 16) hipLaunchKernelGGL((k_kge_march2d<STAGE, 16>), grid, blk, lds, s, a, j, emax);
+      else if (cpt == 32) hipLaunchKernelGGL((k_kge_march2d<STAGE, 32>), grid, blk, lds, s, a, j, emax);
+      else hipLaunchKernelGGL((k_kge_march2d<STAGE, 64>), grid, blk, lds, s, a, j, emax);
+    } else if (pass_kind == 1) {
+      const size_t lds = pass2d_lds_bytes(cpt, nmax);
+      const int emax = FBS * cpt + 2 * nmax;
+      if (cpt == 4) hipLaunchKernelGGL((k_kge_pass2d<STAGE, 4>), grid, blk, lds, s, a, j, emax);
+      else hipLaunchKernelGGL((k_kge_pass2d<STAGE, 8>), grid, blk, lds, s, a, j, emax);
+    } else {
+      if (cpt == 4) hipLaunchKernelGGL((k_kge_pass<DIM, STAGE, 4>), grid, blk, 0, s, a, j);
+      else if (cpt == 8) hipLaunchKernelGGL((k_kge_pass<DIM, STAGE, 8>), grid, blk, 0, s, a, j);
+      else hipLaunchKernelGGL((k_kge_pass<DIM, STAGE, 16>), grid, blk, 0, s, a, j);
+    }
+  }
+
+  template <int DIM> void launch_step_t(int cur, bool snapshot, int slot, bool prof) {
     const KgeArgs a = args(cur);
     const dim3 grid((unsigned)lay.cmap.size()), blk(FBS);
-    hipLaunchKernelGGL((k_kge_pass<DIM, 0, CPT>), grid, blk, 0, s, a, 0);
-    if (m - 2 >= 1) hipLaunchKernelGGL((k_kge_pass<DIM, 1, CPT>), grid, blk, 0, s, a, 1);
+    launch_pass<DIM, 0>(a, 0);
+    if (m - 2 >= 1) launch_pass<DIM, 1>(a, 1);
     for (int j = 2; j <= m - 2; ++j) {
       if (prof) NLS_HIP(hipEventRecord(next_event(), s));
-      hipLaunchKernelGGL((k_kge_pass<DIM, 2, CPT>), grid, blk, 0, s, a, j);
+      launch_pass<DIM, 2>(a, j);
       if (prof) NLS_HIP(hipEventRecord(next_event(), s));
     }
     hipLaunchKernelGGL(k_kge_small, dim3(ntraj * NSEQ), dim3(kWave), 0, s, a, m - 2);
-    SnapArgs sn{nullptr, nullptr};
     if (snapshot) {
-      sn.u = d_snapU + (size_t)slot * lay.cells;
-      sn.v = d_snapV + (size_t)slot * lay.cells;
+      SnapArgs sn{d_snapU + (size_t)slot * lay.cells, d_snapV + (size_t)slot * lay.cells};
+      hipLaunchKernelGGL((k_kge_combine_snap<DIM>), grid, blk, 0, s, a, sn, FBS * cpt);
+    } else {
+      hipLaunchKernelGGL((k_kge_combine_rt<DIM>), grid, blk, 0, s, a, FBS * cpt);
     }
-    hipLaunchKernelGGL((k_kge_combine<DIM, CPT>), grid, blk, 0, s, a, sn);
     check_launch("fused kge step");
   }
 
@@ -896,29 +1376,35 @@ struct nls_batch {
       }
       return;
     }
-    if (p.dim == 2) {
-      if (cpt == 4) launch_step_t<2, 4>(cur, snapshot, slot, prof);
-      else if (cpt == 8) launch_step_t<2, 8>(cur, snapshot, slot, prof);
-      else launch_step_t<2, 16>(cur, snapshot, slot, prof);
-    } else {
-      if (cpt == 4) launch_step_t<3, 4>(cur, snapshot, slot, prof);
-      else if (cpt == 8) launch_step_t<3, 8>(cur, snapshot, slot, prof);
-      else launch_step_t<3, 16>(cur, snapshot, slot, prof);
-    }
+    if (p.dim == 2) launch_step_t<2>(cur, snapshot, slot, prof);
+    else launch_step_t<3>(cur, snapshot, slot, prof);
   }
 
   // Tile autotuner: times one step per cells-per-thread candidate on the live
   // buffers (a step only writes the next-state buffer and Krylov scratch, so
   // timing runs leave the trajectory untouched) and keeps the fastest.
+  bool kind_ok(int kind, int c) const {
+    const int nm = nmax_of();
+    if (kind == 2) return p.dim == 2 && 2 * nm <= MSUB && march2d_lds_bytes(nm) <= 65536 &&
+                          (c == 16 || c == 32 || c == 64);
+    if (kind == 1) return p.dim == 2 && (c == 4 || c == 8) && pass2d_lds_bytes(c, nm) <= 65536;
+    return c == 4 || c == 8 || c == 16;
+  }
+
+  // Tile autotuner: times steps per (pass kernel, tile) candidate on the live
+  // buffers (a step only writes the next-state buffer and Krylov scratch, so
+  // timing runs leave the trajectory untouched) and keeps the fastest.
   void autotune() {
-    const int cands[3] = {4, 8, 16};
+    const int cands[8][2] = {{2, 32}, {2, 16}, {2, 64}, {1, 4}, {1, 8}, {0, 4}, {0, 8}, {0, 16}};
     float best = 1e30f;
-    int best_cpt = 8;
+    int best_cpt = 4, best_kind = 0;
     hipEvent_t e0, e1;
     NLS_HIP(hipEventCreate(&e0));
     NLS_HIP(hipEventCreate(&e1));
-    for (int c : cands) {
-      set_layout(c);
+    for (auto &cd : cands) {
+      if (!kind_ok(cd[0], cd[1])) continue;
+      pass_kind = cd[0];
+      set_layout(cd[1]);
       launch_step((int)(step % nstates()), false, 0, false); // warm-up
       NLS_HIP(hipEventRecord(e0, s));
       for (int r = 0; r < 3; ++r) launch_step((int)(step % nstates()), false, 0, false);
@@ -926,10 +1412,11 @@ struct nls_batch {
       NLS_HIP(hipEventSynchronize(e1));
       float ms = 0;
       NLS_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best) { best = ms; best_cpt = c; }
+      if (ms < best) { best = ms; best_cpt = cd[1]; best_kind = cd[0]; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    pass_kind = best_kind;
     set_layout(best_cpt);
     tuned = true;
   }
@@ -958,7 +1445,10 @@ struct nls_batch {
   void run(int nsteps, float *ms) {
     if (!tuned) {
       if (p.cells_per_thread > 0) {
-        set_layout(p.cells_per_thread);
+        const int c = p.cells_per_thread;
+        pass_kind = kind_ok(2, c) ? 2 : kind_ok(1, c) ? 1 : 0;
+        NLS_REQUIRE(kind_ok(pass_kind, c), "cells_per_thread not supported for this batch");
+        set_layout(c);
         tuned = true;
       } else {
         autotune();
@@ -1069,8 +1559,9 @@ int nls_batch_create(const nls_batch_params *p, int ntraj, const nls_traj_desc *
     NLS_REQUIRE(p->dim == 2 || p->dim == 3, "dim must be 2 or 3");
     NLS_REQUIRE(p->dt > 0, "dt must be positive");
     NLS_REQUIRE(p->cells_per_thread == 0 || p->cells_per_thread == 4 ||
-                    p->cells_per_thread == 8 || p->cells_per_thread == 16,
-                "cells_per_thread must be 0 (autotune), 4, 8 or 16");
+                    p->cells_per_thread == 8 || p->cells_per_thread == 16 ||
+                    p->cells_per_thread == 32 || p->cells_per_thread == 64,
+                "cells_per_thread must be 0 (autotune), 4, 8, 16, 32 or 64");
     for (int t = 0; t < ntraj; ++t) NLS_REQUIRE(trajs[t].n >= 3 && trajs[t].L > 0, "bad trajectory");
     auto b = std::make_unique<nls_batch>();
     b->p = *p;
@@ -1219,6 +1710,14 @@ int nls_batch_info(nls_batch *b, int *krylov_dim, int64_t *cells, size_t *device
     if (krylov_dim) *krylov_dim = b->m;
     if (cells) *cells = b->total_cells();
     if (device_bytes) *device_bytes = b->bytes;
+  });
+}
+
+int nls_batch_tile(nls_batch *b, int *cells_per_thread, int *lds_tiled) {
+  return guard([&] {
+    NLS_REQUIRE(b, "null batch");
+    if (cells_per_thread) *cells_per_thread = b->cpt;
+    if (lds_tiled) *lds_tiled = b->pass_kind;
   });
 }
 

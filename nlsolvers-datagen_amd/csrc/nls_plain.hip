@@ -1032,4 +1032,41 @@ int nls_philox_normals(uint64_t seed, uint32_t stream_id, uint32_t step, int64_t
   });
 }
 
+// STREAM-style copy (float4 loads/stores) used as the achievable-bandwidth
+// reference next to the roofline figures and to calibrate the HBM counters.
+__global__ __launch_bounds__(BS) void k_stream_copy(int64_t n4, const float4 *__restrict__ x,
+                                                    float4 *__restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < n4; i += (int64_t)gridDim.x * BS)
+    y[i] = x[i];
+}
+
+int nls_stream_copy_bench(size_t bytes, int reps, double *gbps, double *avg_ms) {
+  return guard([&] {
+    NLS_REQUIRE(bytes >= 1024 && reps >= 1 && gbps, "nls_stream_copy_bench: bad arguments");
+    const int64_t n4 = (int64_t)(bytes / 16);
+    DevBuf a(16 * n4), b(16 * n4);
+    NLS_HIP(hipMemset(a.p, 0, 16 * n4));
+    hipStream_t st;
+    NLS_HIP(hipStreamCreate(&st));
+    const unsigned grid = (unsigned)std::min<int64_t>(blocks_for(n4, BS), 256 * 64);
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(BS), 0, st, n4, a.as<float4>(), b.as<float4>());
+    hipEvent_t e0, e1;
+    NLS_HIP(hipEventCreate(&e0));
+    NLS_HIP(hipEventCreate(&e1));
+    NLS_HIP(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(BS), 0, st, n4, a.as<float4>(), b.as<float4>());
+    NLS_HIP(hipEventRecord(e1, st));
+    NLS_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    NLS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    check_launch("stream copy");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    *gbps = 2.0 * 16.0 * n4 * reps / (ms / 1e3) / 1e9;
+    if (avg_ms) *avg_ms = ms / reps;
+  });
+}
+
 } // extern "C"

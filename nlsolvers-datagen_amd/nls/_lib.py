@@ -131,6 +131,7 @@ _SIGS = {
     "nls_sp4_transfer_snapshots": [_P, _D, _D],
     "nls_sp4_run": [_P],
     "nls_philox_normals": [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64, _D],
+    "nls_stream_copy_bench": [C.c_size_t, C.c_int, _D, _D],
     "nls_batch_create": [C.POINTER(BatchParams), C.c_int, C.POINTER(TrajDesc),
                          C.POINTER(C.c_void_p)],
     "nls_batch_destroy": [_P],
@@ -140,6 +141,7 @@ _SIGS = {
     "nls_batch_get_state": [_P, C.c_int, _D, _D],
     "nls_batch_info": [_P, C.POINTER(C.c_int), _I64, C.POINTER(C.c_size_t)],
     "nls_batch_set_profiling": [_P, C.c_int],
+    "nls_batch_tile": [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "nls_batch_pass_timing": [_P, _D, C.POINTER(C.c_int), _D],
     "nls_batch_estimate_bytes": [C.POINTER(BatchParams), C.c_int, C.POINTER(TrajDesc),
                                  C.POINTER(C.c_size_t)],

@@ -94,6 +94,12 @@ class FusedBatch:
         call("nls_batch_info", self.h, C.byref(m), C.byref(cells), C.byref(b))
         return {"krylov_dim": m.value, "cells": cells.value, "device_bytes": b.value}
 
+    def tile(self):
+        cpt = C.c_int()
+        lds = C.c_int()
+        call("nls_batch_tile", self.h, C.byref(cpt), C.byref(lds))
+        return {"cells_per_thread": cpt.value, "pass_kind": lds.value}
+
     def set_profiling(self, on):
         call("nls_batch_set_profiling", self.h, int(bool(on)))
 

@@ -26,13 +26,16 @@ def make_case(n, seed, traj, L=10.0, ring=False):
     return m, c, u0, v0
 
 
-def test_fused_kge_uneven_batch_vs_oracle(orc):
+@pytest.mark.parametrize("cpt", [4, 32])  # LDS-tiled chunk / row-march pass kernels
+def test_fused_kge_uneven_batch_vs_oracle(orc, cpt):
     from nls.batch import FusedBatch
-    ns_list = [48, 64, 33]
+    ns_list = [48, 64, 33, 130]
     L, dt, nt, ns = 10.0, 0.005, 201, 5
     every = nt // ns  # the reference cadence: freq = nt / num_snapshots
     b = FusedBatch(ns_list, L, dt, krylov_dim=10, mu2=1.0, snapshot_every=every,
-                   max_snapshots=ns, cells_per_thread=8)
+                   max_snapshots=ns, cells_per_thread=cpt)
+    b.run(0)  # settles the tile
+    assert b.tile()["pass_kind"] == {4: 1, 32: 2}[cpt]
     cases = [make_case(n, 2, t, ring=(t == 1)) for t, n in enumerate(ns_list)]
     for t, (m, c, u0, v0) in enumerate(cases):
         b.set_initial(t, m, c, u0, v0)
@@ -68,18 +71,19 @@ def test_fused_kge_3d_vs_oracle(orc, method):
             assert rel_l2(u[k], ur[k]) < TOL, (q6, k)
 
 
-def test_fused_alone_equals_in_batch_bitwise():
+@pytest.mark.parametrize("cpt", [8, 32])
+def test_fused_alone_equals_in_batch_bitwise(cpt):
     from nls.batch import FusedBatch
     L, dt, steps = 10.0, 0.005, 40
-    mix = [64, 40, 96, 64]
+    mix = [64, 40, 96, 64, 300]
     target = 2
     cases = [make_case(n, 4, t) for t, n in enumerate(mix)]
-    b = FusedBatch(mix, L, dt, krylov_dim=8, mu2=1.0, cells_per_thread=8)
+    b = FusedBatch(mix, L, dt, krylov_dim=8, mu2=1.0, cells_per_thread=cpt)
     for t, cs in enumerate(cases):
         b.set_initial(t, *cs)
     b.run(steps)
     u_mix, _ = b.state(target)
-    a = FusedBatch([mix[target]], L, dt, krylov_dim=8, mu2=1.0, cells_per_thread=8)
+    a = FusedBatch([mix[target]], L, dt, krylov_dim=8, mu2=1.0, cells_per_thread=cpt)
     a.set_initial(0, *cases[target])
     a.run(steps)
     u_alone, _ = a.state(0)
@@ -93,7 +97,7 @@ def test_fused_graph_replay_equals_eager():
     cases = [make_case(n, 5, t) for t, n in enumerate(ns_list)]
     out = []
     for graph in (True, False):
-        b = FusedBatch(ns_list, L, dt, krylov_dim=9, mu2=1.0, use_graph=graph, cells_per_thread=4)
+        b = FusedBatch(ns_list, L, dt, krylov_dim=9, mu2=1.0, use_graph=graph, cells_per_thread=16)
         for t, cs in enumerate(cases):
             b.set_initial(t, *cs)
         for _ in range(3):  # several replays with host synchronisation between them
