@@ -23,6 +23,10 @@ sys.path.insert(0, os.path.join(ROOT, "nlsolvers-datagen_amd"))
 
 METRIC = "KGE-2D 300² cell-steps/s batched vs plain at relL2≤1e-5; p50/p99 traj latency"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the dominant pass, measured with rocprofv3 counters on this
+# workload: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB copy kernel in the
+# same runs).  Keyed by (kernel, cells per launch); reported only when the tile matches.
+MEASURED_TRAFFIC = {("k_kge_march2d<2, 16>", 9_000_000): 351.9e6}
 
 
 def log(*a):
@@ -126,6 +130,59 @@ def cpu_baseline(args, inputs, krylov):
                       f"QL), one trajectory per OpenMP thread, {wall:.1f} s"}
 
 
+def serve_run(args):
+    """Serving leg: stochastic 2D phi^4 requests (u_tt = Lu + u - u^3 - 0.1 u_t + sigma xi,
+    side 20) arriving as a Poisson process; grid n from {64, 128, 256, 512}, nt from
+    U{1000..8000}, sigma from U(0.05, 0.5), own Philox seed, 10 snapshots each, all drawn
+    from seed 5.  Reports completion - submission latency percentiles and the aggregate
+    cell-steps/s over the window from the first submission to the last completion."""
+    from nls.server import TrajectoryServer
+    rng = np.random.default_rng(5)
+    k = args.serve_requests
+    gaps = rng.exponential(1.0 / args.serve_rate, k)
+    sizes = rng.choice([64, 128, 256, 512], k)
+    nts = rng.integers(1000, 8001, k)
+    sigmas = rng.uniform(0.05, 0.5, k)
+    srv = TrajectoryServer(hbm_budget_bytes=int(args.serve_budget_gib * (1 << 30)),
+                           max_active=args.serve_max_active, L_=10.0, mode="damped", gamma=0.1)
+    w = srv.submit(64, 50, args.dt, 0.1, 1, 0)  # warm the worker and the kernel
+    srv.wait(w)
+    srv.release(w)
+    base = srv.stats()
+    t0 = time.perf_counter()
+    nxt = t0
+    ids = []
+    for i in range(k):
+        nxt += gaps[i]
+        d = nxt - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        ids.append(srv.submit(int(sizes[i]), int(nts[i]), args.dt, float(sigmas[i]),
+                              10_000 + i, 10))
+    srv.drain(args.serve_timeout)
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    lat = []
+    for r in ids:
+        lat.append(srv.fetch(r)["latency_ms"])
+        srv.release(r)
+    srv.close()
+    lat = np.array(lat)
+    cell_steps = st["cell_steps"] - base["cell_steps"]
+    offered = float(np.sum(sizes.astype(np.float64) ** 2 * (nts - 1)))
+    return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
+            "requests": k, "completed": int(st["completed"] - base["completed"]),
+            "failed": int(st["failed"]), "arrival_rate_per_s": args.serve_rate,
+            "cell_steps_per_s": cell_steps / wall, "offered_cell_steps": offered,
+            "window_s": wall, "busy_s": st["busy_s"] - base["busy_s"],
+            "peak_active": st["peak_active"], "peak_bytes": st["peak_bytes"],
+            "hbm_budget_bytes": int(args.serve_budget_gib * (1 << 30)),
+            "workload": f"stochastic phi^4 2D serving, Poisson {args.serve_rate:g} req/s, "
+                        f"{k} requests, n in {{64,128,256,512}}, nt in U{{1000..8000}}, "
+                        f"dt={args.dt}, gamma=0.1, sigma in U(0.05,0.5), 10 snapshots each"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,11 +202,17 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-parity", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--serve-budget-gib", type=float, default=16.0)
+    ap.add_argument("--serve-max-active", type=int, default=256)
+    ap.add_argument("--serve-timeout", type=float, default=300.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not "
                                    "supported (trajectories would shard across GPUs with no "
-                                   "communication, see DESIGN.md)"}))
+                                   "communication between them)"}))
         sys.exit(2)
     from nls._lib import load
     load()
@@ -181,12 +244,22 @@ def main():
         log("[bench] parity vs FP64 oracle")
         err, psteps = parity_check(args, inputs, info["krylov_dim"])
         log(f"[bench] rel-L2 after {psteps} steps: {err:.2e}")
+    serve = None
+    if not args.skip_serve:
+        log(f"[bench] serving: {args.serve_requests} phi^4 requests at {args.serve_rate:g}/s")
+        serve = serve_run(args)
+        log(f"[bench] serving: p50 {serve['p50_ms']:.1f} ms, p99 {serve['p99_ms']:.1f} ms, "
+            f"{serve['cell_steps_per_s']:.3e} cell-steps/s, peak {serve['peak_active']} active")
     cpu = None
     if not args.skip_cpu:
         log("[bench] CPU baseline (oracle)")
         cpu = cpu_baseline(args, inputs, info["krylov_dim"])
         log(f"[bench] cpu: {cpu['value']:.3e} cell-steps/s on {cpu['cores']} threads")
     m = info["krylov_dim"]
+    kname = {2: f"k_kge_march2d<2, {info['cells_per_thread']}>",
+             1: f"k_kge_pass2d<2, {info['cells_per_thread']}>",
+             0: f"k_kge_pass<2, 2, {info['cells_per_thread']}>"}[info["pass_kind"]]
+    traffic = MEASURED_TRAFFIC.get((kname, cells))
     line = {
         "metric": METRIC,
         "value": value,
@@ -207,12 +280,11 @@ def main():
                    "cells": cells, "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-                     "traffic": None,
-                     "kernel": {2: f"k_kge_march2d<2, {info['cells_per_thread']}> (row-march fused "
-                                   "stencil+Lanczos pass)",
-                                1: "k_kge_pass2d<2,CPT> (LDS-tiled fused stencil+Lanczos pass)",
-                                0: "k_kge_pass<2,2,CPT> (fused stencil+Lanczos pass)"}[
-                                    info["pass_kind"]],
+                     "traffic": traffic,
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
+                     "kernel": kname + {2: " (row-march fused stencil+Lanczos pass)",
+                                        1: " (LDS-tiled fused stencil+Lanczos pass)",
+                                        0: " (fused stencil+Lanczos pass)"}[info["pass_kind"]],
                      "bytes_per_launch": pt["bytes_per_launch"],
                      "avg_launch_ms": pt["avg_ms"], "launches_timed": pt["launches"],
                      "stream_copy_gbs": stream_gbs},
@@ -221,7 +293,7 @@ def main():
                           "at a time", "steps": plain_steps},
         "speedup_vs_plain": (value / plain_rate) if plain_rate else None,
         "rel_l2_vs_fp64_oracle": err,
-        "latency": None,
+        "latency": serve,
         "cpu_baseline": cpu,
         "wall_s_timed": wall,
     }
