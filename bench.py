@@ -46,7 +46,8 @@ def fused_run(args, inputs):
     from nls.batch import FusedBatch
     n, L = args.n, args.L
     b = FusedBatch([n] * args.ntraj, L, args.dt, krylov_dim=args.krylov, mu2=1.0,
-                   use_graph=not args.no_graph, cells_per_thread=args.cpt)
+                   use_graph=not args.no_graph, cells_per_thread=args.cpt,
+                   pass_kernel=None if args.kernel < 0 else args.kernel)
     for t, (m, c, u0, v0) in enumerate(inputs):
         b.set_initial(t, m, c, u0, v0)
     b.run(1)  # autotune + graph capture
@@ -194,6 +195,9 @@ def main():
     ap.add_argument("--dt", type=float, default=0.005)
     ap.add_argument("--krylov", type=int, default=0, help="0 = autotuner")
     ap.add_argument("--cpt", type=int, default=0, help="cells per thread, 0 = autotuner")
+    ap.add_argument("--kernel", type=int, default=-1,
+                    help="pass kernel: -1 = autotuner, 0 direct, 1 LDS chunk, 2 row march, "
+                         "3 row march storing only the basis")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--prof-steps", type=int, default=10)
     ap.add_argument("--plain-steps", type=int, default=40)
@@ -256,7 +260,8 @@ def main():
         cpu = cpu_baseline(args, inputs, info["krylov_dim"])
         log(f"[bench] cpu: {cpu['value']:.3e} cell-steps/s on {cpu['cores']} threads")
     m = info["krylov_dim"]
-    kname = {2: f"k_kge_march2d<2, {info['cells_per_thread']}>",
+    kname = {3: "k_kge_rmarch2d<3>",
+             2: "k_kge_march2d<2>",
              1: f"k_kge_pass2d<2, {info['cells_per_thread']}>",
              0: f"k_kge_pass<2, 2, {info['cells_per_thread']}>"}[info["pass_kind"]]
     traffic = MEASURED_TRAFFIC.get((kname, cells))
@@ -282,7 +287,9 @@ def main():
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": traffic,
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
-                     "kernel": kname + {2: " (row-march fused stencil+Lanczos pass)",
+                     "kernel": kname + {3: " (row-march fused stencil+Lanczos pass, "
+                                           "w' recomputed, basis-only stores)",
+                                        2: " (row-march fused stencil+Lanczos pass)",
                                         1: " (LDS-tiled fused stencil+Lanczos pass)",
                                         0: " (fused stencil+Lanczos pass)"}[info["pass_kind"]],
                      "bytes_per_launch": pt["bytes_per_launch"],

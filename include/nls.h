@@ -243,10 +243,16 @@ typedef struct {
   int use_graph;     /* capture steps into HIP graphs */
   int snapshot_every;/* steps between snapshots (0 = none kept on device) */
   int max_snapshots; /* snapshot slots per trajectory */
-  int cells_per_thread; /* tile: cells per workgroup / 256 in {4, 8, 16, 32, 64}; 0 =
-                           autotune.  A fixed tile makes every trajectory's reduction
+  int cells_per_thread; /* tile: cells per workgroup / 256, in [4, 64]; 0 = autotune.
+                           The tiled pass kernels take 4, 8 or 16; the row-march kernels
+                           any value (each trajectory is cut into equal ranges of about
+                           that size).  A fixed tile makes every trajectory's reduction
                            order, hence its result, independent of the other
                            trajectories in the batch */
+  int pass_kernel;   /* 0 = chosen with the tile (autotuner, or the preferred kernel for a
+                        fixed tile); k + 1 = use pass kernel k only (0 direct stencil,
+                        1 LDS-tiled chunk, 2 row march, 3 row march storing only the
+                        basis), autotuning its tile when cells_per_thread is 0 */
 } nls_batch_params;
 typedef struct {
   int n;             /* full side */
@@ -278,11 +284,62 @@ NLS_API int nls_batch_set_profiling(nls_batch *b, int on);
 NLS_API int nls_batch_pass_timing(nls_batch *b, double *avg_ms, int *launches,
                                   double *bytes_per_launch);
 /* Tile in use after the first nls_batch_run: cells per thread and the pass
- * kernel (0 direct stencil, 1 LDS-tiled chunk, 2 row march with LDS window). */
+ * kernel (0 direct stencil, 1 LDS-tiled chunk, 2 row march with LDS window, 3 row
+ * march storing only the basis vectors, w' recomputed). */
 NLS_API int nls_batch_tile(nls_batch *b, int *cells_per_thread, int *pass_kind);
 /* Device bytes a batch with these trajectories would hold (admission control). */
 NLS_API int nls_batch_estimate_bytes(const nls_batch_params *p, int ntraj,
                                      const nls_traj_desc *trajs, size_t *bytes);
+
+/* ---------------------------------------------------------------- HDF5 trajectory files */
+/* One file per trajectory in the launchers' layout: groups metadata (problem_type,
+ * boundary_condition = "noflux", run_id, run_index, timestamp, elapsed_time,
+ * phenomenon, phenomenon_<key>), grid (nx, ny[, nz], Lx, Ly[, Lz]), time (T, nt,
+ * num_snapshots), initial_condition/u0[, v0], focusing/m (attr type) and c (real
+ * runs: focusing/c; complex runs: root c), root u[, v], X, Y[, Z].  Replaces
+ * Launcher.save_to_hdf5 (scripts/real_launcher_2d.py:201-245,
+ * scripts/complex_launcher_2d.py:193-236).  libhdf5 is loaded at run time
+ * ($NLS_HDF5_LIB, libhdf5.so, libhdf5.so.103, /opt/conda/lib/libhdf5.so); without it
+ * these calls fail with a message and the rest of the library is unaffected. */
+NLS_API int nls_h5_available(char *version, int len);
+typedef struct {
+  const char *path;
+  int complex_field;   /* 1: NLSE run (u0, u complex128; no v0, v) */
+  int dim;             /* 2 or 3 */
+  int nx, ny, nz;      /* solver grid (nz = 1 in 2D) */
+  double Lx, Ly, Lz;
+  double T;
+  int nt, num_snapshots;
+  const char *problem_type, *run_id, *phenomenon, *m_type;
+  const char *timestamp; /* NULL: current local time */
+  int run_index;
+  double elapsed_time;
+  int n_params;        /* metadata attributes phenomenon_<key> = value (strings) */
+  const char *const *param_keys;
+  const char *const *param_values;
+  const double *u0, *v0, *m, *c; /* solver-grid arrays; v0 ignored for complex runs */
+  int ns_out, ox, oy, oz;         /* stored trajectory shape (after any downsampling) */
+  const double *u, *v;            /* ns_out * ox * oy * oz values (u complex128 if complex) */
+} nls_h5_run;
+/* Writes the file now. */
+NLS_API int nls_h5_write_run(const nls_h5_run *r);
+/* Asynchronous writer: submit copies the arrays and returns (waiting only while
+ * more than max_queued_bytes are queued); a background thread writes the files in
+ * submission order.  drain waits for the queue and reports the first failure. */
+typedef struct nls_h5_writer nls_h5_writer;
+NLS_API int nls_h5_writer_create(size_t max_queued_bytes, nls_h5_writer **out);
+NLS_API int nls_h5_writer_submit(nls_h5_writer *w, const nls_h5_run *r);
+NLS_API int nls_h5_writer_drain(nls_h5_writer *w, int64_t *files_written);
+NLS_API int nls_h5_writer_destroy(nls_h5_writer *w);
+/* Read-back: dataset `name` (e.g. "u", "focusing/m") as doubles (complex: interleaved
+ * re, im, *is_complex = 1); dims receives up to 4 extents.  cap = capacity of buf in
+ * doubles (buf may be NULL to query dims and type). */
+NLS_API int nls_h5_read_dataset(const char *path, const char *name, double *buf, int64_t cap,
+                                int64_t *dims, int *ndim, int *is_complex);
+/* Attribute `name` of object `obj` ("metadata", "grid", ...): numeric into *num, or
+ * string into str (len bytes, NUL-terminated); *is_string tells which. */
+NLS_API int nls_h5_read_attr(const char *path, const char *obj, const char *name, double *num,
+                             char *str, int len, int *is_string);
 
 /* ---------------------------------------------------------------- serving loop */
 /* Continuous batching of stochastic phi^4 trajectory requests of different
@@ -301,6 +358,10 @@ typedef struct {
   int mode;                /* NLS_SP4_DAMPED or NLS_SP4_REFERENCE */
   double gamma;            /* damping */
   int max_steps_per_group; /* steps launched between admission checks (0 = 64) */
+  const char *h5_dir;      /* if set: each request streams its snapshots (u and
+                              v = (u_k - u_{k-1}) / dt) to h5_dir/run_<run_id>_<id:04d>.h5
+                              in the launchers' layout, written by a background thread */
+  const char *run_id;      /* file name / metadata run_id (NULL: "serve") */
 } nls_server_params;
 typedef struct {
   int n;             /* grid side */

@@ -52,6 +52,7 @@ constexpr int NXCD = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin
 
 struct TrajDev {
   int n, dim, nchunks, chunk0;
+  int range; // cells per chunk (one workgroup's contiguous range)
   int64_t off, N;
   float s, mu2;
   double tau;
@@ -535,15 +536,393 @@ __device__ inline void block_sum_n(double (&v)[NV], double *sh /* NV * (FBS/64) 
 constexpr int MSPT = 4;               // cells per thread per sub-chunk
 constexpr int MSUB = FBS * MSPT;      // sub-chunk cells
 
-template <int STAGE, int RCPT>
-__global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax) {
-  constexpr int range = FBS * RCPT; // cells of the block's row range
+inline size_t march2d_lds_bytes(int nmax) {
+  return sizeof(float) * 5 * (size_t)(MSUB + 2 * nmax);
+}
+
+// ---------------------------------------------------------------- 2D row march, recomputed w'
+// Variant of the row march that stores only the basis vectors.  Pass j re-reads
+// v_{j-1} (two grid rows of halo), v_{j-2} (one row) and c (two rows), recomputes
+//   w'_{j-1} = A v_{j-1} - b_{j-1} v_{j-2}
+// at the cells it needs, forms v_j = (w'_{j-1} - alpha_{j-1} v_{j-1}) / b_j in LDS
+// (same float expressions as the stored-w' passes), applies A and emits the
+// partials of w'_j = A v_j - b_j v_{j-1}.  Per cell, both sequences: read
+// v_{j-1}, v_{j-2}, c (20 B), write v_j (8 B) = 28 B, against 36 B when w' is
+// stored.  Pass 0 stores nothing (pass 1 recomputes its A x); the last pass also
+// stores w'_{m-2} for the combine.
+// Blocks with an odd range index march downward (alt != 0): the halo rows that
+// two neighbouring ranges share are then read by both blocks at about the same
+// time, so the second read is served by the XCD's L2 instead of HBM.
+// STAGE 0: j = 0 on the seeds x (partials <x,x>, <x,Ax>, <Ax,Ax>);
+// STAGE 1: j = 1 (P = x, no Q); STAGE 2: j = 2 (P = v_1, Q = x);
+// STAGE 3: j >= 3 (P = v_{j-1}, Q = v_{j-2}).
+struct Sten5 {
+  float wu, wl, wr, wd, wsum;
+  bool hu, hl, hr, hd;
+};
+
+// Row of cell i (0 <= i < 2^24) from a float reciprocal of n, corrected to the
+// exact quotient.
+__device__ inline int row_of(int i, int n, float rn) {
+  int r = (int)((float)i * rn);
+  if (r * n > i) --r;
+  else if ((r + 1) * n <= i) ++r;
+  return r;
+}
+
+// Weights (c_i + c_nb)/2 of cell i (grid row/column from i) read from an LDS
+// window at e, neighbour offsets (up, left, right, down) in window coordinates.
+__device__ inline Sten5 sten5(const float *sc, int e, int oU, int oL, int oR, int oD, int i,
+                              int n, float rn) {
+  const int r = row_of(i, n, rn), col = i - r * n;
+  Sten5 w;
+  w.hu = r > 0;
+  w.hl = col > 0;
+  w.hr = col < n - 1;
+  w.hd = r < n - 1;
+  const float ci = sc[e];
+  w.wu = w.hu ? 0.5f * (ci + sc[e + oU]) : 0.0f;
+  w.wl = w.hl ? 0.5f * (ci + sc[e + oL]) : 0.0f;
+  w.wr = w.hr ? 0.5f * (ci + sc[e + oR]) : 0.0f;
+  w.wd = w.hd ? 0.5f * (ci + sc[e + oD]) : 0.0f;
+  w.wsum = ((w.wu + w.wl) + w.wr) + w.wd;
+  return w;
+}
+
+__device__ inline float apply5(const Sten5 &w, const float *v, int e, int oU, int oL, int oR,
+                               int oD, float scl, float mu2) {
+  const float vi = v[e];
+  float nb = 0.0f;
+  if (w.hu) nb += w.wu * v[e + oU];
+  if (w.hl) nb += w.wl * v[e + oL];
+  if (w.hr) nb += w.wr * v[e + oR];
+  if (w.hd) nb += w.wd * v[e + oD];
+  return scl * (w.wsum * vi - nb) + mu2 * vi;
+}
+
+constexpr int RKS = 6; // window shift registers: supports n <= RKS * FBS / 3 = 512
+
+template <int STAGE>
+__global__ __launch_bounds__(FBS) void k_kge_rmarch2d(KgeArgs a, int j, int emaxP, int emaxV, int alt) {
   extern __shared__ float lds[];
   const int2 cm = a.cmap[blockIdx.x];
   if (cm.x < 0) return;
   const TrajDev T = a.traj[cm.x];
   __shared__ double red[NSEQ * PN];
   __shared__ double shn[NSEQ * PN * (FBS / kWave)];
+  // v_j = ((A P - qc Q) - pa P) * pb ;  w'_j = A v_j - pc P
+  float pa[NSEQ] = {0.f, 0.f}, pb[NSEQ] = {0.f, 0.f}, qc[NSEQ] = {0.f, 0.f}, pc[NSEQ] = {0.f, 0.f};
+  if (STAGE >= 1) {
+    reduce_partials(a, T, (j - 1) & 1, red);
+#pragma unroll
+    for (int s = 0; s < NSEQ; ++s) {
+      const double q0 = red[s * PN], q1 = red[s * PN + 1], q2 = red[s * PN + 2];
+      if (STAGE == 1) {
+        const double b0 = sqrt(q0);
+        const double al = q0 > 0.0 ? q1 / q0 : 0.0;
+        const double b1sq = q0 > 0.0 ? (q2 - q1 * q1 / q0) / q0 : 0.0;
+        const double b1 = (b1sq > 1e-14 * (q0 > 0.0 ? q2 / q0 : 0.0) && b1sq > 0.0) ? sqrt(b1sq) : 0.0;
+        pa[s] = (float)al;
+        pb[s] = (b0 > 0.0 && b1 > 0.0) ? (float)(1.0 / (b0 * b1)) : 0.0f;
+        pc[s] = b0 > 0.0 ? (float)(b1 / b0) : 0.0f;
+        if (cm.y == 0 && threadIdx.x == 0) {
+          a.scal[cm.x].b0[s] = b0;
+          a.scal[cm.x].alpha[s][0] = al;
+          a.scal[cm.x].beta[s][1] = b1;
+        }
+      } else {
+        double al, b;
+        lanczos_scalars(q0, q1, q2, al, b);
+        pa[s] = (float)al;
+        pb[s] = b > 0.0 ? (float)(1.0 / b) : 0.0f;
+        pc[s] = (float)b;
+        if (STAGE == 2) {
+          const double b0 = a.scal[cm.x].b0[s], b1 = a.scal[cm.x].beta[s][1];
+          qc[s] = b0 > 0.0 ? (float)(b1 / b0) : 0.0f;
+        } else {
+          qc[s] = (float)a.scal[cm.x].beta[s][j - 1];
+        }
+        if (cm.y == 0 && threadIdx.x == 0) {
+          a.scal[cm.x].alpha[s][j - 1] = al;
+          a.scal[cm.x].beta[s][j] = b;
+        }
+      }
+    }
+  }
+  const int n = T.n;
+  const int N = (int)T.N; // 2D grids: N < 2^31
+  const int64_t cells = a.cells;
+  // LDS: c, P (both sequences) over local [cb - 2n, cb + SUB + 2n); v_j over [cb - n, cb + SUB + n)
+  float *sc = lds, *sp0 = lds + emaxP, *sp1 = lds + 2 * emaxP;
+  float *sv0 = lds + 3 * emaxP, *sv1 = sv0 + emaxV;
+  // per-trajectory bases (uniform), 32-bit cell offsets
+  const size_t m2 = (size_t)(a.m - 2);
+  const float *cT = a.cf + T.off, *mT = a.mf + T.off;
+  const double *uT = a.u + T.off;
+  const float *P0 = a.V + (STAGE >= 2 ? (0 * m2 + (size_t)(j - 2)) * cells : 0) + T.off;
+  const float *P1 = a.V + (STAGE >= 2 ? (1 * m2 + (size_t)(j - 2)) * cells : 0) + T.off;
+  const float *Q0 = a.V + (STAGE >= 3 ? (0 * m2 + (size_t)(j - 3)) * cells : 0) + T.off;
+  const float *Q1 = a.V + (STAGE >= 3 ? (1 * m2 + (size_t)(j - 3)) * cells : 0) + T.off;
+  float *Vout0 = a.V + (STAGE >= 1 ? (0 * m2 + (size_t)(j - 1)) * cells : 0) + T.off;
+  float *Vout1 = a.V + (STAGE >= 1 ? (1 * m2 + (size_t)(j - 1)) * cells : 0) + T.off;
+  float *Wout0 = a.W + ((size_t)0 * 2 + (j & 1)) * cells + T.off;
+  float *Wout1 = a.W + ((size_t)1 * 2 + (j & 1)) * cells + T.off;
+  const bool last = j == a.m - 2;
+  const float scl = T.s, mu2 = T.mu2;
+  double acc[NSEQ * PN];
+#pragma unroll
+  for (int k = 0; k < NSEQ * PN; ++k) acc[k] = 0.0;
+  const int rbeg = cm.y * T.range;
+  const int rend = rbeg + T.range < N ? rbeg + T.range : N;
+  const int len = rend - rbeg;
+  // local coordinate l -> cell org + dir * l (valid while 0 <= cell < N)
+  const bool down = alt != 0 && (cm.y & 1);
+  const int org = down ? rend - 1 : rbeg;
+  const int dir = down ? -1 : 1;
+  const int oU = -dir * n, oL = -dir, oR = dir, oD = dir * n; // grid neighbours in local offsets
+  const float rn = 1.0f / (float)n;
+  auto cell = [&](int l) { return org + dir * l; };
+  auto seed = [&](int s, int i) -> float {
+    const double uu = uT[i];
+    if (s == 0) return (float)uu;
+    return (float)(-(double)mT[i] * uu * uu * uu);
+  };
+  auto loadP = [&](int i, float &x0, float &x1) {
+    if (STAGE <= 1) {
+      x0 = seed(0, i);
+      x1 = seed(1, i);
+    } else {
+      x0 = P0[i];
+      x1 = P1[i];
+    }
+  };
+  auto loadQ = [&](int i, float &x0, float &x1) {
+    if (STAGE == 2) {
+      x0 = seed(0, i);
+      x1 = seed(1, i);
+    } else if (STAGE == 3) {
+      x0 = Q0[i];
+      x1 = Q1[i];
+    } else {
+      x0 = 0.0f;
+      x1 = 0.0f;
+    }
+  };
+  // v_j at cell i: P window index eP, v window index eV
+  auto phaseA = [&](int i, int eP, int eV, float q0, float q1) {
+    const Sten5 w = sten5(sc, eP, oU, oL, oR, oD, i, n, rn);
+    const float a0 = apply5(w, sp0, eP, oU, oL, oR, oD, scl, mu2);
+    const float a1 = apply5(w, sp1, eP, oU, oL, oR, oD, scl, mu2);
+    const float w0 = a0 - qc[0] * q0;
+    const float w1 = a1 - qc[1] * q1;
+    sv0[eV] = (w0 - pa[0] * sp0[eP]) * pb[0];
+    sv1[eV] = (w1 - pa[1] * sp1[eP]) * pb[1];
+  };
+  // first window: c, P on [-2n, SUB + 2n), Q parked in the v_j window on [-n, SUB + n);
+  // loads are issued MSPT at a time per thread before their LDS stores
+  for (int e0 = 0; e0 < MSUB + 4 * n; e0 += MSPT * FBS) {
+    float rc[MSPT], r0[MSPT], r1[MSPT], s0[MSPT], s1[MSPT];
+#pragma unroll
+    for (int q = 0; q < MSPT; ++q) {
+      const int e = e0 + threadIdx.x + q * FBS;
+      const int i = cell(e - 2 * n);
+      if (e < MSUB + 4 * n && i >= 0 && i < N) {
+        rc[q] = cT[i];
+        loadP(i, r0[q], r1[q]);
+      }
+      const int iq = cell(e - n);
+      if (STAGE >= 2 && e < MSUB + 2 * n && iq >= 0 && iq < N) loadQ(iq, s0[q], s1[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < MSPT; ++q) {
+      const int e = e0 + threadIdx.x + q * FBS;
+      const int i = cell(e - 2 * n);
+      if (e < MSUB + 4 * n && i >= 0 && i < N) {
+        sc[e] = rc[q];
+        sp0[e] = r0[q];
+        sp1[e] = r1[q];
+      }
+      const int iq = cell(e - n);
+      if (STAGE >= 2 && e < MSUB + 2 * n && iq >= 0 && iq < N) {
+        sv0[e] = s0[q];
+        sv1[e] = s1[q];
+      }
+    }
+  }
+  // Register prefetch of the new cells of sub-chunk cb: c, P at [cb + 2n, cb + SUB + 2n),
+  // Q at [cb + n, cb + SUB + n).  Issued before the compute phases that precede
+  // their deposit, so the loads overlap two phases of work.
+  float nc[MSPT], np0[MSPT], np1[MSPT], nq0[MSPT], nq1[MSPT];
+  auto prefetch = [&](int cb) {
+#pragma unroll
+    for (int q = 0; q < MSPT; ++q) {
+      const int l = cb + threadIdx.x + q * FBS;
+      const int ip = cell(l + 2 * n);
+      if (ip >= 0 && ip < N) {
+        nc[q] = cT[ip];
+        loadP(ip, np0[q], np1[q]);
+      }
+      const int iq = cell(l + n);
+      if (STAGE >= 2 && iq >= 0 && iq < N) loadQ(iq, nq0[q], nq1[q]);
+    }
+  };
+  if (MSUB < len) prefetch(MSUB);
+  __syncthreads();
+  if (STAGE >= 1) {
+    // v_j on [-n, SUB + n); each element's Q is read and overwritten by the same thread
+    for (int e = threadIdx.x; e < MSUB + 2 * n; e += FBS) {
+      const int i = cell(e - n);
+      if (i >= 0 && i < N) phaseA(i, e + n, e, STAGE >= 2 ? sv0[e] : 0.0f, STAGE >= 2 ? sv1[e] : 0.0f);
+    }
+    __syncthreads();
+  }
+  for (int cb = 0; cb < len; cb += MSUB) {
+    const int ce = cb + MSUB < len ? cb + MSUB : len;
+#pragma unroll
+    for (int kk = 0; kk < MSPT; ++kk) {
+      const int l = cb + threadIdx.x + kk * FBS;
+      if (l < ce) {
+        const int i = cell(l);
+        const int eP = l - cb + 2 * n, eV = l - cb + n;
+        const Sten5 w = sten5(sc, eP, oU, oL, oR, oD, i, n, rn);
+        if (STAGE == 0) {
+#pragma unroll
+          for (int s = 0; s < NSEQ; ++s) {
+            const float *sp = s == 0 ? sp0 : sp1;
+            const float x = sp[eP];
+            const float Av = apply5(w, sp, eP, oU, oL, oR, oD, scl, mu2);
+            acc[s * PN + 0] += (double)x * x;
+            acc[s * PN + 1] += (double)x * Av;
+            acc[s * PN + 2] += (double)Av * Av;
+          }
+        } else {
+#pragma unroll
+          for (int s = 0; s < NSEQ; ++s) {
+            const float *sv = s == 0 ? sv0 : sv1;
+            const float vi = sv[eV];
+            const float Av = apply5(w, sv, eV, oU, oL, oR, oD, scl, mu2);
+            const float wp = Av - pc[s] * (s == 0 ? sp0 : sp1)[eP];
+            (s == 0 ? Vout0 : Vout1)[i] = vi;
+            if (last) (s == 0 ? Wout0 : Wout1)[i] = wp;
+            acc[s * PN + 0] += (double)wp * vi;
+            acc[s * PN + 1] += (double)wp * wp;
+            acc[s * PN + 2] += (double)vi * vi;
+          }
+        }
+      }
+    }
+    if (cb + MSUB >= len) break;
+    __syncthreads(); // every read of this window is done
+    // keep c on [cb' - n, cb' + 2n), v_j on [cb' - n, cb' + n) and P on [cb', cb' + 2n)
+    // (STAGE 0 applies A to P itself: P on [cb' - n, cb' + 2n))
+    {
+      const int pk = STAGE == 0 ? 3 * n : 2 * n, p0 = 4 * n - pk;
+      if (3 * n <= MSUB) {
+        // source and destination ranges are disjoint: copy in place
+        for (int e = threadIdx.x; e < 3 * n; e += FBS) sc[n + e] = sc[MSUB + n + e];
+        for (int e = threadIdx.x; e < pk; e += FBS) {
+          sp0[p0 + e] = sp0[MSUB + p0 + e];
+          sp1[p0 + e] = sp1[MSUB + p0 + e];
+        }
+        if (STAGE >= 1)
+          for (int e = threadIdx.x; e < 2 * n; e += FBS) {
+            sv0[e] = sv0[MSUB + e];
+            sv1[e] = sv1[MSUB + e];
+          }
+        __syncthreads(); // copies done before the deposit overwrites their sources
+      } else {
+        // ranges of length 3n overlap (n > SUB / 3): stage them through registers
+        float tc[RKS], t0[RKS], t1[RKS];
+#pragma unroll
+        for (int k = 0; k < RKS; ++k) {
+          const int e = threadIdx.x + k * FBS;
+          if (e < 3 * n) {
+            tc[k] = sc[MSUB + n + e];
+            if (STAGE == 0) {
+              t0[k] = sp0[MSUB + n + e];
+              t1[k] = sp1[MSUB + n + e];
+            }
+          }
+        }
+        if (STAGE >= 1)
+          for (int e = threadIdx.x; e < 2 * n; e += FBS) { // 2n <= SUB: disjoint
+            sp0[2 * n + e] = sp0[MSUB + 2 * n + e];
+            sp1[2 * n + e] = sp1[MSUB + 2 * n + e];
+            sv0[e] = sv0[MSUB + e];
+            sv1[e] = sv1[MSUB + e];
+          }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < RKS; ++k) {
+          const int e = threadIdx.x + k * FBS;
+          if (e < 3 * n) {
+            sc[n + e] = tc[k];
+            if (STAGE == 0) {
+              sp0[n + e] = t0[k];
+              sp1[n + e] = t1[k];
+            }
+          }
+        }
+      }
+    }
+    // deposit the prefetched cells: c, P at [4n, SUB + 4n), Q parked at [2n, SUB + 2n)
+#pragma unroll
+    for (int q = 0; q < MSPT; ++q) {
+      const int l = cb + MSUB + threadIdx.x + q * FBS;
+      const int e = 4 * n + threadIdx.x + q * FBS;
+      const int ip = cell(l + 2 * n);
+      if (ip >= 0 && ip < N) {
+        sc[e] = nc[q];
+        sp0[e] = np0[q];
+        sp1[e] = np1[q];
+      }
+      const int iq = cell(l + n);
+      if (STAGE >= 2 && iq >= 0 && iq < N) {
+        sv0[e - 2 * n] = nq0[q];
+        sv1[e - 2 * n] = nq1[q];
+      }
+    }
+    if (cb + 2 * MSUB < len) prefetch(cb + 2 * MSUB);
+    __syncthreads();
+    if (STAGE >= 1) {
+      // v_j on the next window's new cells [cb' + n, cb' + SUB + n)
+#pragma unroll
+      for (int q = 0; q < MSPT; ++q) {
+        const int eV = 2 * n + threadIdx.x + q * FBS;
+        const int i = cell(cb + MSUB + n + threadIdx.x + q * FBS);
+        if (i >= 0 && i < N) phaseA(i, eV + n, eV, STAGE >= 2 ? sv0[eV] : 0.0f, STAGE >= 2 ? sv1[eV] : 0.0f);
+      }
+      __syncthreads();
+    }
+  }
+  block_sum_n<NSEQ * PN>(acc, shn);
+  if (threadIdx.x == 0) {
+    double *dst = a.part + ((size_t)(j & 1) * a.total_chunks + T.chunk0 + cm.y) * (NSEQ * PN);
+#pragma unroll
+    for (int k = 0; k < NSEQ * PN; ++k) dst[k] = acc[k];
+  }
+}
+
+inline size_t rmarch2d_lds_bytes(int nmax) {
+  return sizeof(float) * (3 * (size_t)(MSUB + 4 * nmax) + 2 * (size_t)(MSUB + 2 * nmax));
+}
+
+// ---------------------------------------------------------------- 2D row march, stored w'
+// Pass j reads w'_{j-1} and v_{j-1} (one grid row of halo), forms
+// v_j = (w'_{j-1} - alpha_{j-1} v_{j-1}) / b_j as it deposits each cell into the LDS
+// window [cb - n, cb + SUB + n), applies A from LDS and stores v_j and
+// w'_j = A v_j - b_j v_{j-1}.  Same local coordinates as k_kge_rmarch2d: blocks with an
+// odd range index march downward (alt != 0), so shared halo rows come from L2.
+// STAGE 0: j = 0 on the seeds x (stores z = A x); 1: j = 1 (from z and x); 2: j >= 2.
+template <int STAGE>
+__global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax, int alt) {
+  extern __shared__ float lds[];
+  const int2 cm = a.cmap[blockIdx.x];
+  if (cm.x < 0) return;
+  const TrajDev T = a.traj[cm.x];
+  __shared__ double red[NSEQ * PN];
+  __shared__ double shn[NSEQ * PN * (FBS / kWave)];
+  // v = (X - cA P) * cB at deposit;  w' = A v - cC P
   float cA[NSEQ] = {0.f, 0.f}, cB[NSEQ] = {0.f, 0.f}, cC[NSEQ] = {0.f, 0.f};
   if (STAGE >= 1) {
     reduce_partials(a, T, (j - 1) & 1, red);
@@ -577,124 +956,118 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax)
     }
   }
   const int n = T.n;
-  const int64_t N = T.N, cells = a.cells;
-  float *sv0 = lds, *sv1 = lds + emax, *sc = lds + 2 * emax;
-  float *sp0 = lds + 3 * emax, *sp1 = lds + 4 * emax;
-  const float *W0 = a.W + (size_t)(0 * 2 + ((j - 1) & 1)) * cells;
-  const float *W1 = a.W + (size_t)(1 * 2 + ((j - 1) & 1)) * cells;
-  const float *V0 = a.V + (size_t)(0 * (a.m - 2) + (j - 2)) * cells;
-  const float *V1 = a.V + (size_t)(1 * (a.m - 2) + (j - 2)) * cells;
-  const float *Z0 = a.W + (size_t)(0 * 2) * cells, *Z1 = a.W + (size_t)(1 * 2) * cells;
-  float *Vout0 = a.V + ((size_t)0 * (a.m - 2) + (j - 1)) * cells;
-  float *Vout1 = a.V + ((size_t)1 * (a.m - 2) + (j - 1)) * cells;
-  float *Wout0 = a.W + ((size_t)0 * 2 + (j & 1)) * cells;
-  float *Wout1 = a.W + ((size_t)1 * 2 + (j & 1)) * cells;
+  const int N = (int)T.N;
+  const int64_t cells = a.cells;
+  float *sc = lds, *sv0 = lds + emax, *sv1 = lds + 2 * emax, *sp0 = lds + 3 * emax,
+        *sp1 = lds + 4 * emax;
+  const size_t m2 = (size_t)(a.m - 2);
+  const float *cT = a.cf + T.off, *mT = a.mf + T.off;
+  const double *uT = a.u + T.off;
+  // X source: STAGE 1 z = W[s][0]; STAGE 2 w'_{j-1}.  P source: STAGE 2 v_{j-1}.
+  const float *X0 = a.W + ((size_t)0 * 2 + (STAGE == 1 ? 0 : ((j - 1) & 1))) * cells + T.off;
+  const float *X1 = a.W + ((size_t)1 * 2 + (STAGE == 1 ? 0 : ((j - 1) & 1))) * cells + T.off;
+  const float *P0 = a.V + (STAGE >= 2 ? (0 * m2 + (size_t)(j - 2)) * cells : 0) + T.off;
+  const float *P1 = a.V + (STAGE >= 2 ? (1 * m2 + (size_t)(j - 2)) * cells : 0) + T.off;
+  float *Vout0 = a.V + (STAGE >= 1 ? (0 * m2 + (size_t)(j - 1)) * cells : 0) + T.off;
+  float *Vout1 = a.V + (STAGE >= 1 ? (1 * m2 + (size_t)(j - 1)) * cells : 0) + T.off;
+  float *Wout0 = a.W + ((size_t)0 * 2 + (j & 1)) * cells + T.off;
+  float *Wout1 = a.W + ((size_t)1 * 2 + (j & 1)) * cells + T.off;
   const float scl = T.s, mu2 = T.mu2;
   double acc[NSEQ * PN];
 #pragma unroll
   for (int k = 0; k < NSEQ * PN; ++k) acc[k] = 0.0;
-  const int64_t rbeg = (int64_t)cm.y * range;
-  const int64_t rend = rbeg + range < N ? rbeg + range : N;
-  // Register double-buffering: the loads of sub-chunk k+1 are issued before the
-  // stencil of sub-chunk k runs from LDS, so memory latency overlaps compute.
-  float px0[MSPT], px1[MSPT], py0[MSPT], py1[MSPT], pcc[MSPT];
-  auto fetch = [&](int64_t flo, int cnt) {
-#pragma unroll
-    for (int q = 0; q < MSPT; ++q) {
-      const int e = threadIdx.x + q * FBS;
-      const int64_t g = T.off + flo + (e < cnt ? e : 0);
-      pcc[q] = a.cf[g];
-      if (STAGE == 0) {
-        px0[q] = seed_x(0, a, g);
-        px1[q] = seed_x(1, a, g);
-        py0[q] = py1[q] = 0.f;
-      } else if (STAGE == 1) {
-        px0[q] = Z0[g];
-        px1[q] = Z1[g];
-        py0[q] = seed_x(0, a, g);
-        py1[q] = seed_x(1, a, g);
-      } else {
-        px0[q] = W0[g];
-        px1[q] = W1[g];
-        py0[q] = V0[g];
-        py1[q] = V1[g];
-      }
+  const int rbeg = cm.y * T.range;
+  const int rend = rbeg + T.range < N ? rbeg + T.range : N;
+  const int len = rend - rbeg;
+  const bool down = alt != 0 && (cm.y & 1);
+  const int org = down ? rend - 1 : rbeg;
+  const int dir = down ? -1 : 1;
+  const int oU = -dir * n, oL = -dir, oR = dir, oD = dir * n;
+  const float rn = 1.0f / (float)n;
+  auto cell = [&](int l) { return org + dir * l; };
+  auto seed = [&](int s, int i) -> float {
+    const double uu = uT[i];
+    if (s == 0) return (float)uu;
+    return (float)(-(double)mT[i] * uu * uu * uu);
+  };
+  // raw inputs of cell i: c, X (both sequences), P (both sequences)
+  auto fetch = [&](int i, float &c, float &x0, float &x1, float &p0, float &p1) {
+    c = cT[i];
+    if (STAGE == 0) {
+      x0 = seed(0, i);
+      x1 = seed(1, i);
+      p0 = p1 = 0.0f;
+    } else if (STAGE == 1) {
+      x0 = X0[i];
+      x1 = X1[i];
+      p0 = seed(0, i);
+      p1 = seed(1, i);
+    } else {
+      x0 = X0[i];
+      x1 = X1[i];
+      p0 = P0[i];
+      p1 = P1[i];
     }
   };
-  auto deposit = [&](int64_t flo, int cnt, int64_t wb) {
-#pragma unroll
-    for (int q = 0; q < MSPT; ++q) {
-      const int e = threadIdx.x + q * FBS;
-      if (e < cnt) {
-        const int w = (int)(flo + e - wb);
-        sc[w] = pcc[q];
-        if (STAGE == 0) {
-          sv0[w] = px0[q];
-          sv1[w] = px1[q];
-        } else {
-          sv0[w] = (px0[q] - cA[0] * py0[q]) * cB[0];
-          sv1[w] = (px1[q] - cA[1] * py1[q]) * cB[1];
-          sp0[w] = py0[q];
-          sp1[w] = py1[q];
-        }
-      }
+  auto deposit = [&](int e, float c, float x0, float x1, float p0, float p1) {
+    sc[e] = c;
+    if (STAGE == 0) {
+      sv0[e] = x0;
+      sv1[e] = x1;
+    } else {
+      sv0[e] = (x0 - cA[0] * p0) * cB[0];
+      sv1[e] = (x1 - cA[1] * p1) * cB[1];
+      sp0[e] = p0;
+      sp1[e] = p1;
     }
   };
-  // first window: [max(rbeg - n, 0), min(rbeg + SUB + n, N)) in slices of SUB
-  {
-    const int64_t wb = rbeg - n;
-    const int64_t flo0 = wb > 0 ? wb : 0;
-    const int64_t ce0 = rbeg + MSUB < rend ? rbeg + MSUB : rend;
-    const int64_t hi0 = ce0 + n < N ? ce0 + n : N;
-    for (int64_t f = flo0; f < hi0; f += MSUB) {
-      const int cnt = (int)((hi0 - f) < MSUB ? (hi0 - f) : MSUB);
-      fetch(f, cnt);
-      deposit(f, cnt, wb);
+  // first window [-n, SUB + n), MSPT loads per thread in flight before their deposits
+  for (int e0 = 0; e0 < MSUB + 2 * n; e0 += MSPT * FBS) {
+    float rc[MSPT], x0[MSPT], x1[MSPT], p0[MSPT], p1[MSPT];
+#pragma unroll
+    for (int q = 0; q < MSPT; ++q) {
+      const int e = e0 + threadIdx.x + q * FBS, i = cell(e - n);
+      if (e < MSUB + 2 * n && i >= 0 && i < N) fetch(i, rc[q], x0[q], x1[q], p0[q], p1[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < MSPT; ++q) {
+      const int e = e0 + threadIdx.x + q * FBS, i = cell(e - n);
+      if (e < MSUB + 2 * n && i >= 0 && i < N) deposit(e, rc[q], x0[q], x1[q], p0[q], p1[q]);
     }
   }
-  for (int64_t cb = rbeg; cb < rend; cb += MSUB) {
-    const int64_t ce = cb + MSUB < rend ? cb + MSUB : rend;
-    const int64_t wb = cb - n;
-    // prefetch the next window's new cells [ce + n, min(ce + SUB + n, N))
-    const int64_t nce = ce + MSUB < rend ? ce + MSUB : rend;
-    const int64_t nflo = ce + n;
-    const int64_t nhi = nce + n < N ? nce + n : N;
-    const int ncnt = ce < rend && nhi > nflo ? (int)(nhi - nflo) : 0;
-    if (ncnt > 0) fetch(nflo, ncnt);
+  float nc[MSPT], nx0[MSPT], nx1[MSPT], np0[MSPT], np1[MSPT];
+  for (int cb = 0; cb < len; cb += MSUB) {
+    const int ce = cb + MSUB < len ? cb + MSUB : len;
+    const bool more = cb + MSUB < len;
+    if (more) {
+      // the next window's new cells [cb' + n, cb' + SUB + n)
+#pragma unroll
+      for (int q = 0; q < MSPT; ++q) {
+        const int i = cell(cb + MSUB + n + threadIdx.x + q * FBS);
+        if (i >= 0 && i < N) fetch(i, nc[q], nx0[q], nx1[q], np0[q], np1[q]);
+      }
+    }
     __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < MSPT; ++kk) {
-      const int64_t i = cb + threadIdx.x + (int64_t)kk * FBS;
-      if (i < ce) {
-        const int e = (int)(i - wb);
-        const int64_t r = i / n, col = i - r * n;
-        const float ci = sc[e];
-        const float wu = r > 0 ? 0.5f * (ci + sc[e - n]) : 0.0f;
-        const float wl = col > 0 ? 0.5f * (ci + sc[e - 1]) : 0.0f;
-        const float wr = col < n - 1 ? 0.5f * (ci + sc[e + 1]) : 0.0f;
-        const float wd = r < n - 1 ? 0.5f * (ci + sc[e + n]) : 0.0f;
-        const float wsum = ((wu + wl) + wr) + wd;
-        const int64_t gi = T.off + i;
+      const int l = cb + threadIdx.x + kk * FBS;
+      if (l < ce) {
+        const int i = cell(l), e = l - cb + n;
+        const Sten5 w = sten5(sc, e, oU, oL, oR, oD, i, n, rn);
 #pragma unroll
         for (int s = 0; s < NSEQ; ++s) {
           const float *sv = s == 0 ? sv0 : sv1;
           const float vi = sv[e];
-          float nbsum = 0.0f;
-          if (r > 0) nbsum += wu * sv[e - n];
-          if (col > 0) nbsum += wl * sv[e - 1];
-          if (col < n - 1) nbsum += wr * sv[e + 1];
-          if (r < n - 1) nbsum += wd * sv[e + n];
-          const float Av = scl * (wsum * vi - nbsum) + mu2 * vi;
+          const float Av = apply5(w, sv, e, oU, oL, oR, oD, scl, mu2);
           if (STAGE == 0) {
-            (s == 0 ? Wout0 : Wout1)[gi] = Av;
+            (s == 0 ? Wout0 : Wout1)[i] = Av;
             acc[s * PN + 0] += (double)vi * vi;
             acc[s * PN + 1] += (double)vi * Av;
             acc[s * PN + 2] += (double)Av * Av;
           } else {
-            const float vprev = (s == 0 ? sp0 : sp1)[e];
-            const float wp = Av - cC[s] * vprev;
-            (s == 0 ? Vout0 : Vout1)[gi] = vi;
-            (s == 0 ? Wout0 : Wout1)[gi] = wp;
+            const float wp = Av - cC[s] * (s == 0 ? sp0 : sp1)[e];
+            (s == 0 ? Vout0 : Vout1)[i] = vi;
+            (s == 0 ? Wout0 : Wout1)[i] = wp;
             acc[s * PN + 0] += (double)wp * vi;
             acc[s * PN + 1] += (double)wp * wp;
             acc[s * PN + 2] += (double)vi * vi;
@@ -702,18 +1075,24 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax)
         }
       }
     }
-    if (ce >= rend) break;
-    __syncthreads(); // all stencil reads of this window are done
-    // shift the shared 2n cells to the front, then deposit the prefetched cells
+    if (!more) break;
+    __syncthreads(); // every read of this window is done
+    // keep [cb' - n, cb' + n) (disjoint source and destination: 2n <= SUB)
     for (int e = threadIdx.x; e < 2 * n; e += FBS) {
+      sc[e] = sc[MSUB + e];
       sv0[e] = sv0[MSUB + e];
       sv1[e] = sv1[MSUB + e];
-      sc[e] = sc[MSUB + e];
-      sp0[e] = sp0[MSUB + e];
-      sp1[e] = sp1[MSUB + e];
+      if (STAGE >= 1) {
+        sp0[e] = sp0[MSUB + e];
+        sp1[e] = sp1[MSUB + e];
+      }
     }
     __syncthreads();
-    if (ncnt > 0) deposit(nflo, ncnt, wb + MSUB);
+#pragma unroll
+    for (int q = 0; q < MSPT; ++q) {
+      const int i = cell(cb + MSUB + n + threadIdx.x + q * FBS);
+      if (i >= 0 && i < N) deposit(2 * n + threadIdx.x + q * FBS, nc[q], nx0[q], nx1[q], np0[q], np1[q]);
+    }
   }
   block_sum_n<NSEQ * PN>(acc, shn);
   if (threadIdx.x == 0) {
@@ -723,16 +1102,12 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax)
   }
 }
 
-inline size_t march2d_lds_bytes(int nmax) {
-  return sizeof(float) * 5 * (size_t)(MSUB + 2 * nmax);
-}
-
-// Combine over a runtime chunk in groups of 4 cells per thread (independent loads).
+// Combine over the workgroup's range in groups of 4 cells per thread (independent loads).
 // v_{m-1} = (w'_{m-2} - alpha_{m-2} v_{m-2}) / b_{m-1} is folded into the
 // coefficients: d = y0 x/b0 + sum_{k<m-2} y_k v_k + (y_{m-2} - y_{m-1} alpha/b) v_{m-2}
 // + (y_{m-1}/b) w'_{m-2}, so each stored vector is read exactly once.
 template <int DIM>
-__global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a, int chunk) {
+__global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a) {
   const int2 cm = a.cmap[blockIdx.x];
   if (cm.x < 0) return;
   const TrajDev T = a.traj[cm.x];
@@ -754,8 +1129,8 @@ __global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a, int chunk) {
   constexpr int G = 4;
   const int n = T.n;
   const int64_t cells = a.cells;
-  const int64_t cbeg = (int64_t)cm.y * chunk;
-  const int64_t cend = cbeg + chunk < T.N ? cbeg + chunk : T.N;
+  const int64_t cbeg = (int64_t)cm.y * T.range;
+  const int64_t cend = cbeg + T.range < T.N ? cbeg + T.range : T.N;
   const double t2 = T.tau * T.tau;
   const float *Wl0 = a.W + ((size_t)0 * 2 + ((m - 2) & 1)) * cells;
   const float *Wl1 = a.W + ((size_t)1 * 2 + ((m - 2) & 1)) * cells;
@@ -792,10 +1167,10 @@ __global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a, int chunk) {
   }
 }
 
-// Snapshot combine over a runtime chunk: also the velocity (u+ - u)/tau before
+// Snapshot combine over the workgroup's range: also the velocity (u+ - u)/tau before
 // the ghost copy and both copies of u+ (same folded coefficients as above).
 template <int DIM>
-__global__ __launch_bounds__(FBS) void k_kge_combine_snap(KgeArgs a, SnapArgs snap, int chunk) {
+__global__ __launch_bounds__(FBS) void k_kge_combine_snap(KgeArgs a, SnapArgs snap) {
   const int2 cm = a.cmap[blockIdx.x];
   if (cm.x < 0) return;
   const TrajDev T = a.traj[cm.x];
@@ -828,8 +1203,8 @@ __global__ __launch_bounds__(FBS) void k_kge_combine_snap(KgeArgs a, SnapArgs sn
     }
     return 2.0 * uu - a.up[g] + 2.0 * d0 + t2 * d1;
   };
-  const int64_t cbeg = (int64_t)cm.y * chunk;
-  const int64_t cend = cbeg + chunk < T.N ? cbeg + chunk : T.N;
+  const int64_t cbeg = (int64_t)cm.y * T.range;
+  const int64_t cend = cbeg + T.range < T.N ? cbeg + T.range : T.N;
   for (int64_t i = cbeg + threadIdx.x; i < cend; i += FBS) {
     const int64_t src = clamp_src<DIM>(i, n);
     const int64_t gi = T.off + i, gs = T.off + src;
@@ -849,8 +1224,13 @@ struct Layout {
   int64_t cells = 0;
 };
 
+// Chunking: fixed ranges of 256 * cpt cells (the tiled pass kernels need exactly
+// that), or, for the row-march kernels (balanced), the same number of chunks with
+// the trajectory split into equal ranges (multiples of 64 cells), so the last
+// workgroup of a trajectory is not a short one.  Either way a trajectory's
+// chunking depends only on its own size and the tile.
 static Layout make_layout(const nls_traj_desc *td, int ntraj, int dim, int cpt, double dt,
-                          double mu2) {
+                          double mu2, bool balanced = false) {
   Layout L;
   const int CH = FBS * cpt;
   L.traj.resize(ntraj);
@@ -863,6 +1243,12 @@ static Layout make_layout(const nls_traj_desc *td, int ntraj, int dim, int cpt, 
     T.off = L.cells;
     L.cells += (T.N + 63) / 64 * 64; // keep every trajectory 256-byte aligned
     T.nchunks = (int)((T.N + CH - 1) / CH);
+    T.range = CH;
+    if (balanced) {
+      const int64_t r = (T.N + T.nchunks - 1) / T.nchunks;
+      T.range = (int)((r + 63) / 64 * 64);
+      T.nchunks = (int)((T.N + T.range - 1) / T.range);
+    }
     T.chunk0 = L.total_chunks;
     L.total_chunks += T.nchunks;
     const double dx = 2.0 * td[t].L / (T.n - 1);
@@ -1199,7 +1585,7 @@ struct nls_batch {
   int nsnap = 0;
   hipGraphExec_t graphs[3] = {nullptr, nullptr, nullptr};
   bool profiling = false;
-  int pass_kind = 0; // 0 direct, 1 LDS-tiled chunk, 2 row march
+  int pass_kind = 0; // 0 direct, 1 LDS-tiled chunk, 2 row march, 3 row march with recomputed w'
   int nmax = 0;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
@@ -1295,7 +1681,7 @@ struct nls_batch {
   void set_layout(int cpt_) {
     cpt = cpt_;
     const double mu2 = p.mu2;
-    lay = make_layout(desc.data(), ntraj, p.dim, cpt, p.dt, mu2);
+    lay = make_layout(desc.data(), ntraj, p.dim, cpt, p.dt, mu2, pass_kind >= 2);
     nmax = 0;
     for (auto &T : lay.traj) nmax = std::max(nmax, T.n);
     NLS_REQUIRE(lay.cmap.size() <= cmap_cap, "internal: chunk map larger than allocated");
@@ -1322,15 +1708,14 @@ struct nls_batch {
   }
 
   // Pass kernels: kind 0 = direct global-memory stencil (any dim), 1 = LDS-tiled
-  // chunk + halo rows (2D), 2 = row march with an LDS window (2D).
+  // chunk + halo rows (2D), 2 = row march with an LDS window (2D), 3 = row march
+  // that stores only the basis and recomputes w' (2D, launch_rpass).
   template <int DIM, int STAGE> void launch_pass(const KgeArgs &a, int j) {
     const dim3 grid((unsigned)lay.cmap.size()), blk(FBS);
     if (pass_kind == 2) {
       const size_t lds = march2d_lds_bytes(nmax);
       const int emax = MSUB + 2 * nmax;
-      if (cpt == 16) hipLaunchKernelGGL((k_kge_march2d<STAGE, 16>), grid, blk, lds, s, a, j, emax);
-      else if (cpt == 32) hipLaunchKernelGGL((k_kge_march2d<STAGE, 32>), grid, blk, lds, s, a, j, emax);
-      else hipLaunchKernelGGL((k_kge_march2d<STAGE, 64>), grid, blk, lds, s, a, j, emax);
+      hipLaunchKernelGGL((k_kge_march2d<STAGE>), grid, blk, lds, s, a, j, emax, 1);
     } else if (pass_kind == 1) {
       const size_t lds = pass2d_lds_bytes(cpt, nmax);
       const int emax = FBS * cpt + 2 * nmax;
@@ -1343,22 +1728,41 @@ struct nls_batch {
     }
   }
 
+  template <int STAGE> void launch_rpass(const KgeArgs &a, int j) {
+    const dim3 grid((unsigned)lay.cmap.size()), blk(FBS);
+    const size_t lds = rmarch2d_lds_bytes(nmax);
+    const int eP = MSUB + 4 * nmax, eV = MSUB + 2 * nmax, alt = 1;
+    hipLaunchKernelGGL((k_kge_rmarch2d<STAGE>), grid, blk, lds, s, a, j, eP, eV, alt);
+  }
+
   template <int DIM> void launch_step_t(int cur, bool snapshot, int slot, bool prof) {
     const KgeArgs a = args(cur);
     const dim3 grid((unsigned)lay.cmap.size()), blk(FBS);
-    launch_pass<DIM, 0>(a, 0);
-    if (m - 2 >= 1) launch_pass<DIM, 1>(a, 1);
-    for (int j = 2; j <= m - 2; ++j) {
-      if (prof) NLS_HIP(hipEventRecord(next_event(), s));
-      launch_pass<DIM, 2>(a, j);
-      if (prof) NLS_HIP(hipEventRecord(next_event(), s));
+    if (pass_kind == 3) {
+      // stored-basis-only passes; events around the general (j >= 3) passes
+      launch_rpass<0>(a, 0);
+      launch_rpass<1>(a, 1);
+      if (m - 2 >= 2) launch_rpass<2>(a, 2);
+      for (int j = 3; j <= m - 2; ++j) {
+        if (prof) NLS_HIP(hipEventRecord(next_event(), s));
+        launch_rpass<3>(a, j);
+        if (prof) NLS_HIP(hipEventRecord(next_event(), s));
+      }
+    } else {
+      launch_pass<DIM, 0>(a, 0);
+      if (m - 2 >= 1) launch_pass<DIM, 1>(a, 1);
+      for (int j = 2; j <= m - 2; ++j) {
+        if (prof) NLS_HIP(hipEventRecord(next_event(), s));
+        launch_pass<DIM, 2>(a, j);
+        if (prof) NLS_HIP(hipEventRecord(next_event(), s));
+      }
     }
     hipLaunchKernelGGL(k_kge_small, dim3(ntraj * NSEQ), dim3(kWave), 0, s, a, m - 2);
     if (snapshot) {
       SnapArgs sn{d_snapU + (size_t)slot * lay.cells, d_snapV + (size_t)slot * lay.cells};
-      hipLaunchKernelGGL((k_kge_combine_snap<DIM>), grid, blk, 0, s, a, sn, FBS * cpt);
+      hipLaunchKernelGGL((k_kge_combine_snap<DIM>), grid, blk, 0, s, a, sn);
     } else {
-      hipLaunchKernelGGL((k_kge_combine_rt<DIM>), grid, blk, 0, s, a, FBS * cpt);
+      hipLaunchKernelGGL((k_kge_combine_rt<DIM>), grid, blk, 0, s, a);
     }
     check_launch("fused kge step");
   }
@@ -1385,8 +1789,12 @@ struct nls_batch {
   // timing runs leave the trajectory untouched) and keeps the fastest.
   bool kind_ok(int kind, int c) const {
     const int nm = nmax_of();
+    if (nlse()) return kind == 0 && (c == 4 || c == 8 || c == 16); // k_nlse_pass tiles
+    // row-march kernels take any range length (tile = target cells per workgroup / 256)
+    if (kind == 3) return p.dim == 2 && 3 * nm <= RKS * FBS && 2 * nm <= MSUB &&
+                          rmarch2d_lds_bytes(nm) <= 65536 && c >= 4 && c <= 64;
     if (kind == 2) return p.dim == 2 && 2 * nm <= MSUB && march2d_lds_bytes(nm) <= 65536 &&
-                          (c == 16 || c == 32 || c == 64);
+                          c >= 4 && c <= 64;
     if (kind == 1) return p.dim == 2 && (c == 4 || c == 8) && pass2d_lds_bytes(c, nm) <= 65536;
     return c == 4 || c == 8 || c == 16;
   }
@@ -1395,7 +1803,9 @@ struct nls_batch {
   // buffers (a step only writes the next-state buffer and Krylov scratch, so
   // timing runs leave the trajectory untouched) and keeps the fastest.
   void autotune() {
-    const int cands[8][2] = {{2, 32}, {2, 16}, {2, 64}, {1, 4}, {1, 8}, {0, 4}, {0, 8}, {0, 16}};
+    const int cands[][2] = {{3, 12}, {3, 16}, {3, 24}, {3, 32}, {3, 36}, {3, 48}, {3, 64},
+                            {2, 12}, {2, 16}, {2, 24}, {2, 32}, {2, 36}, {2, 48}, {2, 64},
+                            {1, 4},  {1, 8},  {0, 4},  {0, 8},  {0, 16}};
     float best = 1e30f;
     int best_cpt = 4, best_kind = 0;
     hipEvent_t e0, e1;
@@ -1403,6 +1813,7 @@ struct nls_batch {
     NLS_HIP(hipEventCreate(&e1));
     for (auto &cd : cands) {
       if (!kind_ok(cd[0], cd[1])) continue;
+      if (p.pass_kernel > 0 && cd[0] != p.pass_kernel - 1) continue;
       pass_kind = cd[0];
       set_layout(cd[1]);
       launch_step((int)(step % nstates()), false, 0, false); // warm-up
@@ -1446,8 +1857,9 @@ struct nls_batch {
     if (!tuned) {
       if (p.cells_per_thread > 0) {
         const int c = p.cells_per_thread;
-        pass_kind = kind_ok(2, c) ? 2 : kind_ok(1, c) ? 1 : 0;
-        NLS_REQUIRE(kind_ok(pass_kind, c), "cells_per_thread not supported for this batch");
+        if (p.pass_kernel > 0) pass_kind = p.pass_kernel - 1;
+        else pass_kind = kind_ok(3, c) ? 3 : kind_ok(2, c) ? 2 : kind_ok(1, c) ? 1 : 0;
+        NLS_REQUIRE(kind_ok(pass_kind, c), "cells_per_thread / pass_kernel not supported for this batch");
         set_layout(c);
         tuned = true;
       } else {
@@ -1493,9 +1905,11 @@ struct nls_batch {
   }
 
   double pass_bytes() const {
-    // Algorithmic bytes of one general Lanczos pass per cell: read w'_{j-1},
-    // v_{j-1} for both sequences and c (4 * 5 B), write v_j, w'_j for both (4 * 4 B).
-    return 36.0 * (double)total_cells();
+    // Algorithmic bytes of one general Lanczos pass per cell.  Stored w': read
+    // w'_{j-1}, v_{j-1} for both sequences and c (4 * 5 B), write v_j, w'_j for both
+    // (4 * 4 B).  Recomputed w' (kind 3): read v_{j-1}, v_{j-2} for both sequences
+    // and c (4 * 5 B), write v_j for both (4 * 2 B).
+    return (pass_kind == 3 ? 28.0 : 36.0) * (double)total_cells();
   }
   int64_t total_cells() const {
     int64_t c = 0;
@@ -1558,10 +1972,10 @@ int nls_batch_create(const nls_batch_params *p, int ntraj, const nls_traj_desc *
       NLS_REQUIRE(p->method == NLS_NLSE_STRANG, "fused NLSE batch supports the Strang method");
     NLS_REQUIRE(p->dim == 2 || p->dim == 3, "dim must be 2 or 3");
     NLS_REQUIRE(p->dt > 0, "dt must be positive");
-    NLS_REQUIRE(p->cells_per_thread == 0 || p->cells_per_thread == 4 ||
-                    p->cells_per_thread == 8 || p->cells_per_thread == 16 ||
-                    p->cells_per_thread == 32 || p->cells_per_thread == 64,
-                "cells_per_thread must be 0 (autotune), 4, 8, 16, 32 or 64");
+    NLS_REQUIRE(p->pass_kernel >= 0 && p->pass_kernel <= 4, "pass_kernel must be in [0, 4]");
+    NLS_REQUIRE(p->cells_per_thread == 0 ||
+                    (p->cells_per_thread >= 4 && p->cells_per_thread <= 64),
+                "cells_per_thread must be 0 (autotune) or in [4, 64]");
     for (int t = 0; t < ntraj; ++t) NLS_REQUIRE(trajs[t].n >= 3 && trajs[t].L > 0, "bad trajectory");
     auto b = std::make_unique<nls_batch>();
     b->p = *p;
@@ -1572,7 +1986,8 @@ int nls_batch_create(const nls_batch_params *p, int ntraj, const nls_traj_desc *
     NLS_HIP(hipStreamCreate(&b->s));
     // layout with the smallest tile (most chunks) sizes the chunk map and partials
     Layout l4 = make_layout(trajs, ntraj, p->dim, 4, p->dt, p->mu2);
-    b->cmap_cap = l4.cmap.size() + 8 * NXCD;
+    // any tile has at most the chunks of the smallest one; a bin holds at most all of them
+    b->cmap_cap = (size_t)NXCD * (l4.total_chunks + 1);
     b->d_traj = b->alloc<TrajDev>(ntraj);
     b->d_cmap = b->alloc<int2>(b->cmap_cap);
     b->d_scal = b->alloc<TrajScal>(ntraj);

@@ -1,0 +1,644 @@
+// HDF5 trajectory files (layout of scripts/real_launcher_2d.py:201-245,
+// scripts/real_launcher_3d.py:206-252, scripts/complex_launcher_2d.py:193-236)
+// written through libhdf5's C API, loaded with dlopen so the rest of libnls has no
+// link-time dependency on it.  Types and constants below are those of the HDF5 1.10
+// public API (H5public.h, H5Ipublic.h, H5Spublic.h, H5Tpublic.h, H5Fpublic.h).
+//
+// Python's h5py, which the reference launchers use, stores str attributes as
+// variable-length UTF-8 strings, int as 64-bit integers, float as doubles and
+// complex128 as a compound {r, i} of doubles; the files here use the same types, so
+// readers of the reference's files read these unchanged.
+#include "nls_h5.hpp"
+#include "nls_common.hpp"
+
+#include <dlfcn.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <deque>
+#include <mutex>
+#include <thread>
+
+namespace nls {
+namespace h5 {
+namespace {
+
+using hid_t = int64_t;
+using herr_t = int;
+using htri_t = int;
+using hsize_t = unsigned long long;
+constexpr hid_t P_DEFAULT = 0, S_ALL = 0, E_DEFAULT = 0;
+constexpr unsigned ACC_RDONLY = 0u, ACC_TRUNC = 2u;
+constexpr int S_SCALAR = 0, S_SELECT_SET = 0, T_COMPOUND = 6, T_CSET_UTF8 = 1;
+constexpr hsize_t S_UNLIMITED = ~0ull;
+constexpr size_t T_VARIABLE = ~(size_t)0;
+
+struct Api {
+  void *lib = nullptr;
+  std::string error, version;
+  herr_t (*open)(void);
+  herr_t (*get_libversion)(unsigned *, unsigned *, unsigned *);
+  herr_t (*eset_auto)(hid_t, void *, void *);
+  hid_t (*fcreate)(const char *, unsigned, hid_t, hid_t);
+  hid_t (*fopen)(const char *, unsigned, hid_t);
+  herr_t (*fclose)(hid_t);
+  herr_t (*fflush)(hid_t, int);
+  hid_t (*gcreate)(hid_t, const char *, hid_t, hid_t, hid_t);
+  herr_t (*gclose)(hid_t);
+  hid_t (*screate)(int);
+  hid_t (*screate_simple)(int, const hsize_t *, const hsize_t *);
+  int (*sget_ndims)(hid_t);
+  int (*sget_dims)(hid_t, hsize_t *, hsize_t *);
+  herr_t (*sselect_hyperslab)(hid_t, int, const hsize_t *, const hsize_t *, const hsize_t *,
+                              const hsize_t *);
+  herr_t (*sclose)(hid_t);
+  hid_t (*dcreate)(hid_t, const char *, hid_t, hid_t, hid_t, hid_t, hid_t);
+  hid_t (*dopen)(hid_t, const char *, hid_t);
+  herr_t (*dwrite)(hid_t, hid_t, hid_t, hid_t, hid_t, const void *);
+  herr_t (*dread)(hid_t, hid_t, hid_t, hid_t, hid_t, void *);
+  herr_t (*dset_extent)(hid_t, const hsize_t *);
+  hid_t (*dget_space)(hid_t);
+  hid_t (*dget_type)(hid_t);
+  herr_t (*dclose)(hid_t);
+  hid_t (*pcreate)(hid_t);
+  herr_t (*pset_chunk)(hid_t, int, const hsize_t *);
+  herr_t (*pclose)(hid_t);
+  hid_t (*acreate)(hid_t, const char *, hid_t, hid_t, hid_t, hid_t);
+  hid_t (*aopen_by_name)(hid_t, const char *, const char *, hid_t, hid_t);
+  herr_t (*awrite)(hid_t, hid_t, const void *);
+  herr_t (*aread)(hid_t, hid_t, void *);
+  hid_t (*aget_type)(hid_t);
+  herr_t (*aclose)(hid_t);
+  hid_t (*tcopy)(hid_t);
+  herr_t (*tset_size)(hid_t, size_t);
+  herr_t (*tset_cset)(hid_t, int);
+  hid_t (*tcreate)(int, size_t);
+  herr_t (*tinsert)(hid_t, const char *, size_t, hid_t);
+  int (*tget_class)(hid_t);
+  htri_t (*tis_variable_str)(hid_t);
+  herr_t (*tclose)(hid_t);
+  herr_t (*free_memory)(void *);
+  hid_t *native_double, *native_llong, *c_s1, *dataset_create;
+};
+
+std::mutex g_mu; // every HDF5 call runs under this lock
+
+template <class F> bool bind(Api &a, F &f, const char *name) {
+  f = reinterpret_cast<F>(dlsym(a.lib, name));
+  if (!f) a.error = std::string("libhdf5 lacks ") + name;
+  return f != nullptr;
+}
+
+Api *load() {
+  static Api api;
+  static bool tried = false;
+  if (tried) return api.lib && api.error.empty() ? &api : nullptr;
+  tried = true;
+  std::vector<std::string> names;
+  if (const char *e = std::getenv("NLS_HDF5_LIB")) names.push_back(e);
+  names.insert(names.end(), {"libhdf5.so", "libhdf5.so.103", "/opt/conda/lib/libhdf5.so"});
+  for (auto &n : names) {
+    api.lib = dlopen(n.c_str(), RTLD_NOW | RTLD_LOCAL);
+    if (api.lib) break;
+  }
+  if (!api.lib) {
+    api.error = "libhdf5 not found (set NLS_HDF5_LIB to its path)";
+    return nullptr;
+  }
+  Api &a = api;
+  bool ok = bind(a, a.open, "H5open") && bind(a, a.get_libversion, "H5get_libversion") &&
+            bind(a, a.eset_auto, "H5Eset_auto2") && bind(a, a.fcreate, "H5Fcreate") &&
+            bind(a, a.fopen, "H5Fopen") && bind(a, a.fclose, "H5Fclose") &&
+            bind(a, a.fflush, "H5Fflush") && bind(a, a.gcreate, "H5Gcreate2") &&
+            bind(a, a.gclose, "H5Gclose") && bind(a, a.screate, "H5Screate") &&
+            bind(a, a.screate_simple, "H5Screate_simple") &&
+            bind(a, a.sget_ndims, "H5Sget_simple_extent_ndims") &&
+            bind(a, a.sget_dims, "H5Sget_simple_extent_dims") &&
+            bind(a, a.sselect_hyperslab, "H5Sselect_hyperslab") && bind(a, a.sclose, "H5Sclose") &&
+            bind(a, a.dcreate, "H5Dcreate2") && bind(a, a.dopen, "H5Dopen2") &&
+            bind(a, a.dwrite, "H5Dwrite") && bind(a, a.dread, "H5Dread") &&
+            bind(a, a.dset_extent, "H5Dset_extent") && bind(a, a.dget_space, "H5Dget_space") &&
+            bind(a, a.dget_type, "H5Dget_type") && bind(a, a.dclose, "H5Dclose") &&
+            bind(a, a.pcreate, "H5Pcreate") && bind(a, a.pset_chunk, "H5Pset_chunk") &&
+            bind(a, a.pclose, "H5Pclose") && bind(a, a.acreate, "H5Acreate2") &&
+            bind(a, a.aopen_by_name, "H5Aopen_by_name") && bind(a, a.awrite, "H5Awrite") &&
+            bind(a, a.aread, "H5Aread") && bind(a, a.aget_type, "H5Aget_type") &&
+            bind(a, a.aclose, "H5Aclose") && bind(a, a.tcopy, "H5Tcopy") &&
+            bind(a, a.tset_size, "H5Tset_size") && bind(a, a.tset_cset, "H5Tset_cset") &&
+            bind(a, a.tcreate, "H5Tcreate") && bind(a, a.tinsert, "H5Tinsert") &&
+            bind(a, a.tget_class, "H5Tget_class") &&
+            bind(a, a.tis_variable_str, "H5Tis_variable_str") && bind(a, a.tclose, "H5Tclose") &&
+            bind(a, a.free_memory, "H5free_memory") &&
+            bind(a, a.native_double, "H5T_NATIVE_DOUBLE_g") &&
+            bind(a, a.native_llong, "H5T_NATIVE_LLONG_g") && bind(a, a.c_s1, "H5T_C_S1_g") &&
+            bind(a, a.dataset_create, "H5P_CLS_DATASET_CREATE_ID_g");
+  if (!ok) return nullptr;
+  if (a.open() < 0) {
+    a.error = "H5open failed";
+    return nullptr;
+  }
+  a.eset_auto(E_DEFAULT, nullptr, nullptr); // errors come back as status codes only
+  unsigned ma = 0, mi = 0, re = 0;
+  a.get_libversion(&ma, &mi, &re);
+  a.version = std::to_string(ma) + "." + std::to_string(mi) + "." + std::to_string(re);
+  return &api;
+}
+
+Api &need() {
+  Api *a = load();
+  if (!a) throw Error("HDF5 unavailable: libhdf5 could not be loaded (set NLS_HDF5_LIB)");
+  return *a;
+}
+
+hid_t chk(hid_t id, const std::string &what) {
+  if (id < 0) throw Error("HDF5: " + what + " failed");
+  return id;
+}
+
+// Scoped id closed by the matching H5*close.
+struct Id {
+  hid_t id;
+  herr_t (*close)(hid_t);
+  Id(hid_t i, herr_t (*c)(hid_t)) : id(i), close(c) {}
+  ~Id() {
+    if (id >= 0) close(id);
+  }
+  Id(const Id &) = delete;
+  Id &operator=(const Id &) = delete;
+};
+
+hid_t complex_type(Api &a) {
+  hid_t t = chk(a.tcreate(T_COMPOUND, 16), "H5Tcreate");
+  a.tinsert(t, "r", 0, *a.native_double);
+  a.tinsert(t, "i", 8, *a.native_double);
+  return t;
+}
+
+void attr_str(Api &a, hid_t obj, const char *name, const std::string &v) {
+  Id t(chk(a.tcopy(*a.c_s1), "H5Tcopy"), a.tclose);
+  a.tset_size(t.id, T_VARIABLE);
+  a.tset_cset(t.id, T_CSET_UTF8);
+  Id sp(chk(a.screate(S_SCALAR), "H5Screate"), a.sclose);
+  Id at(chk(a.acreate(obj, name, t.id, sp.id, P_DEFAULT, P_DEFAULT), std::string("attribute ") + name),
+        a.aclose);
+  const char *p = v.c_str();
+  if (a.awrite(at.id, t.id, &p) < 0) throw Error(std::string("HDF5: writing attribute ") + name);
+}
+
+void attr_f64(Api &a, hid_t obj, const char *name, double v) {
+  Id sp(chk(a.screate(S_SCALAR), "H5Screate"), a.sclose);
+  Id at(chk(a.acreate(obj, name, *a.native_double, sp.id, P_DEFAULT, P_DEFAULT),
+            std::string("attribute ") + name),
+        a.aclose);
+  if (a.awrite(at.id, *a.native_double, &v) < 0) throw Error(std::string("HDF5: writing ") + name);
+}
+
+void attr_i64(Api &a, hid_t obj, const char *name, long long v) {
+  Id sp(chk(a.screate(S_SCALAR), "H5Screate"), a.sclose);
+  Id at(chk(a.acreate(obj, name, *a.native_llong, sp.id, P_DEFAULT, P_DEFAULT),
+            std::string("attribute ") + name),
+        a.aclose);
+  if (a.awrite(at.id, *a.native_llong, &v) < 0) throw Error(std::string("HDF5: writing ") + name);
+}
+
+void dataset(Api &a, hid_t loc, const char *name, const std::vector<hsize_t> &dims,
+             const double *data, bool cplx) {
+  Id sp(chk(a.screate_simple((int)dims.size(), dims.data(), nullptr), "H5Screate_simple"), a.sclose);
+  Id ty(cplx ? complex_type(a) : a.tcopy(*a.native_double), a.tclose);
+  Id ds(chk(a.dcreate(loc, name, ty.id, sp.id, P_DEFAULT, P_DEFAULT, P_DEFAULT),
+            std::string("dataset ") + name),
+        a.dclose);
+  if (a.dwrite(ds.id, ty.id, S_ALL, S_ALL, P_DEFAULT, data) < 0)
+    throw Error(std::string("HDF5: writing dataset ") + name);
+}
+
+std::string now_string() {
+  // str(datetime.datetime.now()): "YYYY-MM-DD HH:MM:SS.ffffff"
+  const auto tp = std::chrono::system_clock::now();
+  const std::time_t t = std::chrono::system_clock::to_time_t(tp);
+  const long us = (long)(std::chrono::duration_cast<std::chrono::microseconds>(
+                             tp.time_since_epoch()).count() % 1000000);
+  std::tm lt{};
+  localtime_r(&t, &lt);
+  char buf[64];
+  std::strftime(buf, sizeof buf, "%Y-%m-%d %H:%M:%S", &lt);
+  char out[80];
+  std::snprintf(out, sizeof out, "%s.%06ld", buf, us);
+  return out;
+}
+
+std::vector<hsize_t> grid_dims(const Run &r) {
+  if (r.dim == 3) return {(hsize_t)r.nx, (hsize_t)r.ny, (hsize_t)r.nz};
+  return {(hsize_t)r.nx, (hsize_t)r.ny};
+}
+
+std::vector<hsize_t> traj_dims(const Run &r, hsize_t ns) {
+  if (r.dim == 3) return {ns, (hsize_t)r.ox, (hsize_t)r.oy, (hsize_t)r.oz};
+  return {ns, (hsize_t)r.ox, (hsize_t)r.oy};
+}
+
+// Everything except the trajectory datasets u, v.
+void write_header(Api &a, hid_t f, const Run &r) {
+  {
+    Id g(chk(a.gcreate(f, "metadata", P_DEFAULT, P_DEFAULT, P_DEFAULT), "group metadata"), a.gclose);
+    attr_str(a, g.id, "problem_type", r.problem_type);
+    attr_str(a, g.id, "boundary_condition", "noflux");
+    attr_str(a, g.id, "run_id", r.run_id);
+    attr_i64(a, g.id, "run_index", r.run_index);
+    attr_str(a, g.id, "timestamp", r.timestamp.empty() ? now_string() : r.timestamp);
+    attr_f64(a, g.id, "elapsed_time", r.elapsed_time);
+    attr_str(a, g.id, "phenomenon", r.phenomenon);
+    for (auto &kv : r.params) attr_str(a, g.id, ("phenomenon_" + kv.first).c_str(), kv.second);
+  }
+  {
+    Id g(chk(a.gcreate(f, "grid", P_DEFAULT, P_DEFAULT, P_DEFAULT), "group grid"), a.gclose);
+    attr_i64(a, g.id, "nx", r.nx);
+    attr_i64(a, g.id, "ny", r.ny);
+    if (r.dim == 3) attr_i64(a, g.id, "nz", r.nz);
+    attr_f64(a, g.id, "Lx", r.Lx);
+    attr_f64(a, g.id, "Ly", r.Ly);
+    if (r.dim == 3) attr_f64(a, g.id, "Lz", r.Lz);
+  }
+  {
+    Id g(chk(a.gcreate(f, "time", P_DEFAULT, P_DEFAULT, P_DEFAULT), "group time"), a.gclose);
+    attr_f64(a, g.id, "T", r.T);
+    attr_i64(a, g.id, "nt", r.nt);
+    attr_i64(a, g.id, "num_snapshots", r.num_snapshots);
+  }
+  const auto gd = grid_dims(r);
+  {
+    Id g(chk(a.gcreate(f, "initial_condition", P_DEFAULT, P_DEFAULT, P_DEFAULT), "group ic"),
+         a.gclose);
+    dataset(a, g.id, "u0", gd, r.u0.data(), r.complex_field);
+    if (!r.complex_field) dataset(a, g.id, "v0", gd, r.v0.data(), false);
+  }
+  {
+    Id g(chk(a.gcreate(f, "focusing", P_DEFAULT, P_DEFAULT, P_DEFAULT), "group focusing"), a.gclose);
+    attr_str(a, g.id, "type", r.m_type);
+    dataset(a, g.id, "m", gd, r.m.data(), false);
+    if (!r.complex_field) dataset(a, g.id, "c", gd, r.c.data(), false);
+  }
+  if (r.complex_field) dataset(a, f, "c", gd, r.c.data(), false);
+  // X, Y: meshgrid(linspace(-L, L, n) ..., indexing='ij') on the solver grid
+  const size_t N = (size_t)r.nx * r.ny * (r.dim == 3 ? r.nz : 1);
+  std::vector<double> X(N), Y(N);
+  // numpy.linspace(-L, L, n): k * step + start with step = (stop - start) / (n - 1),
+  // the last point set to stop
+  auto lin = [](double L, int n, int k) {
+    if (n <= 1) return -L;
+    if (k == n - 1) return L;
+    const double step = (L - (-L)) / (double)(n - 1);
+    return (double)k * step + (-L);
+  };
+  const int nz = r.dim == 3 ? r.nz : 1;
+  for (int i = 0; i < r.nx; ++i)
+    for (int j = 0; j < r.ny; ++j)
+      for (int k = 0; k < nz; ++k) {
+        const size_t idx = ((size_t)i * r.ny + j) * nz + k;
+        X[idx] = lin(r.Lx, r.nx, i);
+        Y[idx] = lin(r.Ly, r.ny, j);
+      }
+  dataset(a, f, "X", gd, X.data(), false);
+  dataset(a, f, "Y", gd, Y.data(), false);
+}
+
+void check_run(const Run &r) {
+  NLS_REQUIRE(!r.path.empty(), "HDF5 run: empty path");
+  NLS_REQUIRE(r.dim == 2 || r.dim == 3, "HDF5 run: dim must be 2 or 3");
+  NLS_REQUIRE(r.nx > 0 && r.ny > 0 && r.nz > 0, "HDF5 run: bad grid");
+  const size_t N = (size_t)r.nx * r.ny * (r.dim == 3 ? r.nz : 1);
+  const size_t cw = r.complex_field ? 2 : 1;
+  NLS_REQUIRE(r.u0.size() == N * cw && r.m.size() == N && r.c.size() == N,
+              "HDF5 run: initial condition / coefficient arrays missing");
+  NLS_REQUIRE(r.complex_field || r.v0.size() == N, "HDF5 run: v0 missing");
+}
+
+} // namespace
+
+bool available(std::string *version) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Api *a = load();
+  if (a && version) *version = a->version;
+  return a != nullptr;
+}
+
+Run from_abi(const nls_h5_run &r) {
+  NLS_REQUIRE(r.path, "HDF5 run: null path");
+  Run o;
+  o.path = r.path;
+  o.complex_field = r.complex_field != 0;
+  o.dim = r.dim;
+  o.nx = r.nx;
+  o.ny = r.ny;
+  o.nz = r.dim == 3 ? r.nz : 1;
+  o.Lx = r.Lx;
+  o.Ly = r.Ly;
+  o.Lz = r.Lz;
+  o.T = r.T;
+  o.nt = r.nt;
+  o.num_snapshots = r.num_snapshots;
+  auto str = [](const char *s) { return s ? std::string(s) : std::string(); };
+  o.problem_type = str(r.problem_type);
+  o.run_id = str(r.run_id);
+  o.phenomenon = str(r.phenomenon);
+  o.m_type = str(r.m_type);
+  o.timestamp = str(r.timestamp);
+  o.run_index = r.run_index;
+  o.elapsed_time = r.elapsed_time;
+  NLS_REQUIRE(r.n_params == 0 || (r.param_keys && r.param_values), "HDF5 run: null params");
+  for (int k = 0; k < r.n_params; ++k)
+    o.params.emplace_back(str(r.param_keys[k]), str(r.param_values[k]));
+  NLS_REQUIRE(o.dim == 2 || o.dim == 3, "HDF5 run: dim must be 2 or 3");
+  NLS_REQUIRE(o.nx > 0 && o.ny > 0 && o.nz > 0, "HDF5 run: bad grid");
+  const size_t N = (size_t)o.nx * o.ny * o.nz, cw = o.complex_field ? 2 : 1;
+  NLS_REQUIRE(r.u0 && r.m && r.c && (o.complex_field || r.v0), "HDF5 run: null grid array");
+  o.u0.assign(r.u0, r.u0 + N * cw);
+  if (!o.complex_field) o.v0.assign(r.v0, r.v0 + N);
+  o.m.assign(r.m, r.m + N);
+  o.c.assign(r.c, r.c + N);
+  o.ns_out = r.ns_out;
+  o.ox = r.ox;
+  o.oy = r.oy;
+  o.oz = r.dim == 3 ? r.oz : 1;
+  NLS_REQUIRE(o.ns_out >= 0 && o.ox > 0 && o.oy > 0 && o.oz > 0, "HDF5 run: bad trajectory shape");
+  const size_t T = (size_t)o.ns_out * o.ox * o.oy * o.oz;
+  NLS_REQUIRE(T == 0 || (r.u && (o.complex_field || r.v)), "HDF5 run: null trajectory");
+  if (T) {
+    o.u.assign(r.u, r.u + T * cw);
+    if (!o.complex_field) o.v.assign(r.v, r.v + T);
+  }
+  return o;
+}
+
+void write_run(const Run &r) {
+  check_run(r);
+  const size_t cw = r.complex_field ? 2 : 1;
+  const size_t T = (size_t)r.ns_out * r.ox * r.oy * r.oz;
+  NLS_REQUIRE(r.u.size() == T * cw && (r.complex_field || r.v.size() == T),
+              "HDF5 run: trajectory size does not match ns_out * ox * oy * oz");
+  std::lock_guard<std::mutex> lk(g_mu);
+  Api &a = need();
+  Id f(chk(a.fcreate(r.path.c_str(), ACC_TRUNC, P_DEFAULT, P_DEFAULT), "create " + r.path), a.fclose);
+  write_header(a, f.id, r);
+  const auto td = traj_dims(r, (hsize_t)r.ns_out);
+  dataset(a, f.id, "u", td, r.u.data(), r.complex_field);
+  if (!r.complex_field) dataset(a, f.id, "v", td, r.v.data(), false);
+}
+
+Stream::Stream(const Run &h) {
+  check_run(h);
+  complex_ = h.complex_field;
+  has_v_ = !h.complex_field;
+  std::lock_guard<std::mutex> lk(g_mu);
+  Api &a = need();
+  file_ = chk(a.fcreate(h.path.c_str(), ACC_TRUNC, P_DEFAULT, P_DEFAULT), "create " + h.path);
+  try {
+    write_header(a, file_, h);
+    auto td = traj_dims(h, 0);
+    auto mx = td;
+    mx[0] = S_UNLIMITED;
+    auto ch = td;
+    ch[0] = 1; // one snapshot per chunk
+    dims_.assign(td.begin(), td.end());
+    Id sp(chk(a.screate_simple((int)td.size(), td.data(), mx.data()), "H5Screate_simple"), a.sclose);
+    Id pl(chk(a.pcreate(*a.dataset_create), "H5Pcreate"), a.pclose);
+    a.pset_chunk(pl.id, (int)ch.size(), ch.data());
+    Id ty(complex_ ? complex_type(a) : a.tcopy(*a.native_double), a.tclose);
+    du_ = chk(a.dcreate(file_, "u", ty.id, sp.id, P_DEFAULT, pl.id, P_DEFAULT), "dataset u");
+    if (has_v_)
+      dv_ = chk(a.dcreate(file_, "v", ty.id, sp.id, P_DEFAULT, pl.id, P_DEFAULT), "dataset v");
+    a.fflush(file_, 0);
+  } catch (...) {
+    if (du_ >= 0) a.dclose(du_);
+    if (dv_ >= 0) a.dclose(dv_);
+    a.fclose(file_);
+    file_ = du_ = dv_ = -1;
+    throw;
+  }
+}
+
+void Stream::append(const double *u, const double *v) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  NLS_REQUIRE(file_ >= 0, "HDF5 stream: closed");
+  Api &a = need();
+  auto put = [&](hid_t d, const double *x) {
+    std::vector<hsize_t> ext(dims_.begin(), dims_.end());
+    ext[0] = (hsize_t)count_ + 1;
+    if (a.dset_extent(d, ext.data()) < 0) throw Error("HDF5: extending a trajectory dataset");
+    Id fs(chk(a.dget_space(d), "H5Dget_space"), a.sclose);
+    std::vector<hsize_t> start(ext.size(), 0), cnt(ext.begin(), ext.end());
+    start[0] = (hsize_t)count_;
+    cnt[0] = 1;
+    a.sselect_hyperslab(fs.id, S_SELECT_SET, start.data(), nullptr, cnt.data(), nullptr);
+    Id ms(chk(a.screate_simple((int)cnt.size(), cnt.data(), nullptr), "H5Screate_simple"), a.sclose);
+    Id ty(complex_ ? complex_type(a) : a.tcopy(*a.native_double), a.tclose);
+    if (a.dwrite(d, ty.id, ms.id, fs.id, P_DEFAULT, x) < 0)
+      throw Error("HDF5: appending a snapshot");
+  };
+  put(du_, u);
+  if (has_v_) put(dv_, v);
+  ++count_;
+  a.fflush(file_, 0);
+}
+
+void Stream::close(double elapsed_time) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (file_ < 0) return;
+  Api &a = need();
+  {
+    // elapsed_time is known only now: rewrite the attribute written at open
+    Id at(a.aopen_by_name(file_, "metadata", "elapsed_time", P_DEFAULT, P_DEFAULT), a.aclose);
+    if (at.id >= 0) a.awrite(at.id, *a.native_double, &elapsed_time);
+  }
+  a.dclose(du_);
+  if (dv_ >= 0) a.dclose(dv_);
+  a.fclose(file_);
+  file_ = du_ = dv_ = -1;
+}
+
+Stream::~Stream() {
+  try {
+    close(0.0);
+  } catch (...) {
+  }
+}
+
+} // namespace h5
+} // namespace nls
+
+// ---------------------------------------------------------------- asynchronous writer
+struct nls_h5_writer {
+  size_t max_bytes = 0, queued = 0;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_space, cv_idle;
+  std::deque<std::unique_ptr<nls::h5::Run>> q;
+  bool stop = false, busy = false;
+  int64_t written = 0;
+  std::string first_error;
+  std::thread worker;
+
+  void loop() {
+    for (;;) {
+      std::unique_ptr<nls::h5::Run> r;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv_work.wait(lk, [&] { return stop || !q.empty(); });
+        if (q.empty()) return;
+        r = std::move(q.front());
+        q.pop_front();
+        busy = true;
+      }
+      std::string err;
+      try {
+        nls::h5::write_run(*r);
+      } catch (const std::exception &e) {
+        err = e.what();
+      }
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        queued -= r->bytes();
+        busy = false;
+        if (err.empty()) ++written;
+        else if (first_error.empty()) first_error = r->path + ": " + err;
+      }
+      cv_space.notify_all();
+      cv_idle.notify_all();
+    }
+  }
+
+  ~nls_h5_writer() {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stop = true;
+    }
+    cv_work.notify_all();
+    if (worker.joinable()) worker.join();
+  }
+};
+
+extern "C" {
+
+int nls_h5_available(char *version, int len) {
+  return nls::guard([&] {
+    std::string v;
+    NLS_REQUIRE(nls::h5::available(&v), "libhdf5 could not be loaded (set NLS_HDF5_LIB)");
+    if (version && len > 0) {
+      std::strncpy(version, v.c_str(), (size_t)len - 1);
+      version[len - 1] = '\0';
+    }
+  });
+}
+
+int nls_h5_write_run(const nls_h5_run *r) {
+  return nls::guard([&] {
+    NLS_REQUIRE(r, "nls_h5_write_run: null run");
+    nls::h5::write_run(nls::h5::from_abi(*r));
+  });
+}
+
+int nls_h5_writer_create(size_t max_queued_bytes, nls_h5_writer **out) {
+  return nls::guard([&] {
+    NLS_REQUIRE(out, "nls_h5_writer_create: null out");
+    NLS_REQUIRE(nls::h5::available(), "libhdf5 could not be loaded (set NLS_HDF5_LIB)");
+    auto w = std::make_unique<nls_h5_writer>();
+    w->max_bytes = max_queued_bytes > 0 ? max_queued_bytes : ((size_t)1 << 30);
+    w->worker = std::thread([raw = w.get()] { raw->loop(); });
+    *out = w.release();
+  });
+}
+
+int nls_h5_writer_submit(nls_h5_writer *w, const nls_h5_run *r) {
+  return nls::guard([&] {
+    NLS_REQUIRE(w && r, "nls_h5_writer_submit: null argument");
+    auto run = std::make_unique<nls::h5::Run>(nls::h5::from_abi(*r));
+    const size_t b = run->bytes();
+    std::unique_lock<std::mutex> lk(w->mu);
+    // wait for room unless the queue is empty (a single large file always fits)
+    w->cv_space.wait(lk, [&] { return w->queued == 0 || w->queued + b <= w->max_bytes; });
+    w->queued += b;
+    w->q.push_back(std::move(run));
+    lk.unlock();
+    w->cv_work.notify_one();
+  });
+}
+
+int nls_h5_writer_drain(nls_h5_writer *w, int64_t *files_written) {
+  return nls::guard([&] {
+    NLS_REQUIRE(w, "nls_h5_writer_drain: null writer");
+    std::unique_lock<std::mutex> lk(w->mu);
+    w->cv_idle.wait(lk, [&] { return w->q.empty() && !w->busy; });
+    if (files_written) *files_written = w->written;
+    NLS_REQUIRE(w->first_error.empty(), "HDF5 writer: " + w->first_error);
+  });
+}
+
+int nls_h5_writer_destroy(nls_h5_writer *w) { return nls::guard([&] { delete w; }); }
+
+int nls_h5_read_dataset(const char *path, const char *name, double *buf, int64_t cap,
+                        int64_t *dims, int *ndim, int *is_complex) {
+  using namespace nls::h5;
+  return nls::guard([&] {
+    NLS_REQUIRE(path && name, "nls_h5_read_dataset: null argument");
+    std::lock_guard<std::mutex> lk(g_mu);
+    Api &a = need();
+    Id f(chk(a.fopen(path, ACC_RDONLY, P_DEFAULT), std::string("open ") + path), a.fclose);
+    Id d(chk(a.dopen(f.id, name, P_DEFAULT), std::string("dataset ") + name), a.dclose);
+    Id sp(chk(a.dget_space(d.id), "H5Dget_space"), a.sclose);
+    const int nd = a.sget_ndims(sp.id);
+    NLS_REQUIRE(nd >= 0 && nd <= 4, "nls_h5_read_dataset: rank above 4");
+    hsize_t dd[4] = {0, 0, 0, 0};
+    a.sget_dims(sp.id, dd, nullptr);
+    size_t count = 1;
+    for (int k = 0; k < nd; ++k) count *= dd[k];
+    if (ndim) *ndim = nd;
+    if (dims)
+      for (int k = 0; k < 4; ++k) dims[k] = k < nd ? (int64_t)dd[k] : 0;
+    Id ft(chk(a.dget_type(d.id), "H5Dget_type"), a.tclose);
+    const bool cplx = a.tget_class(ft.id) == T_COMPOUND;
+    if (is_complex) *is_complex = cplx ? 1 : 0;
+    if (!buf) return;
+    const size_t need_d = count * (cplx ? 2 : 1);
+    NLS_REQUIRE((size_t)cap >= need_d, "nls_h5_read_dataset: buffer too small");
+    Id mt(cplx ? complex_type(a) : a.tcopy(*a.native_double), a.tclose);
+    if (a.dread(d.id, mt.id, S_ALL, S_ALL, P_DEFAULT, buf) < 0)
+      throw nls::Error(std::string("HDF5: reading ") + name);
+  });
+}
+
+int nls_h5_read_attr(const char *path, const char *obj, const char *name, double *num, char *str,
+                     int len, int *is_string) {
+  using namespace nls::h5;
+  return nls::guard([&] {
+    NLS_REQUIRE(path && obj && name, "nls_h5_read_attr: null argument");
+    std::lock_guard<std::mutex> lk(g_mu);
+    Api &a = need();
+    Id f(chk(a.fopen(path, ACC_RDONLY, P_DEFAULT), std::string("open ") + path), a.fclose);
+    Id at(chk(a.aopen_by_name(f.id, obj, name, P_DEFAULT, P_DEFAULT),
+              std::string("attribute ") + obj + "/" + name),
+          a.aclose);
+    Id ty(chk(a.aget_type(at.id), "H5Aget_type"), a.tclose);
+    const bool s = a.tis_variable_str(ty.id) > 0;
+    if (is_string) *is_string = s ? 1 : 0;
+    if (s) {
+      Id mt(chk(a.tcopy(*a.c_s1), "H5Tcopy"), a.tclose);
+      a.tset_size(mt.id, T_VARIABLE);
+      a.tset_cset(mt.id, T_CSET_UTF8);
+      char *p = nullptr;
+      if (a.aread(at.id, mt.id, &p) < 0) throw nls::Error("HDF5: reading attribute");
+      if (str && len > 0) {
+        std::strncpy(str, p ? p : "", (size_t)len - 1);
+        str[len - 1] = '\0';
+      }
+      if (p) a.free_memory(p);
+    } else {
+      double v = 0;
+      if (a.aread(at.id, *a.native_double, &v) < 0) throw nls::Error("HDF5: reading attribute");
+      if (num) *num = v;
+    }
+  });
+}
+
+} // extern "C"

@@ -40,12 +40,29 @@ class BatchParams(C.Structure):
     _fields_ = [("equation", C.c_int), ("method", C.c_int), ("dim", C.c_int), ("dt", C.c_double),
                 ("krylov_dim", C.c_int), ("mu2", C.c_double), ("q6", C.c_int),
                 ("gamma", C.c_double), ("use_graph", C.c_int), ("snapshot_every", C.c_int),
-                ("max_snapshots", C.c_int), ("cells_per_thread", C.c_int)]
+                ("max_snapshots", C.c_int), ("cells_per_thread", C.c_int),
+                ("pass_kernel", C.c_int)]
 
 
 class ServerParams(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_size_t), ("max_active", C.c_int), ("L", C.c_double),
-                ("mode", C.c_int), ("gamma", C.c_double), ("max_steps_per_group", C.c_int)]
+                ("mode", C.c_int), ("gamma", C.c_double), ("max_steps_per_group", C.c_int),
+                ("h5_dir", C.c_char_p), ("run_id", C.c_char_p)]
+
+
+class H5Run(C.Structure):
+    _fields_ = [("path", C.c_char_p), ("complex_field", C.c_int), ("dim", C.c_int),
+                ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("Lx", C.c_double), ("Ly", C.c_double), ("Lz", C.c_double), ("T", C.c_double),
+                ("nt", C.c_int), ("num_snapshots", C.c_int),
+                ("problem_type", C.c_char_p), ("run_id", C.c_char_p), ("phenomenon", C.c_char_p),
+                ("m_type", C.c_char_p), ("timestamp", C.c_char_p), ("run_index", C.c_int),
+                ("elapsed_time", C.c_double), ("n_params", C.c_int),
+                ("param_keys", C.POINTER(C.c_char_p)), ("param_values", C.POINTER(C.c_char_p)),
+                ("u0", C.POINTER(C.c_double)), ("v0", C.POINTER(C.c_double)),
+                ("m", C.POINTER(C.c_double)), ("c", C.POINTER(C.c_double)),
+                ("ns_out", C.c_int), ("ox", C.c_int), ("oy", C.c_int), ("oz", C.c_int),
+                ("u", C.POINTER(C.c_double)), ("v", C.POINTER(C.c_double))]
 
 
 class Request(C.Structure):
@@ -155,6 +172,16 @@ _SIGS = {
     "nls_server_release": [_P, C.c_int64],
     "nls_server_get_stats": [_P, C.POINTER(ServerStats)],
     "nls_pool_simulate": [C.c_size_t, C.c_int, _I64, _I64],
+    "nls_h5_available": [C.c_char_p, C.c_int],
+    "nls_h5_write_run": [C.POINTER(H5Run)],
+    "nls_h5_writer_create": [C.c_size_t, C.POINTER(C.c_void_p)],
+    "nls_h5_writer_submit": [_P, C.POINTER(H5Run)],
+    "nls_h5_writer_drain": [_P, _I64],
+    "nls_h5_writer_destroy": [_P],
+    "nls_h5_read_dataset": [C.c_char_p, C.c_char_p, _D, C.c_int64, _I64, C.POINTER(C.c_int),
+                            C.POINTER(C.c_int)],
+    "nls_h5_read_attr": [C.c_char_p, C.c_char_p, C.c_char_p, _D, C.c_char_p, C.c_int,
+                         C.POINTER(C.c_int)],
 }
 _RESTYPE = {"nls_last_error": C.c_char_p, "nls_laplacian_nnz": C.c_int64}
 

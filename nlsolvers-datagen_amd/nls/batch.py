@@ -14,7 +14,8 @@ from ._lib import as_f64, call, dptr
 class FusedBatch:
     def __init__(self, ns_list, Ls, dt, equation="kge", method="gautschi", dim=2, krylov_dim=0,
                  mu2=0.0, q6=True, gamma=0.0, use_graph=True, snapshot_every=0,
-                 max_snapshots=0, sigmas=None, seeds=None, cells_per_thread=0):
+                 max_snapshots=0, sigmas=None, seeds=None, cells_per_thread=0,
+                 pass_kernel=None):
         eq = {"kge": L.EQ_KGE, "nlse": L.EQ_NLSE, "sp4": L.EQ_SP4}[equation]
         meth = {"gautschi": L.KGE_GAUTSCHI, "sewi": L.KGE_SEWI, "sv": L.KGE_SV,
                 "strang": L.NLSE_STRANG, "reference": L.SP4_REFERENCE,
@@ -27,7 +28,8 @@ class FusedBatch:
         sigmas = list(sigmas) if sigmas is not None else [0.0] * k
         seeds = list(seeds) if seeds is not None else [0] * k
         self.p = L.BatchParams(eq, meth, dim, dt, krylov_dim, mu2, int(q6), gamma, int(use_graph),
-                               snapshot_every, max_snapshots, cells_per_thread)
+                               snapshot_every, max_snapshots, cells_per_thread,
+                               0 if pass_kernel is None else pass_kernel + 1)
         self.td = (L.TrajDesc * k)(*[L.TrajDesc(n, l, s, sd)
                                      for n, l, s, sd in zip(self.ns, Ls, sigmas, seeds)])
         self.h = C.c_void_p()
@@ -39,7 +41,7 @@ class FusedBatch:
         k = len(ns_list)
         Ls = list(Ls) if np.ndim(Ls) else [Ls] * k
         eq = {"kge": L.EQ_KGE, "nlse": L.EQ_NLSE, "sp4": L.EQ_SP4}[equation]
-        p = L.BatchParams(eq, 0, dim, dt, krylov_dim, mu2, 1, 0.0, 1, 0, max_snapshots, 0)
+        p = L.BatchParams(eq, 0, dim, dt, krylov_dim, mu2, 1, 0.0, 1, 0, max_snapshots, 0, 0)
         td = (L.TrajDesc * k)(*[L.TrajDesc(n, l, 0.0, 0) for n, l in zip(ns_list, Ls)])
         out = C.c_size_t()
         call("nls_batch_estimate_bytes", C.byref(p), k, td, C.byref(out))

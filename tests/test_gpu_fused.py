@@ -26,16 +26,18 @@ def make_case(n, seed, traj, L=10.0, ring=False):
     return m, c, u0, v0
 
 
-@pytest.mark.parametrize("cpt", [4, 32])  # LDS-tiled chunk / row-march pass kernels
-def test_fused_kge_uneven_batch_vs_oracle(orc, cpt):
+# (tile, pass kernel): LDS-tiled chunk, row march, row march storing only the basis
+# (384^2 takes its window-shift path through registers, 3n > 1024)
+@pytest.mark.parametrize("cpt,kernel", [(4, 1), (32, 2), (16, 3), (64, 3)])
+def test_fused_kge_uneven_batch_vs_oracle(orc, cpt, kernel):
     from nls.batch import FusedBatch
-    ns_list = [48, 64, 33, 130]
+    ns_list = [48, 64, 33, 130] + ([384] if kernel == 3 else [])
     L, dt, nt, ns = 10.0, 0.005, 201, 5
     every = nt // ns  # the reference cadence: freq = nt / num_snapshots
     b = FusedBatch(ns_list, L, dt, krylov_dim=10, mu2=1.0, snapshot_every=every,
-                   max_snapshots=ns, cells_per_thread=cpt)
+                   max_snapshots=ns, cells_per_thread=cpt, pass_kernel=kernel)
     b.run(0)  # settles the tile
-    assert b.tile()["pass_kind"] == {4: 1, 32: 2}[cpt]
+    assert b.tile()["pass_kind"] == kernel
     cases = [make_case(n, 2, t, ring=(t == 1)) for t, n in enumerate(ns_list)]
     for t, (m, c, u0, v0) in enumerate(cases):
         b.set_initial(t, m, c, u0, v0)
@@ -71,23 +73,45 @@ def test_fused_kge_3d_vs_oracle(orc, method):
             assert rel_l2(u[k], ur[k]) < TOL, (q6, k)
 
 
-@pytest.mark.parametrize("cpt", [8, 32])
-def test_fused_alone_equals_in_batch_bitwise(cpt):
+@pytest.mark.parametrize("cpt,kernel", [(8, 1), (32, 2), (16, 3)])
+def test_fused_alone_equals_in_batch_bitwise(cpt, kernel):
     from nls.batch import FusedBatch
     L, dt, steps = 10.0, 0.005, 40
     mix = [64, 40, 96, 64, 300]
     target = 2
     cases = [make_case(n, 4, t) for t, n in enumerate(mix)]
-    b = FusedBatch(mix, L, dt, krylov_dim=8, mu2=1.0, cells_per_thread=cpt)
+    b = FusedBatch(mix, L, dt, krylov_dim=8, mu2=1.0, cells_per_thread=cpt, pass_kernel=kernel)
     for t, cs in enumerate(cases):
         b.set_initial(t, *cs)
     b.run(steps)
     u_mix, _ = b.state(target)
-    a = FusedBatch([mix[target]], L, dt, krylov_dim=8, mu2=1.0, cells_per_thread=cpt)
+    a = FusedBatch([mix[target]], L, dt, krylov_dim=8, mu2=1.0, cells_per_thread=cpt,
+                   pass_kernel=kernel)
     a.set_initial(0, *cases[target])
     a.run(steps)
     u_alone, _ = a.state(0)
     assert np.array_equal(u_mix, u_alone)
+
+
+@pytest.mark.parametrize("m", [3, 4, 5])
+def test_fused_basis_only_pass_small_krylov(orc, m):
+    """The basis-only pass kernel at the smallest Krylov dimensions, where the last
+    pass is the j = 1 or j = 2 variant and stores w' for the combine."""
+    from nls.batch import FusedBatch
+    ns_list = [64, 40]
+    L, dt, nt = 10.0, 0.005, 41
+    cases = [make_case(n, 7, t) for t, n in enumerate(ns_list)]
+    b = FusedBatch(ns_list, L, dt, krylov_dim=m, mu2=1.0, snapshot_every=nt // 2,
+                   max_snapshots=2, cells_per_thread=16, pass_kernel=3)
+    for t, cs in enumerate(cases):
+        b.set_initial(t, *cs)
+    b.run(nt - 1)
+    assert b.tile()["pass_kind"] == 3
+    for t, n in enumerate(ns_list):
+        m_, c, u0, v0 = cases[t]
+        ur, _ = orc.kge_run(n, L, dt * nt, nt, 2, m_, c, u0, v0, krylov=m, mu2=1.0)
+        u, _ = b.snapshots(t)
+        assert rel_l2(u[1], ur[1]) < TOL, t
 
 
 def test_fused_graph_replay_equals_eager():
