@@ -296,7 +296,6 @@ __global__ __launch_bounds__(kWave) void k_kge_small(KgeArgs a, int last_pass) {
   const TrajDev T = a.traj[t];
   __shared__ SmallWS ws;
   __shared__ double red[NSEQ * PN];
-  __shared__ double yr[kMaxKrylov];
   const int m = a.m;
   // reduce last-pass partials (one wave = whole block)
   {
@@ -313,26 +312,25 @@ __global__ __launch_bounds__(kWave) void k_kge_small(KgeArgs a, int last_pass) {
   }
   __syncthreads();
   TrajScal &S = a.scal[t];
-  if (threadIdx.x == 0) {
-    double al, b;
-    lanczos_scalars(red[0], red[1], red[2], al, b);
+  const int lane = threadIdx.x;
+  double al, b;
+  lanczos_scalars(red[0], red[1], red[2], al, b);
+  if (lane == 0) {
     S.alpha[s][m - 2] = al;
     S.beta[s][m - 1] = b;
-    for (int k = 0; k < m - 1; ++k) ws.d[k] = S.alpha[s][k];
-    ws.d[m - 1] = 0.0; // Q1
-    for (int k = 0; k < m - 1; ++k) ws.e[k] = S.beta[s][k + 1];
-    for (int k = 0; k < m; ++k) { ws.phr[k] = 1.0; ws.phi[k] = 0.0; }
   }
-  __syncthreads();
+  // lane k: d_k = alpha_k (d_{m-1} = 0, Q1), e_k = b_{k+1}
+  double dk = lane < m - 2 ? S.alpha[s][lane] : (lane == m - 2 ? al : 0.0);
+  double ek = lane < m - 2 ? S.beta[s][lane + 1] : (lane == m - 2 ? b : 0.0);
   const double b0 = S.b0[s];
-  if (small_ql(m, ws)) {
-    if (threadIdx.x == 0) S.status = 1;
+  if (small_ql_wave(m, dk, ek, ws.Z)) {
+    if (lane == 0) S.status = 1;
     return;
   }
   const int kind = s == 0 ? F_COSM1_SQRT : a.kind1;
-  small_apply_f(m, ws, kind, T.tau, 0.0, b0, yr, nullptr);
-  __syncthreads();
-  if (threadIdx.x < m) S.y[s][threadIdx.x] = yr[threadIdx.x];
+  double y, yi;
+  small_apply_f_wave(m, dk, ws.Z, kind, T.tau, 0.0, b0, 1.0, 0.0, y, yi);
+  if (lane < m) S.y[s][lane] = y;
 }
 
 // ---------------------------------------------------------------- combine
@@ -985,32 +983,38 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax,
   const int oU = -dir * n, oL = -dir, oR = dir, oD = dir * n;
   const float rn = 1.0f / (float)n;
   auto cell = [&](int l) { return org + dir * l; };
-  auto seed = [&](int s, int i) -> float {
-    const double uu = uT[i];
-    if (s == 0) return (float)uu;
-    return (float)(-(double)mT[i] * uu * uu * uu);
-  };
-  // raw inputs of cell i: c, X (both sequences), P (both sequences)
-  auto fetch = [&](int i, float &c, float &x0, float &x1, float &p0, float &p1) {
+  // raw inputs of cell i: c, X (both sequences), P (both sequences), and u, m for
+  // the seeds; nothing is computed on them here, so the loads stay in flight until
+  // deposit() uses them
+  auto fetch = [&](int i, float &c, float &x0, float &x1, float &p0, float &p1, double &uu,
+                   float &mm) {
     c = cT[i];
-    if (STAGE == 0) {
-      x0 = seed(0, i);
-      x1 = seed(1, i);
-      p0 = p1 = 0.0f;
-    } else if (STAGE == 1) {
+    if (STAGE <= 1) {
+      uu = uT[i];
+      mm = mT[i];
+    }
+    if (STAGE >= 1) {
       x0 = X0[i];
       x1 = X1[i];
-      p0 = seed(0, i);
-      p1 = seed(1, i);
-    } else {
-      x0 = X0[i];
-      x1 = X1[i];
+    }
+    if (STAGE >= 2) {
       p0 = P0[i];
       p1 = P1[i];
     }
   };
-  auto deposit = [&](int e, float c, float x0, float x1, float p0, float p1) {
+  auto deposit = [&](int e, float c, float x0, float x1, float p0, float p1, double uu,
+                     float mm) {
     sc[e] = c;
+    if (STAGE <= 1) { // seeds x0 = u, x1 = -m u^3 (FP64 -> FP32)
+      const float s0 = (float)uu, s1 = (float)(-(double)mm * uu * uu * uu);
+      if (STAGE == 0) {
+        x0 = s0;
+        x1 = s1;
+      } else {
+        p0 = s0;
+        p1 = s1;
+      }
+    }
     if (STAGE == 0) {
       sv0[e] = x0;
       sv1[e] = x1;
@@ -1023,19 +1027,23 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax,
   };
   // first window [-n, SUB + n), MSPT loads per thread in flight before their deposits
   for (int e0 = 0; e0 < MSUB + 2 * n; e0 += MSPT * FBS) {
-    float rc[MSPT], x0[MSPT], x1[MSPT], p0[MSPT], p1[MSPT];
+    float rc[MSPT], x0[MSPT], x1[MSPT], p0[MSPT], p1[MSPT], rm[MSPT];
+    double ru[MSPT];
 #pragma unroll
     for (int q = 0; q < MSPT; ++q) {
       const int e = e0 + threadIdx.x + q * FBS, i = cell(e - n);
-      if (e < MSUB + 2 * n && i >= 0 && i < N) fetch(i, rc[q], x0[q], x1[q], p0[q], p1[q]);
+      if (e < MSUB + 2 * n && i >= 0 && i < N)
+        fetch(i, rc[q], x0[q], x1[q], p0[q], p1[q], ru[q], rm[q]);
     }
 #pragma unroll
     for (int q = 0; q < MSPT; ++q) {
       const int e = e0 + threadIdx.x + q * FBS, i = cell(e - n);
-      if (e < MSUB + 2 * n && i >= 0 && i < N) deposit(e, rc[q], x0[q], x1[q], p0[q], p1[q]);
+      if (e < MSUB + 2 * n && i >= 0 && i < N)
+        deposit(e, rc[q], x0[q], x1[q], p0[q], p1[q], ru[q], rm[q]);
     }
   }
-  float nc[MSPT], nx0[MSPT], nx1[MSPT], np0[MSPT], np1[MSPT];
+  float nc[MSPT], nx0[MSPT], nx1[MSPT], np0[MSPT], np1[MSPT], nm[MSPT];
+  double nu[MSPT];
   for (int cb = 0; cb < len; cb += MSUB) {
     const int ce = cb + MSUB < len ? cb + MSUB : len;
     const bool more = cb + MSUB < len;
@@ -1044,7 +1052,7 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax,
 #pragma unroll
       for (int q = 0; q < MSPT; ++q) {
         const int i = cell(cb + MSUB + n + threadIdx.x + q * FBS);
-        if (i >= 0 && i < N) fetch(i, nc[q], nx0[q], nx1[q], np0[q], np1[q]);
+        if (i >= 0 && i < N) fetch(i, nc[q], nx0[q], nx1[q], np0[q], np1[q], nu[q], nm[q]);
       }
     }
     __syncthreads();
@@ -1091,7 +1099,8 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax,
 #pragma unroll
     for (int q = 0; q < MSPT; ++q) {
       const int i = cell(cb + MSUB + n + threadIdx.x + q * FBS);
-      if (i >= 0 && i < N) deposit(2 * n + threadIdx.x + q * FBS, nc[q], nx0[q], nx1[q], np0[q], np1[q]);
+      if (i >= 0 && i < N)
+        deposit(2 * n + threadIdx.x + q * FBS, nc[q], nx0[q], nx1[q], np0[q], np1[q], nu[q], nm[q]);
     }
   }
   block_sum_n<NSEQ * PN>(acc, shn);
@@ -1450,7 +1459,6 @@ __global__ __launch_bounds__(kWave) void k_nlse_small(NlseArgs a, int last_pass)
   const TrajDev T = a.traj[t];
   __shared__ SmallWS ws;
   __shared__ double red[PN];
-  __shared__ double yr[kMaxKrylov], yi[kMaxKrylov];
   const int m = a.m;
   {
     const int lane = threadIdx.x;
@@ -1465,26 +1473,24 @@ __global__ __launch_bounds__(kWave) void k_nlse_small(NlseArgs a, int last_pass)
   }
   __syncthreads();
   TrajScal &S = a.scal[t];
-  if (threadIdx.x == 0) {
-    double al, b;
-    lanczos_scalars(red[0], red[1], red[2], al, b);
+  const int lane = threadIdx.x;
+  double al, b;
+  lanczos_scalars(red[0], red[1], red[2], al, b);
+  if (lane == 0) {
     S.alpha[0][m - 2] = al;
     S.beta[0][m - 1] = b;
-    for (int k = 0; k < m - 1; ++k) ws.d[k] = S.alpha[0][k];
-    ws.d[m - 1] = 0.0; // Q1
-    for (int k = 0; k < m - 1; ++k) ws.e[k] = S.beta[0][k + 1];
-    for (int k = 0; k < m; ++k) { ws.phr[k] = 1.0; ws.phi[k] = 0.0; }
   }
-  __syncthreads();
-  if (small_ql(m, ws)) {
-    if (threadIdx.x == 0) S.status = 1;
+  double dk = lane < m - 2 ? S.alpha[0][lane] : (lane == m - 2 ? al : 0.0); // d_{m-1} = 0 (Q1)
+  double ek = lane < m - 2 ? S.beta[0][lane + 1] : (lane == m - 2 ? b : 0.0);
+  if (small_ql_wave(m, dk, ek, ws.Z)) {
+    if (lane == 0) S.status = 1;
     return;
   }
-  small_apply_f(m, ws, F_EXP, 0.0, a.dt, S.b0[0], yr, yi);
-  __syncthreads();
-  if (threadIdx.x < m) {
-    S.y[0][threadIdx.x] = yr[threadIdx.x];
-    S.y[1][threadIdx.x] = yi[threadIdx.x];
+  double yr, yi;
+  small_apply_f_wave(m, dk, ws.Z, F_EXP, 0.0, a.dt, S.b0[0], 1.0, 0.0, yr, yi);
+  if (lane < m) {
+    S.y[0][lane] = yr;
+    S.y[1][lane] = yi;
   }
 }
 

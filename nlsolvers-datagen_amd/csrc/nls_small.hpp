@@ -154,4 +154,103 @@ __device__ inline void small_apply_f(int m, SmallWS &ws, int kind, double t_re, 
   }
 }
 
+// ---------------------------------------------------------------- register-resident variant
+// Same implicit QL and the same f evaluation as small_ql / small_apply_f, organised
+// for a single wavefront with no barriers: lane k < m keeps d_k and e_k in registers,
+// every lane follows the (uniform) rotation recurrence reading d_i, e_i with
+// readlane, lane k applies each rotation to its own row of Z (LDS, rows never
+// read by other lanes during the sweep), and lane i evaluates f(lambda_i) for its
+// own eigenvalue.  r = sqrt(f^2 + g^2) replaces hypot: the entries of a Lanczos T
+// are far from the overflow range.  Used by the fused batch, where this stage is
+// on the critical path of every step.
+__device__ inline double lane_read(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// dk, ek: d_lane, e_lane on entry (lanes >= m ignored); eigenvalues in dk on exit.
+// Z: row stride kMaxKrylov + 1.  Returns 0 on success (uniform).
+__device__ inline int small_ql_wave(int m, double &dk, double &ek, double (*Z)[kMaxKrylov + 1]) {
+  const int lane = threadIdx.x & (kWave - 1);
+  if (lane < m)
+    for (int i = 0; i < m; ++i) Z[lane][i] = lane == i ? 1.0 : 0.0;
+  if (lane == m - 1) ek = 0.0;
+  for (int l = 0; l < m; ++l) {
+    int iter = 0;
+    for (;;) {
+      int mm;
+      for (mm = l; mm < m - 1; ++mm) {
+        const double dd = fabs(lane_read(dk, mm)) + fabs(lane_read(dk, mm + 1));
+        if (fabs(lane_read(ek, mm)) <= 2.220446049250313e-16 * dd) break;
+      }
+      if (mm == l) break;
+      if (++iter > 60) return 1;
+      const double dl = lane_read(dk, l), el = lane_read(ek, l);
+      double g = (lane_read(dk, l + 1) - dl) / (2.0 * el);
+      double r = sqrt(g * g + 1.0);
+      g = lane_read(dk, mm) - dl + el / (g + (g >= 0.0 ? r : -r));
+      double s = 1.0, c = 1.0, p = 0.0;
+      bool deflated = false;
+      for (int i = mm - 1; i >= l; --i) {
+        const double ei = lane_read(ek, i);
+        const double f = s * ei, b = c * ei;
+        r = sqrt(f * f + g * g);
+        if (lane == i + 1) ek = r;
+        if (r == 0.0) {
+          const double di1 = lane_read(dk, i + 1);
+          if (lane == i + 1) dk = di1 - p;
+          if (lane == mm) ek = 0.0;
+          deflated = true;
+          break;
+        }
+        const double ir = 1.0 / r;
+        s = f * ir;
+        c = g * ir;
+        g = lane_read(dk, i + 1) - p;
+        r = (lane_read(dk, i) - g) * s + 2.0 * c * b;
+        p = s * r;
+        const double dnew = g + p;
+        g = c * r - b;
+        if (lane < m) {
+          const double zf = Z[lane][i + 1], zi = Z[lane][i];
+          Z[lane][i + 1] = s * zi + c * zf;
+          Z[lane][i] = c * zi - s * zf;
+        }
+        if (lane == i + 1) dk = dnew;
+      }
+      if (deflated) continue;
+      if (lane == l) {
+        dk -= p;
+        ek = g;
+      }
+      if (lane == mm) ek = 0.0;
+    }
+  }
+  return 0;
+}
+
+// After small_ql_wave: y_k = beta0 * ph_k * sum_i Z[k][i] f(lambda_i) Z[0][i] for lanes
+// k < m (ph_k = phr + i phi, 1 for a real symmetric T).
+__device__ inline void small_apply_f_wave(int m, double lam, double (*Z)[kMaxKrylov + 1],
+                                          int kind, double t_re, double t_im, double beta0,
+                                          double phr, double phi, double &yr, double &yi) {
+  const int lane = threadIdx.x & (kWave - 1);
+  cplx f = {0.0, 0.0};
+  if (lane < m) f = eval_f(kind, t_re, t_im, lam);
+  __syncthreads(); // rows of Z written by every lane are read below
+  double ar = 0.0, ai = 0.0;
+  for (int i = 0; i < m; ++i) {
+    const double fr = lane_read(f.re, i), fi = lane_read(f.im, i);
+    if (lane < m) {
+      const double w = Z[lane][i] * Z[0][i];
+      ar += w * fr;
+      ai += w * fi;
+    }
+  }
+  yr = beta0 * (phr * ar - phi * ai);
+  yi = beta0 * (phr * ai + phi * ar);
+}
+
 } // namespace nls
