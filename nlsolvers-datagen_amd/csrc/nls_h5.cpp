@@ -47,6 +47,7 @@ struct Api {
   herr_t (*fclose)(hid_t);
   herr_t (*fflush)(hid_t, int);
   hid_t (*gcreate)(hid_t, const char *, hid_t, hid_t, hid_t);
+  hid_t (*gopen)(hid_t, const char *, hid_t);
   herr_t (*gclose)(hid_t);
   hid_t (*screate)(int);
   hid_t (*screate_simple)(int, const hsize_t *, const hsize_t *);
@@ -113,6 +114,7 @@ Api *load() {
             bind(a, a.eset_auto, "H5Eset_auto2") && bind(a, a.fcreate, "H5Fcreate") &&
             bind(a, a.fopen, "H5Fopen") && bind(a, a.fclose, "H5Fclose") &&
             bind(a, a.fflush, "H5Fflush") && bind(a, a.gcreate, "H5Gcreate2") &&
+            bind(a, a.gopen, "H5Gopen2") &&
             bind(a, a.gclose, "H5Gclose") && bind(a, a.screate, "H5Screate") &&
             bind(a, a.screate_simple, "H5Screate_simple") &&
             bind(a, a.sget_ndims, "H5Sget_simple_extent_ndims") &&
@@ -150,6 +152,7 @@ Api *load() {
 Api &need() {
   Api *a = load();
   if (!a) throw Error("HDF5 unavailable: libhdf5 could not be loaded (set NLS_HDF5_LIB)");
+  a->eset_auto(E_DEFAULT, nullptr, nullptr); // per thread: errors come back as status codes
   return *a;
 }
 
@@ -240,8 +243,9 @@ std::vector<hsize_t> traj_dims(const Run &r, hsize_t ns) {
   return {ns, (hsize_t)r.ox, (hsize_t)r.oy};
 }
 
-// Everything except the trajectory datasets u, v.
-void write_header(Api &a, hid_t f, const Run &r) {
+// Everything except the trajectory datasets u, v (and elapsed_time when a stream
+// writes it at close).
+void write_header(Api &a, hid_t f, const Run &r, bool with_elapsed = true) {
   {
     Id g(chk(a.gcreate(f, "metadata", P_DEFAULT, P_DEFAULT, P_DEFAULT), "group metadata"), a.gclose);
     attr_str(a, g.id, "problem_type", r.problem_type);
@@ -249,7 +253,7 @@ void write_header(Api &a, hid_t f, const Run &r) {
     attr_str(a, g.id, "run_id", r.run_id);
     attr_i64(a, g.id, "run_index", r.run_index);
     attr_str(a, g.id, "timestamp", r.timestamp.empty() ? now_string() : r.timestamp);
-    attr_f64(a, g.id, "elapsed_time", r.elapsed_time);
+    if (with_elapsed) attr_f64(a, g.id, "elapsed_time", r.elapsed_time);
     attr_str(a, g.id, "phenomenon", r.phenomenon);
     for (auto &kv : r.params) attr_str(a, g.id, ("phenomenon_" + kv.first).c_str(), kv.second);
   }
@@ -396,7 +400,7 @@ Stream::Stream(const Run &h) {
   Api &a = need();
   file_ = chk(a.fcreate(h.path.c_str(), ACC_TRUNC, P_DEFAULT, P_DEFAULT), "create " + h.path);
   try {
-    write_header(a, file_, h);
+    write_header(a, file_, h, false);
     auto td = traj_dims(h, 0);
     auto mx = td;
     mx[0] = S_UNLIMITED;
@@ -449,9 +453,8 @@ void Stream::close(double elapsed_time) {
   if (file_ < 0) return;
   Api &a = need();
   {
-    // elapsed_time is known only now: rewrite the attribute written at open
-    Id at(a.aopen_by_name(file_, "metadata", "elapsed_time", P_DEFAULT, P_DEFAULT), a.aclose);
-    if (at.id >= 0) a.awrite(at.id, *a.native_double, &elapsed_time);
+    Id g(a.gopen(file_, "metadata", P_DEFAULT), a.gclose); // elapsed_time is known only now
+    if (g.id >= 0) attr_f64(a, g.id, "elapsed_time", elapsed_time);
   }
   a.dclose(du_);
   if (dv_ >= 0) a.dclose(dv_);

@@ -14,8 +14,14 @@
 // or completion of any active request; snapshots and final states go to host
 // memory, finished requests release their memory, and new ones are admitted
 // between groups.  Latency = completion - submission, wall clock.
+//
+// With an HDF5 directory set, each request's file (the launchers' per-run layout)
+// is opened at admission and every snapshot is appended as it reaches host memory;
+// the HDF5 calls run on a writer thread of their own, so disk never stalls the
+// device loop.
 #include "nls.h"
 #include "nls_common.hpp"
+#include "nls_h5.hpp"
 #include "nls_philox.hpp"
 #include "nls_pool.hpp"
 
@@ -26,6 +32,7 @@
 #include <cmath>
 #include <condition_variable>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -104,6 +111,8 @@ struct Request {
   nls_request q;
   std::vector<float> mf, cf;
   std::vector<double> u0, v0;
+  std::vector<double> m64, c64;        // caller's fields, kept for the HDF5 header
+  std::shared_ptr<h5::Stream> stream;  // touched by the writer thread only
   int64_t N = 0;
   size_t bytes = 0, off = SIZE_MAX;
   int64_t steps_done = 0;
@@ -141,6 +150,14 @@ struct nls_server {
   int64_t completed = 0, failed = 0;
   std::vector<double> latencies;
   std::string fatal;
+  // HDF5 streaming: jobs run in order on their own thread
+  std::string h5_dir, run_id;
+  std::thread h5_thread;
+  std::mutex h5_mu;
+  std::condition_variable h5_cv, h5_idle;
+  std::deque<std::function<void()>> h5_jobs;
+  bool h5_stop = false, h5_busy = false;
+  std::string h5_error;
   // device work lists
   ReqDev *d_reqs = nullptr;
   int2 *d_cmap = nullptr;
@@ -152,6 +169,12 @@ struct nls_server {
     stop = true;
     cv.notify_all();
     if (worker.joinable()) worker.join();
+    {
+      std::lock_guard<std::mutex> lk(h5_mu);
+      h5_stop = true;
+    }
+    h5_cv.notify_all();
+    if (h5_thread.joinable()) h5_thread.join();
     if (s) (void)hipStreamSynchronize(s);
     if (slab) (void)hipFree(slab);
     if (d_reqs) (void)hipFree(d_reqs);
@@ -159,6 +182,77 @@ struct nls_server {
     if (h_reqs) (void)hipHostFree(h_reqs);
     if (h_cmap) (void)hipHostFree(h_cmap);
     if (s) (void)hipStreamDestroy(s);
+  }
+
+  void h5_loop() {
+    for (;;) {
+      std::function<void()> job;
+      {
+        std::unique_lock<std::mutex> lk(h5_mu);
+        h5_cv.wait(lk, [&] { return h5_stop || !h5_jobs.empty(); });
+        if (h5_jobs.empty()) return;
+        job = std::move(h5_jobs.front());
+        h5_jobs.pop_front();
+        h5_busy = true;
+      }
+      std::string err;
+      try {
+        job();
+      } catch (const std::exception &e) {
+        err = e.what();
+      }
+      {
+        std::lock_guard<std::mutex> lk(h5_mu);
+        h5_busy = false;
+        if (!err.empty() && h5_error.empty()) h5_error = err;
+      }
+      h5_idle.notify_all();
+    }
+  }
+
+  void h5_push(std::function<void()> job) {
+    {
+      std::lock_guard<std::mutex> lk(h5_mu);
+      h5_jobs.push_back(std::move(job));
+    }
+    h5_cv.notify_one();
+  }
+
+  // Opens the request's file (header, snapshot 0 = initial state) on the writer thread.
+  void h5_open(const std::shared_ptr<Request> &r) {
+    auto hdr = std::make_shared<h5::Run>();
+    char name[64];
+    std::snprintf(name, sizeof name, "_%04lld.h5", (long long)r->id);
+    hdr->path = h5_dir + "/run_" + run_id + name;
+    hdr->dim = 2;
+    hdr->nx = hdr->ny = hdr->ox = hdr->oy = r->q.n;
+    hdr->Lx = hdr->Ly = p.L;
+    hdr->T = r->q.dt * r->q.nt;
+    hdr->nt = r->q.nt;
+    hdr->num_snapshots = r->q.num_snapshots;
+    hdr->problem_type = "phi4";
+    hdr->run_id = run_id;
+    hdr->run_index = (int)r->id;
+    hdr->phenomenon = "stochastic";
+    hdr->m_type = "custom";
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%.17g", r->q.sigma);
+    hdr->params.emplace_back("sigma", buf);
+    std::snprintf(buf, sizeof buf, "%llu", (unsigned long long)r->q.seed);
+    hdr->params.emplace_back("seed", buf);
+    std::snprintf(buf, sizeof buf, "%.17g", p.gamma);
+    hdr->params.emplace_back("gamma", buf);
+    std::snprintf(buf, sizeof buf, "%.17g", r->q.dt);
+    hdr->params.emplace_back("dt", buf);
+    hdr->u0 = r->u0;
+    hdr->v0 = r->v0;
+    hdr->m = std::move(r->m64);
+    hdr->c = std::move(r->c64);
+    const bool snap0 = r->q.num_snapshots > 0;
+    h5_push([r, hdr, snap0] {
+      r->stream = std::make_shared<h5::Stream>(*hdr);
+      if (snap0) r->stream->append(hdr->u0.data(), hdr->v0.data());
+    });
   }
 
   static size_t req_bytes(int64_t N) {
@@ -261,11 +355,13 @@ struct nls_server {
     return e;
   }
 
-  void copy_state(const Request &r, double *dst) {
+  void copy_state(const Request &r, double *dst, int back = 0) {
     const ReqDev d = desc(r);
     double *ring[3] = {d.u0, d.u1, d.u2};
-    NLS_HIP(hipMemcpyAsync(dst, ring[r.steps_done % 3], 8 * r.N, hipMemcpyDeviceToHost, s));
+    NLS_HIP(hipMemcpyAsync(dst, ring[(r.steps_done + 3 - back) % 3], 8 * r.N,
+                           hipMemcpyDeviceToHost, s));
   }
+
 
   void loop() {
     try {
@@ -283,7 +379,10 @@ struct nls_server {
           }
           act = active;
         }
-        for (auto &r : fresh) upload_initial(*r);
+        for (auto &r : fresh) {
+          if (!h5_dir.empty()) h5_open(r); // before the upload releases u0, v0
+          upload_initial(*r);
+        }
         // group length: until the first snapshot/completion among active requests
         int64_t K = p.max_steps_per_group > 0 ? p.max_steps_per_group : 64;
         for (auto &r : act) K = std::min(K, steps_to_event(*r));
@@ -329,10 +428,16 @@ struct nls_server {
         busy_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         // events: snapshots and completions
         std::vector<std::shared_ptr<Request>> finished;
+        std::vector<std::pair<std::shared_ptr<Request>, std::shared_ptr<std::vector<double>>>> prevs;
         for (auto &r : act) {
           if (r->snap_every > 0 && r->nsnap < r->q.num_snapshots && r->steps_done > 0 &&
               r->steps_done % r->snap_every == 0) {
             copy_state(*r, r->snaps.data() + (size_t)r->nsnap * r->N);
+            if (!h5_dir.empty()) { // u_{k-1} for the velocity (u_k - u_{k-1}) / dt
+              auto prev = std::make_shared<std::vector<double>>(r->N);
+              copy_state(*r, prev->data(), 1);
+              prevs.emplace_back(r, prev);
+            }
             r->nsnap++;
           }
           if (r->steps_done >= (int64_t)r->q.nt - 1) {
@@ -342,6 +447,19 @@ struct nls_server {
           }
         }
         NLS_HIP(hipStreamSynchronize(s));
+        for (auto &pr : prevs) { // append the snapshot just copied (writer thread)
+          auto r = pr.first;
+          auto u = std::make_shared<std::vector<double>>(
+              r->snaps.begin() + (size_t)(r->nsnap - 1) * r->N,
+              r->snaps.begin() + (size_t)r->nsnap * r->N);
+          auto up = pr.second;
+          const double dt = r->q.dt;
+          h5_push([r, u, up, dt] {
+            std::vector<double> v(u->size());
+            for (size_t i = 0; i < v.size(); ++i) v[i] = ((*u)[i] - (*up)[i]) / dt;
+            r->stream->append(u->data(), v.data());
+          });
+        }
         if (!finished.empty()) {
           std::lock_guard<std::mutex> lk(mu);
           const auto now = std::chrono::steady_clock::now();
@@ -355,6 +473,12 @@ struct nls_server {
             else { ++completed; latencies.push_back(r->latency_ms); }
             pool->release(r->off);
             active.erase(std::find(active.begin(), active.end(), r));
+            if (!h5_dir.empty()) {
+              const double el = r->latency_ms / 1e3;
+              h5_push([r, el] {
+                if (r->stream) r->stream->close(el);
+              });
+            }
           }
           done_cv.notify_all();
         }
@@ -380,6 +504,15 @@ int nls_server_create(const nls_server_params *p, nls_server **out) {
     NLS_REQUIRE(p->L > 0, "L must be positive");
     auto sv = std::make_unique<nls_server>();
     sv->p = *p;
+    if (p->h5_dir && *p->h5_dir) {
+      std::string ver;
+      NLS_REQUIRE(h5::available(&ver), "h5_dir set but libhdf5 could not be loaded");
+      sv->h5_dir = p->h5_dir;
+      sv->run_id = p->run_id && *p->run_id ? p->run_id : "serve";
+      sv->h5_thread = std::thread([raw = sv.get()] { raw->h5_loop(); });
+    }
+    sv->p.h5_dir = nullptr; // the strings live in sv->h5_dir / sv->run_id
+    sv->p.run_id = nullptr;
     NLS_HIP(hipStreamCreate(&sv->s));
     NLS_HIP(hipMalloc((void **)&sv->slab, p->hbm_budget_bytes));
     sv->pool = std::make_unique<SlabPool>(p->hbm_budget_bytes);
@@ -407,6 +540,12 @@ int nls_server_submit(nls_server *sv, const nls_request *q, const double *m, con
     for (int64_t i = 0; i < r->N; ++i) {
       r->mf[i] = m ? (float)m[i] : 1.0f;
       r->cf[i] = c ? (float)c[i] : 1.0f;
+    }
+    if (!sv->h5_dir.empty()) {
+      r->m64.assign(r->N, 1.0);
+      r->c64.assign(r->N, 1.0);
+      if (m) std::copy(m, m + r->N, r->m64.begin());
+      if (c) std::copy(c, c + r->N, r->c64.begin());
     }
     r->u0.assign(u0 ? u0 : nullptr, u0 ? u0 + r->N : nullptr);
     if (!u0) r->u0.assign(r->N, 0.0);
@@ -457,6 +596,15 @@ int nls_server_drain(nls_server *sv, double timeout_s) {
     });
     NLS_REQUIRE(ok, "timeout draining the server");
     NLS_REQUIRE(sv->fatal.empty(), "server stopped after a device error: " + sv->fatal);
+    lk.unlock();
+    if (!sv->h5_dir.empty()) { // files complete on disk
+      std::unique_lock<std::mutex> hl(sv->h5_mu);
+      const bool done = sv->h5_idle.wait_for(hl, std::chrono::duration<double>(timeout_s), [&] {
+        return sv->h5_jobs.empty() && !sv->h5_busy;
+      });
+      NLS_REQUIRE(done, "timeout waiting for the HDF5 writer");
+      NLS_REQUIRE(sv->h5_error.empty(), "HDF5 writer: " + sv->h5_error);
+    }
   });
 }
 

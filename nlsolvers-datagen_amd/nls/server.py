@@ -11,10 +11,14 @@ class TrajectoryServer:
     STATES = {0: "queued", 1: "running", 2: "done", 3: "failed"}
 
     def __init__(self, hbm_budget_bytes=8 << 30, max_active=256, L_=10.0, mode="damped",
-                 gamma=0.1, max_steps_per_group=64):
+                 gamma=0.1, max_steps_per_group=64, h5_dir=None, run_id=None):
+        """h5_dir: stream every request's snapshots to h5_dir/run_<run_id>_<id:04d>.h5
+        (launchers' per-run layout) from a background writer thread."""
         md = {"reference": L.SP4_REFERENCE, "damped": L.SP4_DAMPED}[mode]
         self.p = L.ServerParams(int(hbm_budget_bytes), max_active, L_, md, gamma,
-                                max_steps_per_group)
+                                max_steps_per_group,
+                                None if h5_dir is None else str(h5_dir).encode(),
+                                None if run_id is None else str(run_id).encode())
         self.h = C.c_void_p()
         self.ns = {}
         call("nls_server_create", C.byref(self.p), C.byref(self.h))

@@ -61,3 +61,38 @@ def test_server_budget_limits_concurrency():
     with pytest.raises(Exception):
         s.submit(400, 10, 0.005, 0.1, 1)  # larger than the whole budget
     s.close()
+
+
+def test_server_streams_snapshots_to_hdf5(tmp_path):
+    """Each request's file holds the launchers' layout; its u snapshots equal the
+    fetched ones bitwise, v_0 = v0, and elapsed_time is the request's latency."""
+    from nls import h5
+    from nls.samplers import gaussian_sum_2d
+    from nls.server import TrajectoryServer
+    if h5.available() is None:
+        pytest.skip("libhdf5 not loadable")
+    dt, L = 0.005, 10.0
+    s = TrajectoryServer(hbm_budget_bytes=64 << 20, max_active=8, L_=L, gamma=0.1,
+                         max_steps_per_group=16, h5_dir=tmp_path, run_id="srv")
+    ids = []
+    for k, (n, nt) in enumerate([(48, 80), (32, 120), (64, 60)]):
+        u0, _, _ = gaussian_sum_2d(n, L, 20 + k, 0)
+        ids.append((s.submit(n, nt, dt, 0.2, 100 + k, num_snapshots=4, u0=u0, v0=0.5 * u0,
+                             m=np.full((n, n), 1.5)), n, nt, u0))
+    s.drain(300)
+    for rid, n, nt, u0 in ids:
+        got = s.fetch(rid)
+        p = str(tmp_path / f"run_srv_{rid:04d}.h5")
+        u = h5.read(p, "u")
+        v = h5.read(p, "v")
+        assert u.shape == (4, n, n) and v.shape == (4, n, n)
+        assert np.array_equal(u, got["snapshots"])
+        assert np.array_equal(v[0], 0.5 * u0)
+        assert np.all(np.isfinite(v))
+        assert np.array_equal(h5.read(p, "initial_condition/u0"), u0)
+        assert np.array_equal(h5.read(p, "focusing/m"), np.full((n, n), 1.5))
+        assert h5.read_attr(p, "metadata", "problem_type") == "phi4"
+        assert h5.read_attr(p, "metadata", "run_index") == rid
+        assert h5.read_attr(p, "time", "nt") == nt
+        assert abs(h5.read_attr(p, "metadata", "elapsed_time") - got["latency_ms"] / 1e3) < 1e-9
+    s.close()
