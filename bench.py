@@ -184,9 +184,178 @@ def serve_run(args):
                         f"dt={args.dt}, gamma=0.1, sigma in U(0.05,0.5), 10 snapshots each"}
 
 
+def _timed_batch(b, warmup, steps):
+    from nls._lib import call
+    b.run(1)  # autotune + graph capture
+    if warmup > 1:
+        b.run(warmup - 1)
+    call("nls_dev_synchronize")
+    ms = b.run(steps)
+    b.set_profiling(True)
+    b.run(3)
+    pt = b.pass_timing()
+    b.set_profiling(False)
+    return ms, pt
+
+
+def run_nlse2d(args):
+    """Secondary workload: 2D cubic NLSE, 128 x 200^2 on [-10, 10]^2, complex64 fields,
+    Strang with Krylov exp(i dt L), m = 12, dt = 0.005, Gaussian packets from seed 1;
+    parity and mass drift of one trajectory against the complex128 oracle."""
+    from nls.batch import FusedBatch
+    from nls.samplers import nlse_packets_2d
+    ntraj, n, L, dt, m = args.ntraj or 128, args.n or 200, 10.0, 0.005, args.krylov or 12
+    one = np.ones((n, n))
+    psis = [nlse_packets_2d(n, L, 1, t)[0] for t in range(ntraj)]
+    b = FusedBatch([n] * ntraj, L, dt, equation="nlse", method="strang", krylov_dim=m,
+                   cells_per_thread=args.cpt)
+    for t, psi in enumerate(psis):
+        b.set_initial(t, one, one, psi)
+    ms, pt = _timed_batch(b, args.warmup, args.steps)
+    info = b.info()
+    info.update(b.tile())
+    cells = info["cells"]
+    value = cells * args.steps / (ms / 1e3)
+    achieved = pt["bytes_per_launch"] / (pt["avg_ms"] / 1e3) / 1e9 if pt["avg_ms"] > 0 else None
+    b.close()
+    log(f"[bench] nlse2d fused: {value:.3e} cell-steps/s ({ms / args.steps:.3f} ms/step), "
+        f"pass {pt['avg_ms'] * 1e3:.1f} us -> {achieved or 0:.0f} GB/s")
+    err = mass_fused = mass_ref = None
+    if not args.skip_parity:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        S = args.parity_steps
+        p = FusedBatch([n], L, dt, equation="nlse", method="strang", krylov_dim=m,
+                       snapshot_every=S, max_snapshots=2, cells_per_thread=8)
+        p.set_initial(0, one, one, psis[0])
+        p.run(S)
+        u, _ = p.snapshots(0)
+        ref = orc.nlse_run(n, L, dt * 2 * S, 2 * S, 2, one, one, psis[0], krylov=m, method=0,
+                           variant=0)
+        err = float(np.linalg.norm(u[1] - ref[1]) / np.linalg.norm(ref[1]))
+        m0 = np.sum(np.abs(psis[0]) ** 2)
+        mass_fused = float(abs(np.sum(np.abs(u[1]) ** 2) - m0) / m0)
+        mass_ref = float(abs(np.sum(np.abs(ref[1]) ** 2) - m0) / m0)
+        log(f"[bench] nlse2d rel-L2 after {S} steps {err:.2e}; mass drift fused {mass_fused:.2e}, "
+            f"oracle {mass_ref:.2e}")
+    cpu = None
+    if not args.skip_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        k = 10
+        t0 = time.perf_counter()
+        orc.nlse_run(n, L, dt * (k + 1), k + 1, 1, one, one, psis[0], krylov=m, method=0)
+        w = time.perf_counter() - t0
+        cpu = {"value": n * n * k / w, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+               "sample": f"one 200^2 trajectory x {k} Strang steps, complex128 oracle, {w:.2f} s"}
+    return {"metric": "NLSE-2D 200^2 cell-steps/s (fused complex64 batch)", "value": value,
+            "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": ms / args.steps, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "complex64 Krylov and state / fp64 small stage",
+            "data": "synthetic (Gaussian packets from seed 1, m = c = 1)",
+            "config": {"workload": f"NLSE-2D Strang batch, {ntraj} x {n}^2, side 20, m={m}, "
+                                   f"dt={dt}", "ntraj": ntraj, "n": n, "krylov_dim": m,
+                       "cells": cells, "cells_per_thread": info["cells_per_thread"]},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                         "traffic": None, "kernel": "k_nlse_pass<2,2,CPT>",
+                         "bytes_per_launch": pt["bytes_per_launch"],
+                         "avg_launch_ms": pt["avg_ms"]},
+            "rel_l2_vs_fp64_oracle": err, "mass_drift_fused": mass_fused,
+            "mass_drift_oracle": mass_ref, "cpu_baseline": cpu}
+
+
+def run_kge3d(args):
+    """Secondary workload: 3D Klein-Gordon, 40 x 220^3 on [-8, 8]^3, sEWI, m = 12,
+    dt = 0.004, mu2 = 1, Gaussian bubbles from seed 3: fused matrix-free passes vs the
+    CSR SpMV of the plain path, both in HBM GB/s; parity on a 48^3 trajectory."""
+    from nls.batch import FusedBatch
+    from nls.device import DeviceArray, DeviceSpMV
+    from nls.operators import build_anisotropic_laplacian_noflux_3d
+    from nls.samplers import bubbles_3d
+    ntraj, n, L, dt, m = args.ntraj or 40, args.n or 220, 8.0, 0.004, args.krylov or 12
+    one = np.ones((n, n, n))
+    zero = np.zeros((n, n, n))
+    b = FusedBatch([n] * ntraj, L, dt, method="sewi", dim=3, krylov_dim=m, mu2=1.0,
+                   cells_per_thread=args.cpt)
+    ics = [bubbles_3d(n, L, 3, t)[0] for t in range(min(ntraj, 4))]  # 4 fields, cycled
+    for t in range(ntraj):
+        b.set_initial(t, one, one, ics[t % len(ics)], zero)
+    del ics
+    ms, pt = _timed_batch(b, args.warmup, args.steps)
+    info = b.info()
+    info.update(b.tile())
+    cells = info["cells"]
+    value = cells * args.steps / (ms / 1e3)
+    achieved = pt["bytes_per_launch"] / (pt["avg_ms"] / 1e3) / 1e9 if pt["avg_ms"] > 0 else None
+    b.close()
+    log(f"[bench] kge3d fused: {value:.3e} cell-steps/s ({ms / args.steps:.1f} ms/step), "
+        f"pass {pt['avg_ms'] * 1e3:.1f} us -> {achieved or 0:.0f} GB/s")
+    # CSR SpMV of the plain drop-in on one 220^3 trajectory (FP64)
+    dx = 2 * L / (n - 1)
+    rp, ci, val = build_anisotropic_laplacian_noflux_3d(n - 2, n - 2, n - 2, dx, dx, dx, one,
+                                                        negate=True, shift=1.0)
+    A = DeviceSpMV(rp, ci, val)
+    x = DeviceArray.from_host(np.random.default_rng(0).standard_normal(n ** 3))
+    y = DeviceArray(n ** 3, np.float64)
+    import ctypes as C
+    from nls._lib import call
+    for _ in range(3):
+        A.multiply(x, y)
+    call("nls_dev_synchronize")
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        A.multiply(x, y)
+    call("nls_dev_synchronize")
+    spmv_s = (time.perf_counter() - t0) / reps
+    nnz = len(ci)
+    spmv_bytes = 12.0 * nnz + 20.0 * n ** 3
+    spmv_gbs = spmv_bytes / spmv_s / 1e9
+    del A, x, y
+    log(f"[bench] kge3d CSR SpMV (FP64, 220^3, nnz {nnz}): {spmv_s * 1e3:.2f} ms -> "
+        f"{spmv_gbs:.0f} GB/s")
+    err = None
+    if not args.skip_parity:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        np_, S = 48, args.parity_steps
+        o3 = np.ones((np_, np_, np_))
+        u0, v0 = bubbles_3d(np_, L, 3, 0)
+        p = FusedBatch([np_], L, dt, method="sewi", dim=3, krylov_dim=m, mu2=1.0,
+                       snapshot_every=S, max_snapshots=2, cells_per_thread=8)
+        p.set_initial(0, o3, o3, u0, v0)
+        p.run(S)
+        u, _ = p.snapshots(0)
+        ur, _ = orc.kge_run(np_, L, dt * 2 * S, 2 * S, 2, o3, o3, u0, v0, krylov=m, method=2,
+                            mu2=1.0, dim=3)
+        err = float(np.linalg.norm(u[1] - ur[1]) / np.linalg.norm(ur[1]))
+        log(f"[bench] kge3d rel-L2 (48^3, {S} steps) {err:.2e}")
+    return {"metric": "KGE-3D 220^3 cell-steps/s and HBM GB/s, fused vs CSR SpMV", "value": value,
+            "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": ms / args.steps, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp32 Krylov vectors / fp64 state and small stage",
+            "data": "synthetic (Gaussian bubbles from seed 3, 4 distinct fields cycled over "
+                    "the trajectories, m = c = 1)",
+            "config": {"workload": f"KGE-3D sEWI batch, {ntraj} x {n}^3, side 16, m={m}, "
+                                   f"dt={dt}, mu2=1", "ntraj": ntraj, "n": n, "krylov_dim": m,
+                       "cells": cells, "cells_per_thread": info["cells_per_thread"],
+                       "pass_kind": info["pass_kind"]},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                         "traffic": None, "kernel": "k_kge_pass<3,2,CPT> (fused 7-point pass)",
+                         "bytes_per_launch": pt["bytes_per_launch"],
+                         "avg_launch_ms": pt["avg_ms"]},
+            "csr_spmv": {"gbs": spmv_gbs, "ms": spmv_s * 1e3, "nnz": nnz,
+                         "bytes": spmv_bytes, "frac": spmv_gbs / HBM_PEAK_GBS},
+            "rel_l2_vs_fp64_oracle_48cubed": err, "cpu_baseline": None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="kge2d", choices=["kge2d", "nlse2d", "kge3d"],
+                    help="kge2d: the headline batch; nlse2d, kge3d: secondary workloads")
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--ntraj", type=int, default=100)
@@ -220,6 +389,18 @@ def main():
         sys.exit(2)
     from nls._lib import load
     load()
+    if args.workload != "kge2d":
+        if "--ntraj" not in sys.argv:
+            args.ntraj = 0
+        if "--n" not in sys.argv:
+            args.n = 0
+        if "--steps" not in sys.argv:
+            args.steps = 40 if args.workload == "nlse2d" else 5
+        if "--warmup" not in sys.argv:
+            args.warmup = 5 if args.workload == "nlse2d" else 2
+        line = run_nlse2d(args) if args.workload == "nlse2d" else run_kge3d(args)
+        print(json.dumps(line), flush=True)
+        return
     log(f"[bench] building inputs: {args.ntraj} x {args.n}^2")
     inputs = make_inputs(args.ntraj, args.n, args.L, seed=2)
     log("[bench] fused batched run")
