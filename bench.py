@@ -47,7 +47,7 @@ def fused_run(args, inputs):
     n, L = args.n, args.L
     b = FusedBatch([n] * args.ntraj, L, args.dt, krylov_dim=args.krylov, mu2=1.0,
                    use_graph=not args.no_graph, cells_per_thread=args.cpt,
-                   pass_kernel=None if args.kernel < 0 else args.kernel)
+                   pass_kernel=None if args.kernel < 0 else args.kernel, lanes=args.lanes)
     for t, (m, c, u0, v0) in enumerate(inputs):
         b.set_initial(t, m, c, u0, v0)
     b.run(1)  # autotune + graph capture
@@ -367,6 +367,8 @@ def main():
     ap.add_argument("--kernel", type=int, default=-1,
                     help="pass kernel: -1 = autotuner, 0 direct, 1 LDS chunk, 2 row march, "
                          "3 row march storing only the basis")
+    ap.add_argument("--lanes", type=int, default=0, help="concurrent parts of the batch, "
+                    "0 = autotuner")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--prof-steps", type=int, default=10)
     ap.add_argument("--plain-steps", type=int, default=40)
@@ -410,7 +412,8 @@ def main():
     args.krylov_used = info["krylov_dim"]
     achieved = pt["bytes_per_launch"] / (pt["avg_ms"] / 1e3) / 1e9 if pt["avg_ms"] > 0 else None
     log(f"[bench] fused: {value:.3e} cell-steps/s ({ms / args.steps:.3f} ms/step), "
-        f"m={info['krylov_dim']}, tile {info['cells_per_thread']} kind={info['pass_kind']}, "
+        f"m={info['krylov_dim']}, tile {info['cells_per_thread']} kind={info['pass_kind']} "
+        f"lanes={info['lanes']}, "
         f"pass avg {pt['avg_ms'] * 1e3:.1f} us -> {achieved or 0:.0f} GB/s")
     del b
     import ctypes as C
@@ -446,6 +449,12 @@ def main():
              1: f"k_kge_pass2d<2, {info['cells_per_thread']}>",
              0: f"k_kge_pass<2, 2, {info['cells_per_thread']}>"}[info["pass_kind"]]
     traffic = MEASURED_TRAFFIC.get((kname, cells))
+    # algorithmic bytes of a whole step per cell (passes + combine) for the stored-w'
+    # kernels: 24 + 40 + 36 (m - 3) + (16 + 8 (m - 2) + 8 + 4 + 8) with FP32 Krylov
+    # vectors of two sequences and FP64 state
+    step_bytes = (24 + 40 + 36 * (m - 3) + 8 * (m - 2) + 36) if info["pass_kind"] in (0, 1, 2) \
+        else (16 + 24 + 32 + 28 * (m - 4) + 8 + 8 * (m - 2) + 36)
+    step_gbs = step_bytes * cells / (ms / args.steps / 1e3) / 1e9
     line = {
         "metric": METRIC,
         "value": value,
@@ -463,7 +472,7 @@ def main():
                                f"mu2=1, dt={args.dt}, Krylov m={m}, HIP graph step capture",
                    "ntraj": args.ntraj, "n": args.n, "dt": args.dt, "krylov_dim": m,
                    "cells_per_thread": info["cells_per_thread"], "pass_kind": info["pass_kind"],
-                   "cells": cells, "parallelism": "1 GPU"},
+                   "lanes": info["lanes"], "cells": cells, "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": traffic,
@@ -475,7 +484,12 @@ def main():
                                         0: " (fused stencil+Lanczos pass)"}[info["pass_kind"]],
                      "bytes_per_launch": pt["bytes_per_launch"],
                      "avg_launch_ms": pt["avg_ms"], "launches_timed": pt["launches"],
-                     "stream_copy_gbs": stream_gbs},
+                     "measured_as": "the same kernel and tile over the whole batch on one "
+                                    "stream (the timed steps run the batch as concurrent lanes)",
+                     "stream_copy_gbs": stream_gbs,
+                     "step_bytes_per_cell": step_bytes,
+                     "step_effective_gbs": step_gbs,
+                     "step_frac": step_gbs / HBM_PEAK_GBS},
         "plain": {"value": plain_rate, "unit": "cell-steps/s",
                   "path": "FP64 CSR SpMV + reference-granularity Lanczos (CGS), one trajectory "
                           "at a time", "steps": plain_steps},

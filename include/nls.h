@@ -253,6 +253,8 @@ typedef struct {
                         fixed tile); k + 1 = use pass kernel k only (0 direct stencil,
                         1 LDS-tiled chunk, 2 row march, 3 row march storing only the
                         basis), autotuning its tile when cells_per_thread is 0 */
+  int lanes;         /* Klein-Gordon: parts of the batch stepped concurrently on their own
+                        streams, 1..4; 0 = autotune (2 with a fixed tile) */
 } nls_batch_params;
 typedef struct {
   int n;             /* full side */
@@ -283,10 +285,11 @@ NLS_API int nls_batch_info(nls_batch *b, int *krylov_dim, int64_t *cells, size_t
 NLS_API int nls_batch_set_profiling(nls_batch *b, int on);
 NLS_API int nls_batch_pass_timing(nls_batch *b, double *avg_ms, int *launches,
                                   double *bytes_per_launch);
-/* Tile in use after the first nls_batch_run: cells per thread and the pass
- * kernel (0 direct stencil, 1 LDS-tiled chunk, 2 row march with LDS window, 3 row
- * march storing only the basis vectors, w' recomputed). */
-NLS_API int nls_batch_tile(nls_batch *b, int *cells_per_thread, int *pass_kind);
+/* Tile in use after the first nls_batch_run: cells per thread, the pass kernel
+ * (0 direct stencil, 1 LDS-tiled chunk, 2 row march with LDS window, 3 row march
+ * storing only the basis vectors, w' recomputed) and the number of lanes (parts of
+ * the batch stepped concurrently on their own streams); any output may be NULL. */
+NLS_API int nls_batch_tile(nls_batch *b, int *cells_per_thread, int *pass_kind, int *lanes);
 /* Device bytes a batch with these trajectories would hold (admission control). */
 NLS_API int nls_batch_estimate_bytes(const nls_batch_params *p, int ntraj,
                                      const nls_traj_desc *trajs, size_t *bytes);

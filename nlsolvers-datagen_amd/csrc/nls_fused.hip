@@ -69,6 +69,7 @@ struct TrajScal {
 struct KgeArgs {
   const TrajDev *traj;
   const int2 *cmap;
+  const int *tlist; // small stage: block -> trajectory (nullptr: identity)
   TrajScal *scal;
   double *part; // [2 parity][total_chunks][NSEQ*PN]
   int total_chunks;
@@ -292,7 +293,8 @@ __global__ __launch_bounds__(FBS) void k_kge_pass(KgeArgs a, int j) {
 // One wavefront per (trajectory, sequence): alpha_{m-2}, b_{m-1} from the last
 // pass, T = tridiag(b, [alpha_0..alpha_{m-2}, 0], b), y = b0 Q f(L) Q^T e1.
 __global__ __launch_bounds__(kWave) void k_kge_small(KgeArgs a, int last_pass) {
-  const int t = blockIdx.x / NSEQ, s = blockIdx.x % NSEQ;
+  const int t = a.tlist ? a.tlist[blockIdx.x / NSEQ] : (int)(blockIdx.x / NSEQ);
+  const int s = blockIdx.x % NSEQ;
   const TrajDev T = a.traj[t];
   __shared__ SmallWS ws;
   __shared__ double red[NSEQ * PN];
@@ -1068,14 +1070,14 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax,
           const float vi = sv[e];
           const float Av = apply5(w, sv, e, oU, oL, oR, oD, scl, mu2);
           if (STAGE == 0) {
-            (s == 0 ? Wout0 : Wout1)[i] = Av;
+            __builtin_nontemporal_store(Av, &(s == 0 ? Wout0 : Wout1)[i]);
             acc[s * PN + 0] += (double)vi * vi;
             acc[s * PN + 1] += (double)vi * Av;
             acc[s * PN + 2] += (double)Av * Av;
           } else {
             const float wp = Av - cC[s] * (s == 0 ? sp0 : sp1)[e];
-            (s == 0 ? Vout0 : Vout1)[i] = vi;
-            (s == 0 ? Wout0 : Wout1)[i] = wp;
+            __builtin_nontemporal_store(vi, &(s == 0 ? Vout0 : Vout1)[i]);
+            __builtin_nontemporal_store(wp, &(s == 0 ? Wout0 : Wout1)[i]);
             acc[s * PN + 0] += (double)wp * vi;
             acc[s * PN + 1] += (double)wp * wp;
             acc[s * PN + 2] += (double)vi * vi;
@@ -1171,7 +1173,9 @@ __global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a) {
 #pragma unroll
     for (int kk = 0; kk < G; ++kk) {
       const int64_t i = gb + threadIdx.x + (int64_t)kk * FBS;
-      if (ok[kk]) a.un[T.off + i] = 2.0 * a.u[gs[kk]] - a.up[gs[kk]] + 2.0 * d0[kk] + t2 * d1[kk];
+      if (ok[kk])
+        __builtin_nontemporal_store(2.0 * a.u[gs[kk]] - a.up[gs[kk]] + 2.0 * d0[kk] + t2 * d1[kk],
+                                    &a.un[T.off + i]);
     }
   }
 }
@@ -1233,6 +1237,31 @@ struct Layout {
   int64_t cells = 0;
 };
 
+// XCD-aware block map of the trajectories in `subset`: workgroup b runs on XCD
+// b % 8, so all chunks of a trajectory are placed in one XCD's list (their halo
+// rows then meet in that XCD's L2); trajectories go largest-first to the
+// least-loaded XCD.
+static std::vector<int2> xcd_cmap(const std::vector<TrajDev> &traj, const std::vector<int> &subset) {
+  std::vector<int> order(subset);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return traj[x].nchunks > traj[y].nchunks; });
+  std::array<std::vector<int2>, NXCD> bins;
+  std::array<int64_t, NXCD> load{};
+  for (int t : order) {
+    int best = 0;
+    for (int x = 1; x < NXCD; ++x)
+      if (load[x] < load[best]) best = x;
+    for (int c = 0; c < traj[t].nchunks; ++c) bins[best].push_back(int2{t, c});
+    load[best] += traj[t].nchunks;
+  }
+  size_t maxlen = 0;
+  for (auto &b : bins) maxlen = std::max(maxlen, b.size());
+  std::vector<int2> cmap(maxlen * NXCD, int2{-1, -1});
+  for (int x = 0; x < NXCD; ++x)
+    for (size_t k = 0; k < bins[x].size(); ++k) cmap[k * NXCD + x] = bins[x][k];
+  return cmap;
+}
+
 // Chunking: fixed ranges of 256 * cpt cells (the tiled pass kernels need exactly
 // that), or, for the row-march kernels (balanced), the same number of chunks with
 // the trajectory split into equal ranges (multiples of 64 cells), so the last
@@ -1243,7 +1272,6 @@ static Layout make_layout(const nls_traj_desc *td, int ntraj, int dim, int cpt, 
   Layout L;
   const int CH = FBS * cpt;
   L.traj.resize(ntraj);
-  std::vector<int> nch(ntraj);
   for (int t = 0; t < ntraj; ++t) {
     TrajDev &T = L.traj[t];
     T.n = td[t].n;
@@ -1264,28 +1292,10 @@ static Layout make_layout(const nls_traj_desc *td, int ntraj, int dim, int cpt, 
     T.s = (float)(1.0 / (dx * dx));
     T.mu2 = (float)mu2;
     T.tau = dt;
-    nch[t] = T.nchunks;
   }
-  // XCD-aware block map: workgroup b runs on XCD b % 8, so all chunks of a
-  // trajectory are placed in one XCD's list (their halo rows then meet in that
-  // XCD's L2); trajectories go largest-first to the least-loaded XCD.
-  std::vector<int> order(ntraj);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return nch[x] > nch[y]; });
-  std::array<std::vector<int2>, NXCD> bins;
-  std::array<int64_t, NXCD> load{};
-  for (int t : order) {
-    int best = 0;
-    for (int x = 1; x < NXCD; ++x)
-      if (load[x] < load[best]) best = x;
-    for (int c = 0; c < nch[t]; ++c) bins[best].push_back(int2{t, c});
-    load[best] += nch[t];
-  }
-  size_t maxlen = 0;
-  for (auto &b : bins) maxlen = std::max(maxlen, b.size());
-  L.cmap.assign(maxlen * NXCD, int2{-1, -1});
-  for (int x = 0; x < NXCD; ++x)
-    for (size_t k = 0; k < bins[x].size(); ++k) L.cmap[k * NXCD + x] = bins[x][k];
+  std::vector<int> all(ntraj);
+  std::iota(all.begin(), all.end(), 0);
+  L.cmap = xcd_cmap(L.traj, all);
   return L;
 }
 
@@ -1593,6 +1603,17 @@ struct nls_batch {
   bool profiling = false;
   int pass_kind = 0; // 0 direct, 1 LDS-tiled chunk, 2 row march, 3 row march with recomputed w'
   int nmax = 0;
+  // Lanes: the batch split into up to MAXL parts (balanced by cells) whose steps run on
+  // their own streams, so one part's small stage and kernel tails overlap the other
+  // parts' passes.  Per-trajectory results do not depend on the lane split.
+  static constexpr int MAXL = 4;
+  int lanes = 1, max_lanes = 1;
+  hipStream_t sl[MAXL] = {}; // sl[0] is s
+  hipEvent_t ev_fork = nullptr, ev_join[MAXL] = {};
+  int2 *d_cmapL[MAXL] = {};
+  int *d_tlistL[MAXL] = {};
+  size_t cmapL_size[MAXL] = {};
+  int ntrajL[MAXL] = {};
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   double pass_ms = 0.0;
@@ -1610,10 +1631,19 @@ struct nls_batch {
     if (s) (void)hipStreamSynchronize(s);
     drop_graphs();
     for (auto e : ev_pool) (void)hipEventDestroy(e);
+    for (int k = 1; k < MAXL; ++k)
+      if (sl[k]) (void)hipStreamSynchronize(sl[k]);
     void *ptrs[] = {d_traj, d_cmap, d_scal, d_part, d_mf, d_cf, d_V, d_W, d_X,
                     d_state[0], d_state[1], d_state[2], d_snapU, d_snapV};
     for (void *q : ptrs)
       if (q) (void)hipFree(q);
+    for (int k = 0; k < MAXL; ++k) {
+      if (d_cmapL[k]) (void)hipFree(d_cmapL[k]);
+      if (d_tlistL[k]) (void)hipFree(d_tlistL[k]);
+      if (ev_join[k]) (void)hipEventDestroy(ev_join[k]);
+      if (k > 0 && sl[k]) (void)hipStreamDestroy(sl[k]);
+    }
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (s) (void)hipStreamDestroy(s);
   }
 
@@ -1695,8 +1725,44 @@ struct nls_batch {
                            hipMemcpyHostToDevice, s));
     NLS_HIP(hipMemcpyAsync(d_cmap, lay.cmap.data(), sizeof(int2) * lay.cmap.size(),
                            hipMemcpyHostToDevice, s));
+    set_lanes(lanes);
+  }
+
+  // Splits the trajectories into L lanes (largest first onto the lane with fewest
+  // cells) and uploads each lane's block map and small-stage trajectory list.
+  void set_lanes(int L) {
+    lanes = std::max(1, std::min(L, max_lanes));
+    if (lanes == 1) {
+      NLS_HIP(hipStreamSynchronize(s)); // host layout copies done
+      return;
+    }
+    std::vector<int> order(ntraj);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return lay.traj[x].N > lay.traj[y].N; });
+    std::vector<std::vector<int>> sub(lanes);
+    std::vector<int64_t> load(lanes, 0);
+    for (int t : order) {
+      const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      sub[k].push_back(t);
+      load[k] += lay.traj[t].N;
+    }
+    for (int k = 0; k < lanes; ++k) {
+      std::sort(sub[k].begin(), sub[k].end());
+      const std::vector<int2> cm = xcd_cmap(lay.traj, sub[k]);
+      NLS_REQUIRE(cm.size() <= cmap_cap, "internal: lane chunk map larger than allocated");
+      cmapL_size[k] = cm.size();
+      ntrajL[k] = (int)sub[k].size();
+      if (!cm.empty())
+        NLS_HIP(hipMemcpyAsync(d_cmapL[k], cm.data(), sizeof(int2) * cm.size(),
+                               hipMemcpyHostToDevice, s));
+      if (!sub[k].empty())
+        NLS_HIP(hipMemcpyAsync(d_tlistL[k], sub[k].data(), sizeof(int) * sub[k].size(),
+                               hipMemcpyHostToDevice, s));
+    }
     NLS_HIP(hipStreamSynchronize(s));
   }
+
 
   int nmax_of() const {
     int x = 0;
@@ -1716,8 +1782,9 @@ struct nls_batch {
   // Pass kernels: kind 0 = direct global-memory stencil (any dim), 1 = LDS-tiled
   // chunk + halo rows (2D), 2 = row march with an LDS window (2D), 3 = row march
   // that stores only the basis and recomputes w' (2D, launch_rpass).
-  template <int DIM, int STAGE> void launch_pass(const KgeArgs &a, int j) {
-    const dim3 grid((unsigned)lay.cmap.size()), blk(FBS);
+  template <int DIM, int STAGE> void launch_pass(const KgeArgs &a, int j, hipStream_t s,
+                                                 unsigned ngrid) {
+    const dim3 grid(ngrid), blk(FBS);
     if (pass_kind == 2) {
       const size_t lds = march2d_lds_bytes(nmax);
       const int emax = MSUB + 2 * nmax;
@@ -1734,43 +1801,67 @@ struct nls_batch {
     }
   }
 
-  template <int STAGE> void launch_rpass(const KgeArgs &a, int j) {
-    const dim3 grid((unsigned)lay.cmap.size()), blk(FBS);
+  template <int STAGE> void launch_rpass(const KgeArgs &a, int j, hipStream_t s, unsigned ngrid) {
+    const dim3 grid(ngrid), blk(FBS);
     const size_t lds = rmarch2d_lds_bytes(nmax);
     const int eP = MSUB + 4 * nmax, eV = MSUB + 2 * nmax, alt = 1;
     hipLaunchKernelGGL((k_kge_rmarch2d<STAGE>), grid, blk, lds, s, a, j, eP, eV, alt);
   }
 
-  template <int DIM> void launch_step_t(int cur, bool snapshot, int slot, bool prof) {
-    const KgeArgs a = args(cur);
-    const dim3 grid((unsigned)lay.cmap.size()), blk(FBS);
+  // One step of the trajectories in a block map (cmap, ngrid workgroups; tlist and
+  // nt for the small stage) on stream st.
+  template <int DIM>
+  void launch_lane(int cur, bool snapshot, int slot, bool prof, hipStream_t st, const int2 *cmap,
+                   unsigned ngrid, const int *tlist, int nt) {
+    KgeArgs a = args(cur);
+    a.cmap = cmap;
+    a.tlist = tlist;
+    const dim3 grid(ngrid), blk(FBS);
     if (pass_kind == 3) {
       // stored-basis-only passes; events around the general (j >= 3) passes
-      launch_rpass<0>(a, 0);
-      launch_rpass<1>(a, 1);
-      if (m - 2 >= 2) launch_rpass<2>(a, 2);
+      launch_rpass<0>(a, 0, st, ngrid);
+      launch_rpass<1>(a, 1, st, ngrid);
+      if (m - 2 >= 2) launch_rpass<2>(a, 2, st, ngrid);
       for (int j = 3; j <= m - 2; ++j) {
-        if (prof) NLS_HIP(hipEventRecord(next_event(), s));
-        launch_rpass<3>(a, j);
-        if (prof) NLS_HIP(hipEventRecord(next_event(), s));
+        if (prof) NLS_HIP(hipEventRecord(next_event(), st));
+        launch_rpass<3>(a, j, st, ngrid);
+        if (prof) NLS_HIP(hipEventRecord(next_event(), st));
       }
     } else {
-      launch_pass<DIM, 0>(a, 0);
-      if (m - 2 >= 1) launch_pass<DIM, 1>(a, 1);
+      launch_pass<DIM, 0>(a, 0, st, ngrid);
+      if (m - 2 >= 1) launch_pass<DIM, 1>(a, 1, st, ngrid);
       for (int j = 2; j <= m - 2; ++j) {
-        if (prof) NLS_HIP(hipEventRecord(next_event(), s));
-        launch_pass<DIM, 2>(a, j);
-        if (prof) NLS_HIP(hipEventRecord(next_event(), s));
+        if (prof) NLS_HIP(hipEventRecord(next_event(), st));
+        launch_pass<DIM, 2>(a, j, st, ngrid);
+        if (prof) NLS_HIP(hipEventRecord(next_event(), st));
       }
     }
-    hipLaunchKernelGGL(k_kge_small, dim3(ntraj * NSEQ), dim3(kWave), 0, s, a, m - 2);
+    hipLaunchKernelGGL(k_kge_small, dim3(nt * NSEQ), dim3(kWave), 0, st, a, m - 2);
     if (snapshot) {
       SnapArgs sn{d_snapU + (size_t)slot * lay.cells, d_snapV + (size_t)slot * lay.cells};
-      hipLaunchKernelGGL((k_kge_combine_snap<DIM>), grid, blk, 0, s, a, sn);
+      hipLaunchKernelGGL((k_kge_combine_snap<DIM>), grid, blk, 0, st, a, sn);
     } else {
-      hipLaunchKernelGGL((k_kge_combine_rt<DIM>), grid, blk, 0, s, a);
+      hipLaunchKernelGGL((k_kge_combine_rt<DIM>), grid, blk, 0, st, a);
     }
     check_launch("fused kge step");
+  }
+
+  template <int DIM> void launch_step_t(int cur, bool snapshot, int slot, bool prof) {
+    if (lanes > 1 && !prof) {
+      // fork s -> lane streams, one part of the batch on each, join back into s
+      NLS_HIP(hipEventRecord(ev_fork, s));
+      for (int k = 1; k < lanes; ++k) NLS_HIP(hipStreamWaitEvent(sl[k], ev_fork, 0));
+      for (int k = 0; k < lanes; ++k)
+        launch_lane<DIM>(cur, snapshot, slot, false, k == 0 ? s : sl[k], d_cmapL[k],
+                         (unsigned)cmapL_size[k], d_tlistL[k], ntrajL[k]);
+      for (int k = 1; k < lanes; ++k) {
+        NLS_HIP(hipEventRecord(ev_join[k], sl[k]));
+        NLS_HIP(hipStreamWaitEvent(s, ev_join[k], 0));
+      }
+      return;
+    }
+    launch_lane<DIM>(cur, snapshot, slot, prof, s, d_cmap, (unsigned)lay.cmap.size(), nullptr,
+                     ntraj);
   }
 
   void launch_step(int cur, bool snapshot, int slot, bool prof) {
@@ -1813,7 +1904,7 @@ struct nls_batch {
                             {2, 12}, {2, 16}, {2, 24}, {2, 32}, {2, 36}, {2, 48}, {2, 64},
                             {1, 4},  {1, 8},  {0, 4},  {0, 8},  {0, 16}};
     float best = 1e30f;
-    int best_cpt = 4, best_kind = 0;
+    int best_cpt = 4, best_kind = 0, best_lanes = 1;
     hipEvent_t e0, e1;
     NLS_HIP(hipEventCreate(&e0));
     NLS_HIP(hipEventCreate(&e1));
@@ -1822,18 +1913,23 @@ struct nls_batch {
       if (p.pass_kernel > 0 && cd[0] != p.pass_kernel - 1) continue;
       pass_kind = cd[0];
       set_layout(cd[1]);
-      launch_step((int)(step % nstates()), false, 0, false); // warm-up
-      NLS_HIP(hipEventRecord(e0, s));
-      for (int r = 0; r < 3; ++r) launch_step((int)(step % nstates()), false, 0, false);
-      NLS_HIP(hipEventRecord(e1, s));
-      NLS_HIP(hipEventSynchronize(e1));
-      float ms = 0;
-      NLS_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best) { best = ms; best_cpt = cd[1]; best_kind = cd[0]; }
+      for (int ln = 1; ln <= max_lanes; ++ln) {
+        if (p.lanes > 0 && ln != std::min(p.lanes, max_lanes)) continue;
+        set_lanes(ln);
+        launch_step((int)(step % nstates()), false, 0, false); // warm-up
+        NLS_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < 3; ++r) launch_step((int)(step % nstates()), false, 0, false);
+        NLS_HIP(hipEventRecord(e1, s));
+        NLS_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        NLS_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) { best = ms; best_cpt = cd[1]; best_kind = cd[0]; best_lanes = ln; }
+      }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     pass_kind = best_kind;
+    lanes = best_lanes;
     set_layout(best_cpt);
     tuned = true;
   }
@@ -1866,6 +1962,7 @@ struct nls_batch {
         if (p.pass_kernel > 0) pass_kind = p.pass_kernel - 1;
         else pass_kind = kind_ok(3, c) ? 3 : kind_ok(2, c) ? 2 : kind_ok(1, c) ? 1 : 0;
         NLS_REQUIRE(kind_ok(pass_kind, c), "cells_per_thread / pass_kernel not supported for this batch");
+        lanes = std::min(p.lanes > 0 ? p.lanes : 2, max_lanes);
         set_layout(c);
         tuned = true;
       } else {
@@ -1979,6 +2076,7 @@ int nls_batch_create(const nls_batch_params *p, int ntraj, const nls_traj_desc *
     NLS_REQUIRE(p->dim == 2 || p->dim == 3, "dim must be 2 or 3");
     NLS_REQUIRE(p->dt > 0, "dt must be positive");
     NLS_REQUIRE(p->pass_kernel >= 0 && p->pass_kernel <= 4, "pass_kernel must be in [0, 4]");
+    NLS_REQUIRE(p->lanes >= 0 && p->lanes <= nls_batch::MAXL, "lanes must be in [0, 4]");
     NLS_REQUIRE(p->cells_per_thread == 0 ||
                     (p->cells_per_thread >= 4 && p->cells_per_thread <= 64),
                 "cells_per_thread must be 0 (autotune) or in [4, 64]");
@@ -1990,12 +2088,26 @@ int nls_batch_create(const nls_batch_params *p, int ntraj, const nls_traj_desc *
     b->m = pick_m(*p, ntraj, trajs);
     NLS_REQUIRE(b->m >= 3 && b->m <= kMaxKrylov, "fused path needs a Krylov dimension in [3, 32]");
     NLS_HIP(hipStreamCreate(&b->s));
+    b->sl[0] = b->s;
+    if (p->equation == NLS_EQ_KGE && ntraj >= 2) {
+      b->max_lanes = std::min(nls_batch::MAXL, ntraj);
+      NLS_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+      for (int k = 1; k < b->max_lanes; ++k) {
+        NLS_HIP(hipStreamCreate(&b->sl[k])); // default priority, as the main stream
+        NLS_HIP(hipEventCreateWithFlags(&b->ev_join[k], hipEventDisableTiming));
+      }
+    }
     // layout with the smallest tile (most chunks) sizes the chunk map and partials
     Layout l4 = make_layout(trajs, ntraj, p->dim, 4, p->dt, p->mu2);
     // any tile has at most the chunks of the smallest one; a bin holds at most all of them
     b->cmap_cap = (size_t)NXCD * (l4.total_chunks + 1);
     b->d_traj = b->alloc<TrajDev>(ntraj);
     b->d_cmap = b->alloc<int2>(b->cmap_cap);
+    if (b->max_lanes > 1)
+      for (int k = 0; k < b->max_lanes; ++k) {
+        b->d_cmapL[k] = b->alloc<int2>(b->cmap_cap);
+        b->d_tlistL[k] = b->alloc<int>(ntraj);
+      }
     b->d_scal = b->alloc<TrajScal>(ntraj);
     b->d_part = b->alloc<double>((size_t)2 * l4.total_chunks * NSEQ * PN);
     const int64_t cells = l4.cells;
@@ -2134,11 +2246,12 @@ int nls_batch_info(nls_batch *b, int *krylov_dim, int64_t *cells, size_t *device
   });
 }
 
-int nls_batch_tile(nls_batch *b, int *cells_per_thread, int *lds_tiled) {
+int nls_batch_tile(nls_batch *b, int *cells_per_thread, int *pass_kind, int *lanes) {
   return guard([&] {
-    NLS_REQUIRE(b, "null batch");
+    NLS_REQUIRE(b, "nls_batch_tile: null batch");
     if (cells_per_thread) *cells_per_thread = b->cpt;
-    if (lds_tiled) *lds_tiled = b->pass_kind;
+    if (pass_kind) *pass_kind = b->pass_kind;
+    if (lanes) *lanes = b->lanes;
   });
 }
 
