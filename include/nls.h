@@ -253,8 +253,8 @@ typedef struct {
                         fixed tile); k + 1 = use pass kernel k only (0 direct stencil,
                         1 LDS-tiled chunk, 2 row march, 3 row march storing only the
                         basis), autotuning its tile when cells_per_thread is 0 */
-  int lanes;         /* Klein-Gordon: parts of the batch stepped concurrently on their own
-                        streams, 1..4; 0 = autotune (2 with a fixed tile) */
+  int lanes;         /* parts of the batch stepped concurrently on their own streams,
+                        1..4; 0 = autotune (2 with a fixed tile) */
 } nls_batch_params;
 typedef struct {
   int n;             /* full side */

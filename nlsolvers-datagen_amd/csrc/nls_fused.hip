@@ -1332,6 +1332,7 @@ __device__ inline c32 operator*(float s, c32 a) { return {s * a.re, s * a.im}; }
 struct NlseArgs {
   const TrajDev *traj;
   const int2 *cmap;
+  const int *tlist; // small stage: block -> trajectory (nullptr: identity)
   TrajScal *scal; // sequence 0 arrays; y[1] holds Im(y)
   double *part;   // [2 parity][total_chunks][PN]
   int total_chunks;
@@ -1465,7 +1466,7 @@ __global__ __launch_bounds__(FBS) void k_nlse_pass(NlseArgs a, int j) {
 }
 
 __global__ __launch_bounds__(kWave) void k_nlse_small(NlseArgs a, int last_pass) {
-  const int t = blockIdx.x;
+  const int t = a.tlist ? a.tlist[blockIdx.x] : (int)blockIdx.x;
   const TrajDev T = a.traj[t];
   __shared__ SmallWS ws;
   __shared__ double red[PN];
@@ -1560,6 +1561,222 @@ __global__ __launch_bounds__(FBS) void k_nlse_combine(NlseArgs a, SnapC snap) {
   for (int kk = 0; kk < CPT; ++kk) {
     const int64_t i = base + threadIdx.x + (int64_t)kk * FBS;
     if (i >= T.N) break;
+    const int64_t src = clamp_src<DIM>(i, n);
+    const int64_t gi = T.off + i, gs = T.off + src;
+    double sr, si;
+    nlse_psi3(a, S, ys[0], ys[1], gs, sr, si);
+    a.psi_next[gi] = c32{(float)sr, (float)si};
+    if (snap.psi) { // snapshot before the ghost copy (nlse_dev.hpp:195-197)
+      double ir = sr, ii = si;
+      if (src != i) nlse_psi3(a, S, ys[0], ys[1], gi, ir, ii);
+      snap.psi[2 * gi] = ir;
+      snap.psi[2 * gi + 1] = ii;
+    }
+  }
+}
+
+
+// ---------------------------------------------------------------- NLSE 2D row march
+// The stored-w' row march of k_kge_march2d for the complex64 NLSE sequence: the real
+// and imaginary parts of v_j are two real sequences sharing the (real) Lanczos
+// scalars, L v = s (sum_nb w v_nb - wsum v) (not negated, as in k_nlse_pass), and the
+// partials add both parts.  STAGE 0 forms the seed psi1 = psi exp(i dt/2 m|psi|^2) when
+// a cell is deposited and stores it (X) with W[0] = L psi1; STAGE 1 forms v_1 from W[0]
+// and X; STAGE 2 is the general pass.
+typedef float f2v __attribute__((ext_vector_type(2)));
+
+template <int STAGE>
+__global__ __launch_bounds__(FBS) void k_nlse_march2d(NlseArgs a, int j, int emax, int alt) {
+  extern __shared__ float lds[];
+  const int2 cm = a.cmap[blockIdx.x];
+  if (cm.x < 0) return;
+  const TrajDev T = a.traj[cm.x];
+  __shared__ double red[PN];
+  __shared__ double shn[PN * (FBS / kWave)];
+  float cA = 0.f, cB = 0.f, cC = 0.f;
+  if (STAGE >= 1) {
+    reduce_partials3(a, T, (j - 1) & 1, red);
+    const double q0 = red[0], q1 = red[1], q2 = red[2];
+    if (STAGE == 1) {
+      const double b0 = sqrt(q0);
+      const double al = q0 > 0.0 ? q1 / q0 : 0.0;
+      const double b1sq = q0 > 0.0 ? (q2 - q1 * q1 / q0) / q0 : 0.0;
+      const double b1 = (b1sq > 1e-14 * (q0 > 0.0 ? q2 / q0 : 0.0) && b1sq > 0.0) ? sqrt(b1sq) : 0.0;
+      cA = (float)al;
+      cB = (b0 > 0.0 && b1 > 0.0) ? (float)(1.0 / (b0 * b1)) : 0.0f;
+      cC = b0 > 0.0 ? (float)(b1 / b0) : 0.0f;
+      if (cm.y == 0 && threadIdx.x == 0) {
+        a.scal[cm.x].b0[0] = b0;
+        a.scal[cm.x].alpha[0][0] = al;
+        a.scal[cm.x].beta[0][1] = b1;
+      }
+    } else {
+      double al, b;
+      lanczos_scalars(q0, q1, q2, al, b);
+      cA = (float)al;
+      cB = b > 0.0 ? (float)(1.0 / b) : 0.0f;
+      cC = (float)b;
+      if (cm.y == 0 && threadIdx.x == 0) {
+        a.scal[cm.x].alpha[0][j - 1] = al;
+        a.scal[cm.x].beta[0][j] = b;
+      }
+    }
+  }
+  const int n = T.n;
+  const int N = (int)T.N;
+  const int64_t cells = a.cells;
+  float *sc = lds, *sv0 = lds + emax, *sv1 = lds + 2 * emax, *sp0 = lds + 3 * emax,
+        *sp1 = lds + 4 * emax;
+  const c32 *psiT = a.psi + T.off;
+  const float *cT = a.cf + T.off, *mT = a.mf + T.off;
+  const c32 *Xin = a.W + (STAGE == 1 ? 0 : (size_t)((j - 1) & 1) * cells) + T.off; // z or w'
+  const c32 *Pin = (STAGE == 1 ? a.X : a.V + (STAGE >= 2 ? (size_t)(j - 2) * cells : 0)) + T.off;
+  c32 *Vout = a.V + (STAGE >= 1 ? (size_t)(j - 1) * cells : 0) + T.off;
+  c32 *Wout = a.W + (size_t)(j & 1) * cells + T.off;
+  c32 *Xout = a.X + T.off;
+  const float scl = T.s;
+  double acc[PN] = {0.0, 0.0, 0.0};
+  const int rbeg = cm.y * T.range;
+  const int rend = rbeg + T.range < N ? rbeg + T.range : N;
+  const int len = rend - rbeg;
+  const bool down = alt != 0 && (cm.y & 1);
+  const int org = down ? rend - 1 : rbeg;
+  const int dir = down ? -1 : 1;
+  const int oU = -dir * n, oL = -dir, oR = dir, oD = dir * n;
+  const float rn = 1.0f / (float)n;
+  auto cell = [&](int l) { return org + dir * l; };
+  auto store2 = [](c32 *p, float re, float im) {
+    __builtin_nontemporal_store(f2v{re, im}, reinterpret_cast<f2v *>(p));
+  };
+  // raw inputs of cell i (no arithmetic here, so the loads stay in flight)
+  auto fetch = [&](int i, float &c, c32 &x, c32 &pv) {
+    c = cT[i];
+    if (STAGE == 0) {
+      x = psiT[i];
+      pv.re = mT[i];
+    } else {
+      x = Xin[i];
+      pv = Pin[i];
+    }
+  };
+  auto deposit = [&](int e, float c, c32 x, c32 pv) {
+    sc[e] = c;
+    if (STAGE == 0) {
+      const c32 v = nl_half_phase(x, pv.re, a.dt);
+      sv0[e] = v.re;
+      sv1[e] = v.im;
+    } else {
+      sv0[e] = cB * (x.re - cA * pv.re);
+      sv1[e] = cB * (x.im - cA * pv.im);
+      sp0[e] = pv.re;
+      sp1[e] = pv.im;
+    }
+  };
+  // L v at LDS index e: s (sum_nb w v_nb - wsum v)
+  auto lap = [&](const Sten5 &w, const float *v, int e) {
+    const float vi = v[e];
+    float nb = 0.0f;
+    if (w.hu) nb += w.wu * v[e + oU];
+    if (w.hl) nb += w.wl * v[e + oL];
+    if (w.hr) nb += w.wr * v[e + oR];
+    if (w.hd) nb += w.wd * v[e + oD];
+    return scl * (nb - w.wsum * vi);
+  };
+  for (int e0 = 0; e0 < MSUB + 2 * n; e0 += MSPT * FBS) {
+    float rc[MSPT];
+    c32 rx[MSPT], rp[MSPT];
+#pragma unroll
+    for (int q = 0; q < MSPT; ++q) {
+      const int e = e0 + threadIdx.x + q * FBS, i = cell(e - n);
+      if (e < MSUB + 2 * n && i >= 0 && i < N) fetch(i, rc[q], rx[q], rp[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < MSPT; ++q) {
+      const int e = e0 + threadIdx.x + q * FBS, i = cell(e - n);
+      if (e < MSUB + 2 * n && i >= 0 && i < N) deposit(e, rc[q], rx[q], rp[q]);
+    }
+  }
+  float nc[MSPT];
+  c32 nx[MSPT], np[MSPT];
+  for (int cb = 0; cb < len; cb += MSUB) {
+    const int ce = cb + MSUB < len ? cb + MSUB : len;
+    const bool more = cb + MSUB < len;
+    if (more) {
+#pragma unroll
+      for (int q = 0; q < MSPT; ++q) {
+        const int i = cell(cb + MSUB + n + threadIdx.x + q * FBS);
+        if (i >= 0 && i < N) fetch(i, nc[q], nx[q], np[q]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < MSPT; ++kk) {
+      const int l = cb + threadIdx.x + kk * FBS;
+      if (l < ce) {
+        const int i = cell(l), e = l - cb + n;
+        const Sten5 w = sten5(sc, e, oU, oL, oR, oD, i, n, rn);
+        const float vr = sv0[e], vi = sv1[e];
+        const float lr = lap(w, sv0, e), li = lap(w, sv1, e);
+        if (STAGE == 0) {
+          store2(Xout + i, vr, vi);
+          store2(Wout + i, lr, li);
+          acc[0] += (double)vr * vr + (double)vi * vi;
+          acc[1] += (double)vr * lr + (double)vi * li;
+          acc[2] += (double)lr * lr + (double)li * li;
+        } else {
+          const float wr = lr - cC * sp0[e], wi = li - cC * sp1[e];
+          store2(Vout + i, vr, vi);
+          store2(Wout + i, wr, wi);
+          acc[0] += (double)wr * vr + (double)wi * vi;
+          acc[1] += (double)wr * wr + (double)wi * wi;
+          acc[2] += (double)vr * vr + (double)vi * vi;
+        }
+      }
+    }
+    if (!more) break;
+    __syncthreads(); // every read of this window is done
+    for (int e = threadIdx.x; e < 2 * n; e += FBS) { // keep [cb' - n, cb' + n); 2n <= SUB
+      sc[e] = sc[MSUB + e];
+      sv0[e] = sv0[MSUB + e];
+      sv1[e] = sv1[MSUB + e];
+      if (STAGE >= 1) {
+        sp0[e] = sp0[MSUB + e];
+        sp1[e] = sp1[MSUB + e];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MSPT; ++q) {
+      const int i = cell(cb + MSUB + n + threadIdx.x + q * FBS);
+      if (i >= 0 && i < N) deposit(2 * n + threadIdx.x + q * FBS, nc[q], nx[q], np[q]);
+    }
+  }
+  block_sum_n<PN>(acc, shn);
+  if (threadIdx.x == 0) {
+    double *dst = a.part + ((size_t)(j & 1) * a.total_chunks + T.chunk0 + cm.y) * PN;
+#pragma unroll
+    for (int k = 0; k < PN; ++k) dst[k] = acc[k];
+  }
+}
+
+// Combine over the workgroup's range (any layout): psi2 = sum_k y_k v_k, the second half
+// phase, the pre-copy snapshot and the ghost copy.
+template <int DIM>
+__global__ __launch_bounds__(FBS) void k_nlse_combine_rt(NlseArgs a, SnapC snap) {
+  const int2 cm = a.cmap[blockIdx.x];
+  if (cm.x < 0) return;
+  const TrajDev T = a.traj[cm.x];
+  const TrajScal &S = a.scal[cm.x];
+  __shared__ double ys[2][kMaxKrylov];
+  if (threadIdx.x < a.m) {
+    ys[0][threadIdx.x] = S.y[0][threadIdx.x];
+    ys[1][threadIdx.x] = S.y[1][threadIdx.x];
+  }
+  __syncthreads();
+  const int n = T.n;
+  const int64_t cbeg = (int64_t)cm.y * T.range;
+  const int64_t cend = cbeg + T.range < T.N ? cbeg + T.range : T.N;
+  for (int64_t i = cbeg + threadIdx.x; i < cend; i += FBS) {
     const int64_t src = clamp_src<DIM>(i, n);
     const int64_t gi = T.off + i, gs = T.off + src;
     double sr, si;
@@ -1697,21 +1914,52 @@ struct nls_batch {
     return a;
   }
 
-  template <int DIM, int CPT> void launch_nlse_t(int cur, bool snapshot, int slot, bool prof) {
-    const NlseArgs a = nargs(cur);
-    const dim3 grid((unsigned)lay.cmap.size()), blk(FBS);
-    hipLaunchKernelGGL((k_nlse_pass<DIM, 0, CPT>), grid, blk, 0, s, a, 0);
-    hipLaunchKernelGGL((k_nlse_pass<DIM, 1, CPT>), grid, blk, 0, s, a, 1);
+  template <int DIM, int CPT>
+  void launch_nlse_lane(int cur, bool snapshot, int slot, bool prof, hipStream_t s,
+                        const int2 *cmap, unsigned ngrid, const int *tlist, int nt) {
+    NlseArgs a = nargs(cur);
+    a.cmap = cmap;
+    a.tlist = tlist;
+    const dim3 grid(ngrid), blk(FBS);
+    const bool march = pass_kind == 2 && DIM == 2;
+    const size_t lds = march2d_lds_bytes(nmax);
+    const int emax = MSUB + 2 * nmax;
+    if (march) {
+      hipLaunchKernelGGL(k_nlse_march2d<0>, grid, blk, lds, s, a, 0, emax, 1);
+      hipLaunchKernelGGL(k_nlse_march2d<1>, grid, blk, lds, s, a, 1, emax, 1);
+    } else {
+      hipLaunchKernelGGL((k_nlse_pass<DIM, 0, CPT>), grid, blk, 0, s, a, 0);
+      hipLaunchKernelGGL((k_nlse_pass<DIM, 1, CPT>), grid, blk, 0, s, a, 1);
+    }
     for (int j = 2; j <= m - 2; ++j) {
       if (prof) NLS_HIP(hipEventRecord(next_event(), s));
-      hipLaunchKernelGGL((k_nlse_pass<DIM, 2, CPT>), grid, blk, 0, s, a, j);
+      if (march) hipLaunchKernelGGL(k_nlse_march2d<2>, grid, blk, lds, s, a, j, emax, 1);
+      else hipLaunchKernelGGL((k_nlse_pass<DIM, 2, CPT>), grid, blk, 0, s, a, j);
       if (prof) NLS_HIP(hipEventRecord(next_event(), s));
     }
-    hipLaunchKernelGGL(k_nlse_small, dim3(ntraj), dim3(kWave), 0, s, a, m - 2);
+    hipLaunchKernelGGL(k_nlse_small, dim3(nt), dim3(kWave), 0, s, a, m - 2);
     SnapC sn{nullptr};
     if (snapshot) sn.psi = d_snapU + (size_t)slot * lay.cells * 2;
-    hipLaunchKernelGGL((k_nlse_combine<DIM, CPT>), grid, blk, 0, s, a, sn);
+    if (march) hipLaunchKernelGGL((k_nlse_combine_rt<DIM>), grid, blk, 0, s, a, sn);
+    else hipLaunchKernelGGL((k_nlse_combine<DIM, CPT>), grid, blk, 0, s, a, sn);
     check_launch("fused nlse step");
+  }
+
+  template <int DIM, int CPT> void launch_nlse_t(int cur, bool snapshot, int slot, bool prof) {
+    if (lanes > 1 && !prof) {
+      NLS_HIP(hipEventRecord(ev_fork, s));
+      for (int k = 1; k < lanes; ++k) NLS_HIP(hipStreamWaitEvent(sl[k], ev_fork, 0));
+      for (int k = 0; k < lanes; ++k)
+        launch_nlse_lane<DIM, CPT>(cur, snapshot, slot, false, k == 0 ? s : sl[k], d_cmapL[k],
+                                   (unsigned)cmapL_size[k], d_tlistL[k], ntrajL[k]);
+      for (int k = 1; k < lanes; ++k) {
+        NLS_HIP(hipEventRecord(ev_join[k], sl[k]));
+        NLS_HIP(hipStreamWaitEvent(s, ev_join[k], 0));
+      }
+      return;
+    }
+    launch_nlse_lane<DIM, CPT>(cur, snapshot, slot, prof, s, d_cmap, (unsigned)lay.cmap.size(),
+                               nullptr, ntraj);
   }
 
   void set_layout(int cpt_) {
@@ -1886,7 +2134,10 @@ struct nls_batch {
   // timing runs leave the trajectory untouched) and keeps the fastest.
   bool kind_ok(int kind, int c) const {
     const int nm = nmax_of();
-    if (nlse()) return kind == 0 && (c == 4 || c == 8 || c == 16); // k_nlse_pass tiles
+    if (nlse()) // k_nlse_pass tiles, or the NLSE row march (2D)
+      return (kind == 0 && (c == 4 || c == 8 || c == 16)) ||
+             (kind == 2 && p.dim == 2 && 2 * nm <= MSUB && march2d_lds_bytes(nm) <= 65536 &&
+              c >= 4 && c <= 64);
     // row-march kernels take any range length (tile = target cells per workgroup / 256)
     if (kind == 3) return p.dim == 2 && 3 * nm <= RKS * FBS && 2 * nm <= MSUB &&
                           rmarch2d_lds_bytes(nm) <= 65536 && c >= 4 && c <= 64;
@@ -2089,7 +2340,7 @@ int nls_batch_create(const nls_batch_params *p, int ntraj, const nls_traj_desc *
     NLS_REQUIRE(b->m >= 3 && b->m <= kMaxKrylov, "fused path needs a Krylov dimension in [3, 32]");
     NLS_HIP(hipStreamCreate(&b->s));
     b->sl[0] = b->s;
-    if (p->equation == NLS_EQ_KGE && ntraj >= 2) {
+    if (ntraj >= 2) {
       b->max_lanes = std::min(nls_batch::MAXL, ntraj);
       NLS_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
       for (int k = 1; k < b->max_lanes; ++k) {

@@ -157,21 +157,24 @@ def test_fused_zero_field_stays_zero():
     assert np.all(u == 0.0)
 
 
-def test_fused_nlse_vs_oracle(orc):
+@pytest.mark.parametrize("kernel,cpt", [(0, 8), (2, 8), (2, 24)])  # direct / row march
+def test_fused_nlse_vs_oracle(orc, kernel, cpt):
     """Config-2 shape at test size: complex64 fused Strang vs the complex128
     oracle (reference Lanczos variant), rel-L2 and mass."""
     from nls.batch import FusedBatch
     from nls.samplers import nlse_packets_2d
-    ns_list = [48, 40]
+    ns_list = [48, 40, 130]
     L, dt, nt, ns = 10.0, 0.005, 101, 5
     b = FusedBatch(ns_list, L, dt, equation="nlse", method="strang", krylov_dim=12,
-                   snapshot_every=nt // ns, max_snapshots=ns, cells_per_thread=8)
+                   snapshot_every=nt // ns, max_snapshots=ns, cells_per_thread=cpt,
+                   pass_kernel=kernel)
     psis = []
     for t, n in enumerate(ns_list):
         psi, _ = nlse_packets_2d(n, L, 1, t)
         psis.append(psi)
         b.set_initial(t, np.ones((n, n)), np.ones((n, n)), psi)
     b.run(nt - 1)
+    assert b.tile()["pass_kind"] == kernel
     for t, n in enumerate(ns_list):
         out, _ = b.snapshots(t)
         ref = orc.nlse_run(n, L, dt * nt, nt, ns, np.ones((n, n)), np.ones((n, n)), psis[t],
