@@ -61,6 +61,9 @@ def fused_run(args, inputs):
     wall = time.perf_counter() - t0
     info = b.info()
     info.update(b.tile())
+    mons = [b.monitor(t) for t in range(args.ntraj)]
+    info["krylov_monitor"] = {"stalls": int(sum(x["stalls"] for x in mons)),
+                              "max_tail": float(max(x["max_tail"] for x in mons))}
     # dominant-kernel timing: a few eager steps with HIP events around each
     # Lanczos pass launch, on the batch stream
     pt = {"avg_ms": 0.0, "launches": 0, "bytes_per_launch": 0.0}
@@ -373,7 +376,7 @@ def main():
     ap.add_argument("--prof-steps", type=int, default=10)
     ap.add_argument("--plain-steps", type=int, default=40)
     ap.add_argument("--parity-steps", type=int, default=100)
-    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--cpu-steps", type=int, default=900)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-parity", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
@@ -495,6 +498,7 @@ def main():
                           "at a time", "steps": plain_steps},
         "speedup_vs_plain": (value / plain_rate) if plain_rate else None,
         "rel_l2_vs_fp64_oracle": err,
+        "krylov_monitor": info["krylov_monitor"],
         "latency": serve,
         "cpu_baseline": cpu,
         "wall_s_timed": wall,

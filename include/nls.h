@@ -290,6 +290,10 @@ NLS_API int nls_batch_pass_timing(nls_batch *b, double *avg_ms, int *launches,
  * storing only the basis vectors, w' recomputed) and the number of lanes (parts of
  * the batch stepped concurrently on their own streams); any output may be NULL. */
 NLS_API int nls_batch_tile(nls_batch *b, int *cells_per_thread, int *pass_kind, int *lanes);
+/* Krylov convergence monitor of trajectory t since creation: steps in which the weight
+ * of the last Lanczos vector in f(T) e1, |y_{m-1}| / |y|, exceeded 1e-6 (the residual did
+ * not fall within the Krylov dimension), and the largest such weight seen. */
+NLS_API int nls_batch_monitor(nls_batch *b, int t, int *stalls, double *max_tail);
 /* Device bytes a batch with these trajectories would hold (admission control). */
 NLS_API int nls_batch_estimate_bytes(const nls_batch_params *p, int ntraj,
                                      const nls_traj_desc *trajs, size_t *bytes);

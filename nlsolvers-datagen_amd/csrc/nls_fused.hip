@@ -64,7 +64,32 @@ struct TrajScal {
   double beta[NSEQ][kMaxKrylov + 1]; // beta[s][j] = b_j, j >= 1
   double y[NSEQ][kMaxKrylov];
   int status;
+  // Krylov monitor, per sequence: the largest weight |y_{m-1}| / |y| of the last basis
+  // vector in y = f(T) e1 seen so far, and the number of steps where it exceeded
+  // kStallTol.  With T[m-1][m-1] = 0 (the reference's Lanczos variant) this weight
+  // falls as dt^2 as dt shrinks, not as dt^(2(m-1)): it reads the accuracy floor of
+  // the variant as well as an unconverged Krylov space.
+  int stalls[NSEQ];
+  double max_tail[NSEQ];
 };
+constexpr double kStallTol = 1e-6;
+
+// Records the tail weight of y (lane k < m holds y_k, complex if yi != 0) for one
+// sequence; y_0 is taken as yr + y0_add (the fused KGE step evaluates cos - 1, the
+// weight is that of cos).  All lanes of the wavefront call it.
+__device__ inline void monitor_tail(TrajScal &S, int s, int m, double yr, double yi,
+                                    double y0_add) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const double r = lane == 0 ? yr + y0_add : yr;
+  const double v2 = lane < m ? r * r + yi * yi : 0.0;
+  const double tot = wave_sum(v2);
+  const double last = lane_read(v2, m - 1);
+  if (lane == 0 && tot > 0.0) {
+    const double tail = sqrt(last / tot);
+    if (tail > S.max_tail[s]) S.max_tail[s] = tail;
+    if (tail > kStallTol) S.stalls[s] += 1;
+  }
+}
 
 struct KgeArgs {
   const TrajDev *traj;
@@ -333,6 +358,7 @@ __global__ __launch_bounds__(kWave) void k_kge_small(KgeArgs a, int last_pass) {
   double y, yi;
   small_apply_f_wave(m, dk, ws.Z, kind, T.tau, 0.0, b0, 1.0, 0.0, y, yi);
   if (lane < m) S.y[s][lane] = y;
+  monitor_tail(S, s, m, y, 0.0, s == 0 ? b0 : 0.0);
 }
 
 // ---------------------------------------------------------------- combine
@@ -1503,6 +1529,7 @@ __global__ __launch_bounds__(kWave) void k_nlse_small(NlseArgs a, int last_pass)
     S.y[0][lane] = yr;
     S.y[1][lane] = yi;
   }
+  monitor_tail(S, 0, m, yr, yi, 0.0);
 }
 
 __device__ inline void nlse_psi3(const NlseArgs &a, const TrajScal &S, const double *yr,
@@ -2182,6 +2209,8 @@ struct nls_batch {
     pass_kind = best_kind;
     lanes = best_lanes;
     set_layout(best_cpt);
+    // the timing steps were discarded: restart the convergence monitor
+    NLS_HIP(hipMemsetAsync(d_scal, 0, sizeof(TrajScal) * ntraj, s));
     tuned = true;
   }
 
@@ -2494,6 +2523,22 @@ int nls_batch_info(nls_batch *b, int *krylov_dim, int64_t *cells, size_t *device
     if (krylov_dim) *krylov_dim = b->m;
     if (cells) *cells = b->total_cells();
     if (device_bytes) *device_bytes = b->bytes;
+  });
+}
+
+int nls_batch_monitor(nls_batch *b, int t, int *stalls, double *max_tail) {
+  return guard([&] {
+    NLS_REQUIRE(b && t >= 0 && t < b->ntraj, "nls_batch_monitor: bad trajectory");
+    TrajScal h;
+    NLS_HIP(hipMemcpy(&h, b->d_scal + t, sizeof(TrajScal), hipMemcpyDeviceToHost));
+    int st = 0;
+    double mt = 0.0;
+    for (int s = 0; s < NSEQ; ++s) {
+      st += h.stalls[s];
+      mt = std::max(mt, h.max_tail[s]);
+    }
+    if (stalls) *stalls = st;
+    if (max_tail) *max_tail = mt;
   });
 }
 

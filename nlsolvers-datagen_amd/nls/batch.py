@@ -103,6 +103,14 @@ class FusedBatch:
         call("nls_batch_tile", self.h, C.byref(cpt), C.byref(kind), C.byref(lanes))
         return {"cells_per_thread": cpt.value, "pass_kind": kind.value, "lanes": lanes.value}
 
+    def monitor(self, t):
+        """Krylov convergence monitor of trajectory t: steps whose last-vector weight
+        |y_{m-1}|/|y| exceeded 1e-6, and the largest weight seen."""
+        st = C.c_int()
+        mt = C.c_double()
+        call("nls_batch_monitor", self.h, t, C.byref(st), C.byref(mt))
+        return {"stalls": st.value, "max_tail": mt.value}
+
     def set_profiling(self, on):
         call("nls_batch_set_profiling", self.h, int(bool(on)))
 

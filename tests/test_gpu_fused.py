@@ -183,3 +183,29 @@ def test_fused_nlse_vs_oracle(orc, kernel, cpt):
             assert rel_l2(out[k], ref[k]) < 1e-4, (t, k)
         m0 = np.sum(np.abs(ref[0]) ** 2)
         assert abs(np.sum(np.abs(out[-1]) ** 2) - np.sum(np.abs(ref[-1]) ** 2)) / m0 < 1e-5
+
+
+
+@pytest.mark.parametrize("krylov,stalls", [(3, 2), (5, 2), (8, 0), (10, 0)])
+def test_fused_krylov_monitor_vs_oracle(orc, krylov, stalls):
+    """Krylov monitor after one step equals the last-vector weight |y_{m-1}| / |y| of the
+    oracle's reference-variant Lanczos (T[m-1][m-1] = 0) on the same inputs: cos(dt W) u
+    and sinc^2(dt W) (-m u^3), the larger of the two; steps above 1e-6 are counted."""
+    from nls.batch import FusedBatch
+    n, L, dt = 64, 10.0, 0.005
+    m, c, u0, v0 = make_case(n, 8, 0)
+    b = FusedBatch([n], L, dt, krylov_dim=krylov, mu2=1.0, cells_per_thread=16)
+    b.set_initial(0, m, c, u0, v0)
+    b.run(1)
+    mon = b.monitor(0)
+    dx = 2 * L / (n - 1)
+    op = orc.laplacian_2d(n, dx, dx, c, negate=True, shift=1.0)
+    tails = []
+    for x, kind in ((u0, 0), (-m * u0 ** 3, 1)):  # F_COS_SQRT, F_SINC2_SQRT
+        beta, _, T = orc.lanczos_real(op, x.ravel(), krylov, 0)
+        y = orc.small_matfunc_real(T, kind, dt) * beta
+        tails.append(abs(y[-1]) / np.linalg.norm(y))
+    assert abs(mon["max_tail"] - max(tails)) < 1e-3 * max(tails), (mon, tails)
+    assert mon["stalls"] == stalls == sum(t > 1e-6 for t in tails)
+    b.run(4)  # counts accumulate per step
+    assert b.monitor(0)["stalls"] == 5 * stalls
