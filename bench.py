@@ -280,7 +280,8 @@ def run_kge3d(args):
     one = np.ones((n, n, n))
     zero = np.zeros((n, n, n))
     b = FusedBatch([n] * ntraj, L, dt, method="sewi", dim=3, krylov_dim=m, mu2=1.0,
-                   cells_per_thread=args.cpt)
+                   cells_per_thread=args.cpt, lanes=args.lanes,
+                   pass_kernel=None if args.kernel < 0 else args.kernel)
     ics = [bubbles_3d(n, L, 3, t)[0] for t in range(min(ntraj, 4))]  # 4 fields, cycled
     for t in range(ntraj):
         b.set_initial(t, one, one, ics[t % len(ics)], zero)
@@ -291,6 +292,9 @@ def run_kge3d(args):
     cells = info["cells"]
     value = cells * args.steps / (ms / 1e3)
     achieved = pt["bytes_per_launch"] / (pt["avg_ms"] / 1e3) / 1e9 if pt["avg_ms"] > 0 else None
+    # algorithmic bytes per cell-step: pass 0 (24), pass 1 (40), m - 3 general passes (36),
+    # combine (read 2(m-2) basis vectors, w', u, u-, m; write u+: 8(m-2) + 36)
+    step_bytes = 24 + 40 + 36 * (m - 3) + 8 * (m - 2) + 36
     b.close()
     log(f"[bench] kge3d fused: {value:.3e} cell-steps/s ({ms / args.steps:.1f} ms/step), "
         f"pass {pt['avg_ms'] * 1e3:.1f} us -> {achieved or 0:.0f} GB/s")
@@ -343,12 +347,17 @@ def run_kge3d(args):
             "config": {"workload": f"KGE-3D sEWI batch, {ntraj} x {n}^3, side 16, m={m}, "
                                    f"dt={dt}, mu2=1", "ntraj": ntraj, "n": n, "krylov_dim": m,
                        "cells": cells, "cells_per_thread": info["cells_per_thread"],
-                       "pass_kind": info["pass_kind"]},
+                       "pass_kind": info["pass_kind"], "lanes": info["lanes"]},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
-                         "traffic": None, "kernel": "k_kge_pass<3,2,CPT> (fused 7-point pass)",
+                         "traffic": None,
+                         "kernel": ("k_kge_march3d<2,CPT> (plane-march fused 7-point pass)"
+                                    if info["pass_kind"] == 4 else
+                                    "k_kge_pass<3,2,CPT> (direct fused 7-point pass)"),
                          "bytes_per_launch": pt["bytes_per_launch"],
-                         "avg_launch_ms": pt["avg_ms"]},
+                         "avg_launch_ms": pt["avg_ms"],
+                         "step_bytes_per_cell": step_bytes,
+                         "step_effective_gbs": step_bytes * value / 1e9},
             "csr_spmv": {"gbs": spmv_gbs, "ms": spmv_s * 1e3, "nnz": nnz,
                          "bytes": spmv_bytes, "frac": spmv_gbs / HBM_PEAK_GBS},
             "rel_l2_vs_fp64_oracle_48cubed": err, "cpu_baseline": None}

@@ -252,7 +252,8 @@ typedef struct {
   int pass_kernel;   /* 0 = chosen with the tile (autotuner, or the preferred kernel for a
                         fixed tile); k + 1 = use pass kernel k only (0 direct stencil,
                         1 LDS-tiled chunk, 2 row march, 3 row march storing only the
-                        basis), autotuning its tile when cells_per_thread is 0 */
+                        basis, 4 plane march (3D, tile 4 or 8)), autotuning its tile
+                        when cells_per_thread is 0 */
   int lanes;         /* parts of the batch stepped concurrently on their own streams,
                         1..4; 0 = autotune (2 with a fixed tile) */
 } nls_batch_params;

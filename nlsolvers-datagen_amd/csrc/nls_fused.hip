@@ -53,6 +53,9 @@ constexpr int NXCD = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin
 struct TrajDev {
   int n, dim, nchunks, chunk0;
   int range; // cells per chunk (one workgroup's contiguous range)
+  // 3D plane march: chunk c = in-plane range c % npen (prange cells per plane) over
+  // the planes [zseg * (c / npen), zseg * (c / npen + 1))
+  int npen, prange, zseg;
   int64_t off, N;
   float s, mu2;
   double tau;
@@ -1139,6 +1142,300 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax,
   }
 }
 
+// ---------------------------------------------------------------- 3D plane march, stored w'
+// A workgroup owns an in-plane range [pa, pb) of one trajectory (flat offsets within a
+// plane, at most 256 * CPT cells) and marches through the planes [z0, z1).  Plane z
+// needs v_j on the flat window [z n^2 + pa - n, z n^2 + pb + n) for the x and y
+// neighbours, and v_j of planes z - 1 and z + 1 at its own cells for the z neighbours.
+// In flat coordinates the window's ends hold the previous plane's last row and the next
+// plane's first row, which are exactly the cross-plane y couplings of the reference's
+// 3D operator (laplacians.hpp:158-218, q6 set); with q6 clear the flags drop them.
+// The window lives in LDS (c and v_j of both sequences); each thread keeps, for its own
+// cells, v_j and c of planes z - 1, z, z + 1 and the previous vector of planes z, z + 1
+// in registers.  The raw inputs of plane z + 2 (own cells) and of the window ends of
+// plane z + 1 are in flight while plane z is computed.  Per cell and pass: the same
+// 36 B as the 2D row march, plus the window ends (2n cells per plane, shared with the
+// neighbouring ranges, which the block map places on the same XCD) and, per z segment,
+// the own cells of planes z0 - 1 and z1.
+// STAGE 0: j = 0 on the seeds x (stores z = A x); 1: j = 1 (from z and x); 2: j >= 2.
+constexpr int M3_HPT = 2; // window-end cells per thread: 2n <= 256 * M3_HPT
+
+inline size_t march3d_lds_bytes(int cpt, int nmax) {
+  return sizeof(float) * 3 * (size_t)(FBS * cpt + 2 * nmax);
+}
+
+template <int STAGE, int CPT>
+__global__ __launch_bounds__(FBS) void k_kge_march3d(KgeArgs a, int j, int emax) {
+  extern __shared__ float lds[];
+  const int2 cm = a.cmap[blockIdx.x];
+  if (cm.x < 0) return;
+  const TrajDev T = a.traj[cm.x];
+  __shared__ double red[NSEQ * PN];
+  __shared__ double shn[NSEQ * PN * (FBS / kWave)];
+  // v = (X - cA P) * cB when formed;  w' = A v - cC P
+  float cA[NSEQ] = {0.f, 0.f}, cB[NSEQ] = {0.f, 0.f}, cC[NSEQ] = {0.f, 0.f};
+  if (STAGE >= 1) {
+    reduce_partials(a, T, (j - 1) & 1, red);
+#pragma unroll
+    for (int s = 0; s < NSEQ; ++s) {
+      const double q0 = red[s * PN], q1 = red[s * PN + 1], q2 = red[s * PN + 2];
+      if (STAGE == 1) {
+        const double b0 = sqrt(q0);
+        const double al = q0 > 0.0 ? q1 / q0 : 0.0;
+        const double b1sq = q0 > 0.0 ? (q2 - q1 * q1 / q0) / q0 : 0.0;
+        const double b1 = (b1sq > 1e-14 * (q0 > 0.0 ? q2 / q0 : 0.0) && b1sq > 0.0) ? sqrt(b1sq) : 0.0;
+        cA[s] = (float)al;
+        cB[s] = (b0 > 0.0 && b1 > 0.0) ? (float)(1.0 / (b0 * b1)) : 0.0f;
+        cC[s] = b0 > 0.0 ? (float)(b1 / b0) : 0.0f;
+        if (cm.y == 0 && threadIdx.x == 0) {
+          a.scal[cm.x].b0[s] = b0;
+          a.scal[cm.x].alpha[s][0] = al;
+          a.scal[cm.x].beta[s][1] = b1;
+        }
+      } else {
+        double al, b;
+        lanczos_scalars(q0, q1, q2, al, b);
+        cA[s] = (float)al;
+        cB[s] = b > 0.0 ? (float)(1.0 / b) : 0.0f;
+        cC[s] = (float)b;
+        if (cm.y == 0 && threadIdx.x == 0) {
+          a.scal[cm.x].alpha[s][j - 1] = al;
+          a.scal[cm.x].beta[s][j] = b;
+        }
+      }
+    }
+  }
+  const int n = T.n, n2 = n * n, N = (int)T.N;
+  const int64_t cells = a.cells;
+  const size_t m2 = (size_t)(a.m - 2);
+  float *sc = lds, *sv0 = lds + emax, *sv1 = lds + 2 * emax;
+  const float *cT = a.cf + T.off, *mT = a.mf + T.off;
+  const double *uT = a.u + T.off;
+  const float *X0 = a.W + ((size_t)0 * 2 + (STAGE == 1 ? 0 : ((j - 1) & 1))) * cells + T.off;
+  const float *X1 = a.W + ((size_t)1 * 2 + (STAGE == 1 ? 0 : ((j - 1) & 1))) * cells + T.off;
+  const float *P0 = a.V + (STAGE >= 2 ? (0 * m2 + (size_t)(j - 2)) * cells : 0) + T.off;
+  const float *P1 = a.V + (STAGE >= 2 ? (1 * m2 + (size_t)(j - 2)) * cells : 0) + T.off;
+  float *Vout0 = a.V + (STAGE >= 1 ? (0 * m2 + (size_t)(j - 1)) * cells : 0) + T.off;
+  float *Vout1 = a.V + (STAGE >= 1 ? (1 * m2 + (size_t)(j - 1)) * cells : 0) + T.off;
+  float *Wout0 = a.W + ((size_t)0 * 2 + (j & 1)) * cells + T.off;
+  float *Wout1 = a.W + ((size_t)1 * 2 + (j & 1)) * cells + T.off;
+  const float scl = T.s, mu2 = T.mu2;
+  const int q6 = a.q6;
+  double acc[NSEQ * PN];
+#pragma unroll
+  for (int k = 0; k < NSEQ * PN; ++k) acc[k] = 0.0;
+
+  const int pen = cm.y % T.npen, zs = cm.y / T.npen;
+  const int pa = pen * T.prange;
+  const int pb = pa + T.prange < n2 ? pa + T.prange : n2;
+  const int z0 = zs * T.zseg;
+  const int z1 = z0 + T.zseg < n ? z0 + T.zseg : n;
+  const int S = pb - pa;
+
+  // raw inputs of one cell (loads only: they stay in flight until form() uses them)
+  struct Raw {
+    float c, x0, x1, p0, p1, mm;
+    double uu;
+  };
+  auto fetch = [&](int g, Raw &r) {
+    r.c = cT[g];
+    if (STAGE <= 1) {
+      r.uu = uT[g];
+      r.mm = mT[g];
+    }
+    if (STAGE >= 1) {
+      r.x0 = X0[g];
+      r.x1 = X1[g];
+    }
+    if (STAGE >= 2) {
+      r.p0 = P0[g];
+      r.p1 = P1[g];
+    }
+  };
+  // v_j of both sequences and the previous vector P from the raw inputs
+  auto form = [&](const Raw &r, float &v0, float &v1, float &q0, float &q1) {
+    float x0 = r.x0, x1 = r.x1, p0 = r.p0, p1 = r.p1;
+    if (STAGE <= 1) { // seeds x0 = u, x1 = -m u^3 (FP64 -> FP32)
+      const float s0 = (float)r.uu, s1 = (float)(-(double)r.mm * r.uu * r.uu * r.uu);
+      if (STAGE == 0) {
+        x0 = s0;
+        x1 = s1;
+      } else {
+        p0 = s0;
+        p1 = s1;
+      }
+    }
+    if (STAGE == 0) {
+      v0 = x0;
+      v1 = x1;
+      q0 = q1 = 0.0f;
+    } else {
+      v0 = (x0 - cA[0] * p0) * cB[0];
+      v1 = (x1 - cA[1] * p1) * cB[1];
+      q0 = p0;
+      q1 = p1;
+    }
+  };
+
+  // own cells: in-plane offset, column flags
+  int po[CPT];
+  bool own[CPT];
+#pragma unroll
+  for (int q = 0; q < CPT; ++q) {
+    po[q] = pa + threadIdx.x + q * FBS;
+    own[q] = threadIdx.x + q * FBS < S;
+  }
+  // planes z - 1, z, z + 1 of the own cells
+  float vm0[CPT], vm1[CPT], vc0[CPT], vc1[CPT], vn0[CPT], vn1[CPT];
+  float cmz[CPT], ccz[CPT], cnz[CPT], pc0[CPT], pc1[CPT], pn0[CPT], pn1[CPT];
+  Raw ro[CPT];        // own cells of plane z + 2 (in flight)
+  Raw rh[M3_HPT];     // window ends of plane z + 1 (in flight)
+  // window end h in [0, 2n): window index e and flat cell of plane z
+  auto hwin = [&](int h) { return h < n ? h : S + h; };
+  auto hcell = [&](int h, int z) { return z * n2 + pa - n + hwin(h); };
+  auto fetch_own = [&](int z) {
+#pragma unroll
+    for (int q = 0; q < CPT; ++q)
+      if (own[q]) fetch(z * n2 + po[q], ro[q]);
+  };
+  auto fetch_halo = [&](int z) {
+#pragma unroll
+    for (int k = 0; k < M3_HPT; ++k) {
+      const int h = threadIdx.x + k * FBS;
+      if (h < 2 * n) {
+        const int g = hcell(h, z);
+        if (g >= 0 && g < N) fetch(g, rh[k]);
+      }
+    }
+  };
+
+  if (z0 < z1) {
+    // prologue: plane z0 - 1 and plane z0 formed, plane z0 + 1 and the ends of z0 in flight
+    if (z0 > 0) {
+      fetch_own(z0 - 1);
+#pragma unroll
+      for (int q = 0; q < CPT; ++q)
+        if (own[q]) {
+          float t0, t1;
+          form(ro[q], vm0[q], vm1[q], t0, t1);
+          cmz[q] = ro[q].c;
+        }
+    }
+    fetch_own(z0);
+#pragma unroll
+    for (int q = 0; q < CPT; ++q)
+      if (own[q]) {
+        form(ro[q], vc0[q], vc1[q], pc0[q], pc1[q]);
+        ccz[q] = ro[q].c;
+      }
+    if (z0 + 1 < n) fetch_own(z0 + 1);
+    fetch_halo(z0);
+  }
+  for (int z = z0; z < z1; ++z) {
+    __syncthreads(); // the previous plane's window reads are done
+    // window of plane z: own cells from registers, ends from the loads in flight
+#pragma unroll
+    for (int q = 0; q < CPT; ++q)
+      if (own[q]) {
+        const int e = po[q] - pa + n;
+        sc[e] = ccz[q];
+        sv0[e] = vc0[q];
+        sv1[e] = vc1[q];
+      }
+#pragma unroll
+    for (int k = 0; k < M3_HPT; ++k) {
+      const int h = threadIdx.x + k * FBS;
+      if (h < 2 * n) {
+        const int g = hcell(h, z);
+        if (g >= 0 && g < N) {
+          const int e = hwin(h);
+          float t0, t1;
+          sc[e] = rh[k].c;
+          form(rh[k], sv0[e], sv1[e], t0, t1);
+        }
+      }
+    }
+    // plane z + 1 of the own cells
+    const bool up = z + 1 < n;
+    if (up) {
+#pragma unroll
+      for (int q = 0; q < CPT; ++q)
+        if (own[q]) {
+          form(ro[q], vn0[q], vn1[q], pn0[q], pn1[q]);
+          cnz[q] = ro[q].c;
+        }
+    }
+    // next loads: own cells of plane z + 2, window ends of plane z + 1
+    if (z + 1 < z1) {
+      if (z + 2 < n) fetch_own(z + 2);
+      fetch_halo(z + 1);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+      if (!own[q]) continue;
+      const int p = po[q], e = p - pa + n, g = z * n2 + p;
+      const int col = p % n;
+      const bool hzm = z > 0, hzp = up;
+      const bool hym = q6 ? g >= n : p >= n;
+      const bool hyp = q6 ? g + n < N : p < n2 - n;
+      const bool hxm = col > 0, hxp = col < n - 1;
+      const float ci = ccz[q];
+      const float wzm = hzm ? 0.5f * (ci + cmz[q]) : 0.0f;
+      const float wym = hym ? 0.5f * (ci + sc[e - n]) : 0.0f;
+      const float wxm = hxm ? 0.5f * (ci + sc[e - 1]) : 0.0f;
+      const float wxp = hxp ? 0.5f * (ci + sc[e + 1]) : 0.0f;
+      const float wyp = hyp ? 0.5f * (ci + sc[e + n]) : 0.0f;
+      const float wzp = hzp ? 0.5f * (ci + cnz[q]) : 0.0f;
+      const float wsum = ((((wzm + wym) + wxm) + wxp) + wyp) + wzp;
+#pragma unroll
+      for (int s = 0; s < NSEQ; ++s) {
+        const float *sv = s == 0 ? sv0 : sv1;
+        const float vi = s == 0 ? vc0[q] : vc1[q];
+        float nb = 0.0f;
+        if (hzm) nb += wzm * (s == 0 ? vm0[q] : vm1[q]);
+        if (hym) nb += wym * sv[e - n];
+        if (hxm) nb += wxm * sv[e - 1];
+        if (hxp) nb += wxp * sv[e + 1];
+        if (hyp) nb += wyp * sv[e + n];
+        if (hzp) nb += wzp * (s == 0 ? vn0[q] : vn1[q]);
+        const float Av = scl * (wsum * vi - nb) + mu2 * vi;
+        if (STAGE == 0) {
+          __builtin_nontemporal_store(Av, &(s == 0 ? Wout0 : Wout1)[g]);
+          acc[s * PN + 0] += (double)vi * vi;
+          acc[s * PN + 1] += (double)vi * Av;
+          acc[s * PN + 2] += (double)Av * Av;
+        } else {
+          const float wp = Av - cC[s] * (s == 0 ? pc0[q] : pc1[q]);
+          __builtin_nontemporal_store(vi, &(s == 0 ? Vout0 : Vout1)[g]);
+          __builtin_nontemporal_store(wp, &(s == 0 ? Wout0 : Wout1)[g]);
+          acc[s * PN + 0] += (double)wp * vi;
+          acc[s * PN + 1] += (double)wp * wp;
+          acc[s * PN + 2] += (double)vi * vi;
+        }
+      }
+    }
+    // roll the planes
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+      vm0[q] = vc0[q];
+      vm1[q] = vc1[q];
+      cmz[q] = ccz[q];
+      vc0[q] = vn0[q];
+      vc1[q] = vn1[q];
+      ccz[q] = cnz[q];
+      pc0[q] = pn0[q];
+      pc1[q] = pn1[q];
+    }
+  }
+  block_sum_n<NSEQ * PN>(acc, shn);
+  if (threadIdx.x == 0) {
+    double *dst = a.part + ((size_t)(j & 1) * a.total_chunks + T.chunk0 + cm.y) * (NSEQ * PN);
+#pragma unroll
+    for (int k = 0; k < NSEQ * PN; ++k) dst[k] = acc[k];
+  }
+}
+
 // Combine over the workgroup's range in groups of 4 cells per thread (independent loads).
 // v_{m-1} = (w'_{m-2} - alpha_{m-2} v_{m-2}) / b_{m-1} is folded into the
 // coefficients: d = y0 x/b0 + sum_{k<m-2} y_k v_k + (y_{m-2} - y_{m-1} alpha/b) v_{m-2}
@@ -1293,8 +1590,13 @@ static std::vector<int2> xcd_cmap(const std::vector<TrajDev> &traj, const std::v
 // the trajectory split into equal ranges (multiples of 64 cells), so the last
 // workgroup of a trajectory is not a short one.  Either way a trajectory's
 // chunking depends only on its own size and the tile.
+// Plane march (3D): in-plane ranges of at most 256 * cpt cells (equal, multiples of 64),
+// and the planes split into equal segments of at most kPlaneSeg planes (each segment
+// re-reads the two planes beside it).  Both depend only on the trajectory's own size.
+constexpr int kPlaneSeg = 64;
+
 static Layout make_layout(const nls_traj_desc *td, int ntraj, int dim, int cpt, double dt,
-                          double mu2, bool balanced = false) {
+                          double mu2, bool balanced = false, bool plane3d = false) {
   Layout L;
   const int CH = FBS * cpt;
   L.traj.resize(ntraj);
@@ -1303,14 +1605,31 @@ static Layout make_layout(const nls_traj_desc *td, int ntraj, int dim, int cpt, 
     T.n = td[t].n;
     T.dim = dim;
     T.N = dim == 3 ? (int64_t)T.n * T.n * T.n : (int64_t)T.n * T.n;
+    T.npen = 1;
+    T.prange = 0;
+    T.zseg = T.n;
+    if (plane3d) {
+      const int n2 = T.n * T.n;
+      T.npen = (n2 + CH - 1) / CH;
+      const int r = (n2 + T.npen - 1) / T.npen;
+      T.prange = std::min(CH, (r + 63) / 64 * 64);
+      T.npen = (n2 + T.prange - 1) / T.prange;
+      const int nseg = (T.n + kPlaneSeg - 1) / kPlaneSeg;
+      T.zseg = (T.n + nseg - 1) / nseg;
+    }
+  }
+  for (int t = 0; t < ntraj; ++t) {
+    TrajDev &T = L.traj[t];
     T.off = L.cells;
     L.cells += (T.N + 63) / 64 * 64; // keep every trajectory 256-byte aligned
     T.nchunks = (int)((T.N + CH - 1) / CH);
     T.range = CH;
-    if (balanced) {
+    if (plane3d) T.nchunks = T.npen * ((T.n + T.zseg - 1) / T.zseg);
+    if (balanced || plane3d) {
+      // the combine's flat ranges (for the plane march: any cover of [0, N))
       const int64_t r = (T.N + T.nchunks - 1) / T.nchunks;
       T.range = (int)((r + 63) / 64 * 64);
-      T.nchunks = (int)((T.N + T.range - 1) / T.range);
+      if (!plane3d) T.nchunks = (int)((T.N + T.range - 1) / T.range);
     }
     T.chunk0 = L.total_chunks;
     L.total_chunks += T.nchunks;
@@ -1845,7 +2164,8 @@ struct nls_batch {
   int nsnap = 0;
   hipGraphExec_t graphs[3] = {nullptr, nullptr, nullptr};
   bool profiling = false;
-  int pass_kind = 0; // 0 direct, 1 LDS-tiled chunk, 2 row march, 3 row march with recomputed w'
+  // 0 direct, 1 LDS-tiled chunk, 2 row march, 3 row march with recomputed w', 4 plane march (3D)
+  int pass_kind = 0;
   int nmax = 0;
   // Lanes: the batch split into up to MAXL parts (balanced by cells) whose steps run on
   // their own streams, so one part's small stage and kernel tails overlap the other
@@ -1992,7 +2312,8 @@ struct nls_batch {
   void set_layout(int cpt_) {
     cpt = cpt_;
     const double mu2 = p.mu2;
-    lay = make_layout(desc.data(), ntraj, p.dim, cpt, p.dt, mu2, pass_kind >= 2);
+    lay = make_layout(desc.data(), ntraj, p.dim, cpt, p.dt, mu2, pass_kind >= 2 && pass_kind <= 3,
+                      pass_kind == 4);
     nmax = 0;
     for (auto &T : lay.traj) nmax = std::max(nmax, T.n);
     NLS_REQUIRE(lay.cmap.size() <= cmap_cap, "internal: chunk map larger than allocated");
@@ -2056,11 +2377,17 @@ struct nls_batch {
 
   // Pass kernels: kind 0 = direct global-memory stencil (any dim), 1 = LDS-tiled
   // chunk + halo rows (2D), 2 = row march with an LDS window (2D), 3 = row march
-  // that stores only the basis and recomputes w' (2D, launch_rpass).
+  // that stores only the basis and recomputes w' (2D, launch_rpass), 4 = plane march
+  // (3D, in-plane ranges of 256 * cpt cells).
   template <int DIM, int STAGE> void launch_pass(const KgeArgs &a, int j, hipStream_t s,
                                                  unsigned ngrid) {
     const dim3 grid(ngrid), blk(FBS);
-    if (pass_kind == 2) {
+    if (pass_kind == 4) {
+      const size_t lds = march3d_lds_bytes(cpt, nmax);
+      const int emax = FBS * cpt + 2 * nmax;
+      if (cpt == 4) hipLaunchKernelGGL((k_kge_march3d<STAGE, 4>), grid, blk, lds, s, a, j, emax);
+      else hipLaunchKernelGGL((k_kge_march3d<STAGE, 8>), grid, blk, lds, s, a, j, emax);
+    } else if (pass_kind == 2) {
       const size_t lds = march2d_lds_bytes(nmax);
       const int emax = MSUB + 2 * nmax;
       hipLaunchKernelGGL((k_kge_march2d<STAGE>), grid, blk, lds, s, a, j, emax, 1);
@@ -2171,6 +2498,7 @@ struct nls_batch {
     if (kind == 2) return p.dim == 2 && 2 * nm <= MSUB && march2d_lds_bytes(nm) <= 65536 &&
                           c >= 4 && c <= 64;
     if (kind == 1) return p.dim == 2 && (c == 4 || c == 8) && pass2d_lds_bytes(c, nm) <= 65536;
+    if (kind == 4) return p.dim == 3 && (c == 4 || c == 8) && 2 * nm <= FBS * M3_HPT;
     return c == 4 || c == 8 || c == 16;
   }
 
@@ -2180,7 +2508,7 @@ struct nls_batch {
   void autotune() {
     const int cands[][2] = {{3, 12}, {3, 16}, {3, 24}, {3, 32}, {3, 36}, {3, 48}, {3, 64},
                             {2, 12}, {2, 16}, {2, 24}, {2, 32}, {2, 36}, {2, 48}, {2, 64},
-                            {1, 4},  {1, 8},  {0, 4},  {0, 8},  {0, 16}};
+                            {1, 4},  {1, 8},  {4, 4},  {4, 8},  {0, 4},  {0, 8},  {0, 16}};
     float best = 1e30f;
     int best_cpt = 4, best_kind = 0, best_lanes = 1;
     hipEvent_t e0, e1;
@@ -2240,7 +2568,7 @@ struct nls_batch {
       if (p.cells_per_thread > 0) {
         const int c = p.cells_per_thread;
         if (p.pass_kernel > 0) pass_kind = p.pass_kernel - 1;
-        else pass_kind = kind_ok(3, c) ? 3 : kind_ok(2, c) ? 2 : kind_ok(1, c) ? 1 : 0;
+        else pass_kind = kind_ok(3, c) ? 3 : kind_ok(2, c) ? 2 : kind_ok(4, c) ? 4 : kind_ok(1, c) ? 1 : 0;
         NLS_REQUIRE(kind_ok(pass_kind, c), "cells_per_thread / pass_kernel not supported for this batch");
         lanes = std::min(p.lanes > 0 ? p.lanes : 2, max_lanes);
         set_layout(c);
@@ -2355,7 +2683,7 @@ int nls_batch_create(const nls_batch_params *p, int ntraj, const nls_traj_desc *
       NLS_REQUIRE(p->method == NLS_NLSE_STRANG, "fused NLSE batch supports the Strang method");
     NLS_REQUIRE(p->dim == 2 || p->dim == 3, "dim must be 2 or 3");
     NLS_REQUIRE(p->dt > 0, "dt must be positive");
-    NLS_REQUIRE(p->pass_kernel >= 0 && p->pass_kernel <= 4, "pass_kernel must be in [0, 4]");
+    NLS_REQUIRE(p->pass_kernel >= 0 && p->pass_kernel <= 5, "pass_kernel must be in [0, 5]");
     NLS_REQUIRE(p->lanes >= 0 && p->lanes <= nls_batch::MAXL, "lanes must be in [0, 4]");
     NLS_REQUIRE(p->cells_per_thread == 0 ||
                     (p->cells_per_thread >= 4 && p->cells_per_thread <= 64),

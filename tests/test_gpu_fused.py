@@ -73,6 +73,61 @@ def test_fused_kge_3d_vs_oracle(orc, method):
             assert rel_l2(u[k], ur[k]) < TOL, (q6, k)
 
 
+def make_case_3d(n, seed, traj, L=8.0):
+    from nls.samplers import bubbles_3d
+    rng = np.random.default_rng(seed * 100 + traj)
+    u0, v0 = bubbles_3d(n, L, seed, traj)
+    m = 0.8 + 0.4 * rng.random((n, n, n))
+    c = 0.8 + 0.4 * rng.random((n, n, n))
+    return m, c, u0, v0
+
+
+_ORACLE_3D = {}
+
+
+# direct 7-point stencil, plane march with 1024- and 2048-cell in-plane ranges; the
+# sizes give one range (20^3), ranges that start mid-row (50^3: 896 cells) and two
+# plane segments (70^3: planes 0-34, 35-69)
+@pytest.mark.parametrize("kernel,cpt", [(0, 8), (4, 4), (4, 8)])
+@pytest.mark.parametrize("q6", [True, False])
+def test_fused_kge_3d_uneven_vs_oracle(orc, kernel, cpt, q6):
+    from nls.batch import FusedBatch
+    ns_list = [20, 50, 70]
+    L, dt, nt, ns = 8.0, 0.004, 41, 2
+    b = FusedBatch(ns_list, L, dt, method="sewi", dim=3, krylov_dim=10, mu2=1.0, q6=q6,
+                   snapshot_every=nt // ns, max_snapshots=ns, cells_per_thread=cpt,
+                   pass_kernel=kernel)
+    cases = [make_case_3d(n, 3, t) for t, n in enumerate(ns_list)]
+    for t, cs in enumerate(cases):
+        b.set_initial(t, *cs)
+    b.run(nt - 1)
+    assert b.tile()["pass_kind"] == kernel
+    for t, n in enumerate(ns_list):
+        key = (n, q6)
+        if key not in _ORACLE_3D:  # the oracle run does not depend on kernel / tile
+            m, c, u0, v0 = cases[t]
+            _ORACLE_3D[key] = orc.kge_run(n, L, dt * nt, nt, ns, m, c, u0, v0, krylov=10,
+                                          mu2=1.0, dim=3, q6=q6, method=2)[0]
+        u, _ = b.snapshots(t)
+        assert rel_l2(u[1], _ORACLE_3D[key][1]) < TOL, (t, n)
+
+
+def test_fused_3d_plane_march_alone_equals_in_batch_bitwise():
+    from nls.batch import FusedBatch
+    mix = [24, 70, 17, 40]
+    target = 1
+    cases = [make_case_3d(n, 5, t) for t, n in enumerate(mix)]
+    out = []
+    for sizes, idx in ((mix, target), ([mix[target]], 0)):
+        b = FusedBatch(sizes, 8.0, 0.004, dim=3, krylov_dim=10, mu2=1.0, cells_per_thread=4,
+                       pass_kernel=4)
+        for t, n in enumerate(sizes):
+            b.set_initial(t, *(cases[t] if len(sizes) > 1 else cases[target]))
+        b.run(20)
+        out.append(b.state(idx)[0])
+    assert np.array_equal(out[0], out[1])
+
+
 @pytest.mark.parametrize("cpt,kernel", [(8, 1), (32, 2), (16, 3)])
 def test_fused_alone_equals_in_batch_bitwise(cpt, kernel):
     from nls.batch import FusedBatch
