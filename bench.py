@@ -280,7 +280,7 @@ def run_kge3d(args):
     one = np.ones((n, n, n))
     zero = np.zeros((n, n, n))
     b = FusedBatch([n] * ntraj, L, dt, method="sewi", dim=3, krylov_dim=m, mu2=1.0,
-                   cells_per_thread=args.cpt, lanes=args.lanes,
+                   cells_per_thread=args.cpt, lanes=args.lanes, use_graph=not args.no_graph,
                    pass_kernel=None if args.kernel < 0 else args.kernel)
     ics = [bubbles_3d(n, L, 3, t)[0] for t in range(min(ntraj, 4))]  # 4 fields, cycled
     for t in range(ntraj):

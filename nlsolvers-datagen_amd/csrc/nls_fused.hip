@@ -1561,27 +1561,19 @@ struct Layout {
 };
 
 // XCD-aware block map of the trajectories in `subset`: workgroup b runs on XCD
-// b % 8, so all chunks of a trajectory are placed in one XCD's list (their halo
-// rows then meet in that XCD's L2); trajectories go largest-first to the
-// least-loaded XCD.
+// b % 8.  The chunks, in trajectory order, are cut into 8 contiguous runs of equal
+// length, one per XCD, so neighbouring ranges of a trajectory share an XCD (their halo
+// rows meet in that XCD's L2; a trajectory is split between two XCDs at most at the 7
+// cut points) and every XCD holds the same number of workgroups (a map that keeps
+// trajectories whole leaves XCDs with one trajectory more than others, and those finish
+// a full range later).
 static std::vector<int2> xcd_cmap(const std::vector<TrajDev> &traj, const std::vector<int> &subset) {
-  std::vector<int> order(subset);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int x, int y) { return traj[x].nchunks > traj[y].nchunks; });
-  std::array<std::vector<int2>, NXCD> bins;
-  std::array<int64_t, NXCD> load{};
-  for (int t : order) {
-    int best = 0;
-    for (int x = 1; x < NXCD; ++x)
-      if (load[x] < load[best]) best = x;
-    for (int c = 0; c < traj[t].nchunks; ++c) bins[best].push_back(int2{t, c});
-    load[best] += traj[t].nchunks;
-  }
-  size_t maxlen = 0;
-  for (auto &b : bins) maxlen = std::max(maxlen, b.size());
-  std::vector<int2> cmap(maxlen * NXCD, int2{-1, -1});
-  for (int x = 0; x < NXCD; ++x)
-    for (size_t k = 0; k < bins[x].size(); ++k) cmap[k * NXCD + x] = bins[x][k];
+  std::vector<int2> all;
+  for (int t : subset)
+    for (int c = 0; c < traj[t].nchunks; ++c) all.push_back(int2{t, c});
+  const size_t per = (all.size() + NXCD - 1) / NXCD;
+  std::vector<int2> cmap(per * NXCD, int2{-1, -1});
+  for (size_t k = 0; k < all.size(); ++k) cmap[(k % per) * NXCD + k / per] = all[k];
   return cmap;
 }
 
