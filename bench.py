@@ -26,7 +26,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant pass, measured with rocprofv3 counters on this
 # workload: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB copy kernel in the
 # same runs).  Keyed by (kernel, cells per launch); reported only when the tile matches.
-MEASURED_TRAFFIC = {("k_kge_march2d<2>", 9_000_000): 326.2e6}
+MEASURED_TRAFFIC = {("k_kge_march2d<2>", 9_000_000): 325.0e6}
 
 
 def log(*a):
@@ -258,10 +258,14 @@ def run_nlse2d(args):
             "data": "synthetic (Gaussian packets from seed 1, m = c = 1)",
             "config": {"workload": f"NLSE-2D Strang batch, {ntraj} x {n}^2, side 20, m={m}, "
                                    f"dt={dt}", "ntraj": ntraj, "n": n, "krylov_dim": m,
-                       "cells": cells, "cells_per_thread": info["cells_per_thread"]},
+                       "cells": cells, "cells_per_thread": info["cells_per_thread"],
+                       "pass_kind": info["pass_kind"], "lanes": info["lanes"]},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
-                         "traffic": None, "kernel": "k_nlse_pass<2,2,CPT>",
+                         "traffic": None,
+                         "kernel": ("k_nlse_march2d<2> (row-march complex64 pass)"
+                                    if info["pass_kind"] == 2 else
+                                    "k_nlse_pass<2,2,CPT> (direct complex64 pass)"),
                          "bytes_per_launch": pt["bytes_per_launch"],
                          "avg_launch_ms": pt["avg_ms"]},
             "rel_l2_vs_fp64_oracle": err, "mass_drift_fused": mass_fused,
