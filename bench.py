@@ -47,7 +47,8 @@ def fused_run(args, inputs):
     n, L = args.n, args.L
     b = FusedBatch([n] * args.ntraj, L, args.dt, krylov_dim=args.krylov, mu2=1.0,
                    use_graph=not args.no_graph, cells_per_thread=args.cpt,
-                   pass_kernel=None if args.kernel < 0 else args.kernel, lanes=args.lanes)
+                   pass_kernel=None if args.kernel < 0 else args.kernel, lanes=args.lanes,
+                   reorth=args.reorth)
     for t, (m, c, u0, v0) in enumerate(inputs):
         b.set_initial(t, m, c, u0, v0)
     b.run(1)  # autotune + graph capture
@@ -64,6 +65,10 @@ def fused_run(args, inputs):
     mons = [b.monitor(t) for t in range(args.ntraj)]
     info["krylov_monitor"] = {"stalls": int(sum(x["stalls"] for x in mons)),
                               "max_tail": float(max(x["max_tail"] for x in mons))}
+    if args.reorth:
+        info["krylov_monitor"].update(
+            max_orth=float(max(x["max_orth"] for x in mons)),
+            reorth_on=int(sum(x["reorth_on"] for x in mons)))
     # dominant-kernel timing: a few eager steps with HIP events around each
     # Lanczos pass launch, on the batch stream
     pt = {"avg_ms": 0.0, "launches": 0, "bytes_per_launch": 0.0}
@@ -385,6 +390,9 @@ def main():
                          "3 row march storing only the basis")
     ap.add_argument("--lanes", type=int, default=0, help="concurrent parts of the batch, "
                     "0 = autotuner")
+    ap.add_argument("--reorth", type=int, default=2, choices=(0, 1, 2),
+                    help="full reorthogonalisation: 0 never, 1 every iteration, 2 (default) "
+                         "for trajectories whose basis lost semi-orthogonality")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--prof-steps", type=int, default=10)
     ap.add_argument("--plain-steps", type=int, default=40)
@@ -488,7 +496,8 @@ def main():
                                f"mu2=1, dt={args.dt}, Krylov m={m}, HIP graph step capture",
                    "ntraj": args.ntraj, "n": args.n, "dt": args.dt, "krylov_dim": m,
                    "cells_per_thread": info["cells_per_thread"], "pass_kind": info["pass_kind"],
-                   "lanes": info["lanes"], "cells": cells, "parallelism": "1 GPU"},
+                   "lanes": info["lanes"], "reorth": args.reorth, "cells": cells,
+                   "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": traffic,

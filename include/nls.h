@@ -256,6 +256,14 @@ typedef struct {
                         when cells_per_thread is 0 */
   int lanes;         /* parts of the batch stepped concurrently on their own streams,
                         1..4; 0 = autotune (2 with a fixed tile) */
+  int reorth;        /* full reorthogonalisation of the FP32 Krylov basis: classical
+                        Gram-Schmidt with FP64 dot products and coefficients after each
+                        Lanczos pass (the reference's one CGS pass per iteration,
+                        device/include/lanczos.hpp:249-250).  0 = never; 1 = in every
+                        iteration; 2 = for a trajectory from the step after its basis lost
+                        semi-orthogonality, |<v_{m-2}, v_0>| > reorth_tol.  Klein-Gordon
+                        only; not with pass kernel 3 */
+  double reorth_tol; /* threshold of mode 2; 0 = 2.5e-4 (about sqrt of the FP32 roundoff) */
 } nls_batch_params;
 typedef struct {
   int n;             /* full side */
@@ -291,10 +299,13 @@ NLS_API int nls_batch_pass_timing(nls_batch *b, double *avg_ms, int *launches,
  * storing only the basis vectors, w' recomputed) and the number of lanes (parts of
  * the batch stepped concurrently on their own streams); any output may be NULL. */
 NLS_API int nls_batch_tile(nls_batch *b, int *cells_per_thread, int *pass_kind, int *lanes);
-/* Krylov convergence monitor of trajectory t since creation: steps in which the weight
- * of the last Lanczos vector in f(T) e1, |y_{m-1}| / |y|, exceeded 1e-6 (the residual did
- * not fall within the Krylov dimension), and the largest such weight seen. */
-NLS_API int nls_batch_monitor(nls_batch *b, int t, int *stalls, double *max_tail);
+/* Krylov monitor of trajectory t since creation: steps in which the weight of the last
+ * Lanczos vector in f(T) e1, |y_{m-1}| / |y|, exceeded 1e-6 (the residual did not fall
+ * within the Krylov dimension), the largest such weight, the largest loss of
+ * orthogonality |<v_{m-2}, v_0>| seen (reorth modes 1 and 2; 0 otherwise) and whether
+ * full reorthogonalisation is active for it.  Any output may be NULL. */
+NLS_API int nls_batch_monitor(nls_batch *b, int t, int *stalls, double *max_tail,
+                              double *max_orth, int *reorth_on);
 /* Device bytes a batch with these trajectories would hold (admission control). */
 NLS_API int nls_batch_estimate_bytes(const nls_batch_params *p, int ntraj,
                                      const nls_traj_desc *trajs, size_t *bytes);

@@ -74,8 +74,18 @@ struct TrajScal {
   // the variant as well as an unconverged Krylov space.
   int stalls[NSEQ];
   double max_tail[NSEQ];
+  // Orthogonality of the FP32 basis (reorthogonalisation modes 1 and 2): |<v_{m-2}, v_0>|
+  // of the previous step and its largest value; reorth_on: full reorthogonalisation is
+  // active for this trajectory (mode 2 sets it once the loss exceeds the threshold).
+  double orth[NSEQ];
+  double max_orth[NSEQ];
+  int reorth_on;
+  double ralpha[NSEQ]; // alpha_j of the pass being reorthogonalised
 };
 constexpr double kStallTol = 1e-6;
+// Default loss of orthogonality that switches reorthogonalisation on (mode 2): about the
+// square root of the FP32 unit roundoff (semi-orthogonality).
+constexpr double kReorthTol = 2.5e-4;
 
 // Records the tail weight of y (lane k < m holds y_k, complex if yi != 0) for one
 // sequence; y_0 is taken as yr + y0_add (the fused KGE step evaluates cos - 1, the
@@ -110,7 +120,43 @@ struct KgeArgs {
   int m;
   int q6;
   int kind1; // f of sequence 1
+  // full reorthogonalisation: 0 off, 1 every iteration, 2 once the basis has lost
+  // semi-orthogonality (|<v_{m-2}, v_0>| > reorth_tol)
+  int reorth;
+  double reorth_tol;
+  double *part2; // [total_chunks][NSEQ][2][kMaxKrylov]: <w'_j, v_k>, <v_j, v_k>
+  double *part3; // [total_chunks][NSEQ]: <v_{m-2}, v_0> of the last combine
+  int *any_reorth; // mode 2: set once any trajectory has switched reorthogonalisation on
 };
+
+__device__ inline bool reorth_active(const KgeArgs &a, const TrajScal &S) {
+  return a.reorth == 1 || (a.reorth == 2 && S.reorth_on);
+}
+
+// v_0 of both sequences at cell g (x / b0, x the FP32 seed)
+__device__ inline void basis0(const KgeArgs &a, const TrajScal &S, int64_t g, double &x0,
+                              double &x1) {
+  const double uu = a.u[g];
+  x0 = S.b0[0] > 0.0 ? (double)(float)uu / S.b0[0] : 0.0;
+  x1 = S.b0[1] > 0.0 ? (double)(float)(-(double)a.mf[g] * uu * uu * uu) / S.b0[1] : 0.0;
+}
+
+// Orthogonality monitor: per-chunk <v_{m-2}, x> of both sequences at cell g, x the FP32
+// seed (v_0 = x / b0; the combine kernels scale the chunk sums by 1 / b0).
+__device__ inline void orth_partial(const KgeArgs &a, int64_t g, double *o) {
+  const size_t m2 = (size_t)(a.m - 2);
+  const double uu = a.u[g];
+  o[0] += (double)(float)uu * (double)a.V[(0 * m2 + m2 - 1) * a.cells + g];
+  o[1] += (double)(float)(-(double)a.mf[g] * uu * uu * uu) *
+          (double)a.V[(1 * m2 + m2 - 1) * a.cells + g];
+}
+
+__device__ inline void store_orth(const KgeArgs &a, const TrajDev &T, const TrajScal &S, int chunk,
+                                  const double *o) {
+#pragma unroll
+  for (int s = 0; s < NSEQ; ++s)
+    a.part3[(size_t)(T.chunk0 + chunk) * NSEQ + s] = S.b0[s] > 0.0 ? o[s] / S.b0[s] : 0.0;
+}
 
 // ---------------------------------------------------------------- neighbour helpers
 struct Nbr {
@@ -343,6 +389,20 @@ __global__ __launch_bounds__(kWave) void k_kge_small(KgeArgs a, int last_pass) {
   __syncthreads();
   TrajScal &S = a.scal[t];
   const int lane = threadIdx.x;
+  if (a.reorth != 0 && m >= 3) {
+    // orthogonality of the previous step's basis (combine partials, fixed order)
+    double o = 0.0;
+    for (int g = lane; g < T.nchunks; g += kWave) o += a.part3[(size_t)(T.chunk0 + g) * NSEQ + s];
+    o = fabs(wave_sum(o));
+    if (lane == 0) {
+      S.orth[s] = o;
+      if (o > S.max_orth[s]) S.max_orth[s] = o;
+      if (a.reorth == 2 && o > a.reorth_tol) {
+        S.reorth_on = 1;
+        *a.any_reorth = 1;
+      }
+    }
+  }
   double al, b;
   lanczos_scalars(red[0], red[1], red[2], al, b);
   if (lane == 0) {
@@ -1440,7 +1500,8 @@ __global__ __launch_bounds__(FBS) void k_kge_march3d(KgeArgs a, int j, int emax)
 // v_{m-1} = (w'_{m-2} - alpha_{m-2} v_{m-2}) / b_{m-1} is folded into the
 // coefficients: d = y0 x/b0 + sum_{k<m-2} y_k v_k + (y_{m-2} - y_{m-1} alpha/b) v_{m-2}
 // + (y_{m-1}/b) w'_{m-2}, so each stored vector is read exactly once.
-template <int DIM>
+// ORTH: also the orthogonality partials <v_{m-2}, v_0> (reorthogonalisation modes 1, 2).
+template <int DIM, bool ORTH>
 __global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a) {
   const int2 cm = a.cmap[blockIdx.x];
   if (cm.x < 0) return;
@@ -1448,7 +1509,10 @@ __global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a) {
   const TrajScal &S = a.scal[cm.x];
   __shared__ double cv[NSEQ][kMaxKrylov]; // coefficient of V slot k-1 (k = 1..m-2); [0]: seed
   __shared__ double cw[NSEQ];             // coefficient of w'_{m-2}
+  __shared__ double sho[NSEQ * (FBS / kWave)];
   const int m = a.m;
+  const bool orth = ORTH && m >= 3;
+  double o[NSEQ] = {0.0, 0.0};
   if (threadIdx.x < NSEQ) {
     const int s = threadIdx.x;
     const double ib0 = S.b0[s] > 0.0 ? 1.0 / S.b0[s] : 0.0;
@@ -1472,6 +1536,7 @@ __global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a) {
     int64_t gs[G];
     bool ok[G];
     double d0[G], d1[G];
+    float xs0[G], xs1[G]; // seeds at the source cell (orthogonality monitor)
 #pragma unroll
     for (int kk = 0; kk < G; ++kk) {
       const int64_t i = gb + threadIdx.x + (int64_t)kk * FBS;
@@ -1481,9 +1546,12 @@ __global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a) {
       const double g1 = -(double)a.mf[gs[kk]] * uu * uu * uu;
       d0[kk] = cv[0][0] * (double)(float)uu + cw[0] * (double)Wl0[gs[kk]];
       d1[kk] = cv[1][0] * (double)(float)g1 + cw[1] * (double)Wl1[gs[kk]];
+      xs0[kk] = (float)uu;
+      xs1[kk] = (float)g1;
     }
+    // with the monitor, v_{m-2} is read after the loop and used twice
 #pragma unroll 2
-    for (int k = 1; k <= m - 2; ++k) {
+    for (int k = 1; k <= (orth ? m - 3 : m - 2); ++k) {
       const float *Vk0 = a.V + ((size_t)0 * (m - 2) + (k - 1)) * cells;
       const float *Vk1 = a.V + ((size_t)1 * (m - 2) + (k - 1)) * cells;
       const double y0 = cv[0][k], y1 = cv[1][k];
@@ -1491,6 +1559,26 @@ __global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a) {
       for (int kk = 0; kk < G; ++kk) {
         d0[kk] += y0 * (double)Vk0[gs[kk]];
         d1[kk] += y1 * (double)Vk1[gs[kk]];
+      }
+    }
+    if (orth) {
+      const float *VL0 = a.V + ((size_t)0 * (m - 2) + (m - 3)) * cells;
+      const float *VL1 = a.V + ((size_t)1 * (m - 2) + (m - 3)) * cells;
+      const double y0 = cv[0][m - 2], y1 = cv[1][m - 2];
+#pragma unroll
+      for (int kk = 0; kk < G; ++kk) {
+        const float l0 = VL0[gs[kk]], l1 = VL1[gs[kk]];
+        d0[kk] += y0 * (double)l0;
+        d1[kk] += y1 * (double)l1;
+        const int64_t gi = T.off + gb + threadIdx.x + (int64_t)kk * FBS;
+        if (ok[kk]) {
+          if (gs[kk] == gi) { // interior cell: its own values are the ones just read
+            o[0] += (double)xs0[kk] * (double)l0;
+            o[1] += (double)xs1[kk] * (double)l1;
+          } else {
+            orth_partial(a, gi, o);
+          }
+        }
       }
     }
 #pragma unroll
@@ -1501,11 +1589,15 @@ __global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a) {
                                     &a.un[T.off + i]);
     }
   }
+  if (orth) {
+    block_sum_n<NSEQ>(o, sho);
+    if (threadIdx.x == 0) store_orth(a, T, S, cm.y, o);
+  }
 }
 
 // Snapshot combine over the workgroup's range: also the velocity (u+ - u)/tau before
 // the ghost copy and both copies of u+ (same folded coefficients as above).
-template <int DIM>
+template <int DIM, bool ORTH>
 __global__ __launch_bounds__(FBS) void k_kge_combine_snap(KgeArgs a, SnapArgs snap) {
   const int2 cm = a.cmap[blockIdx.x];
   if (cm.x < 0) return;
@@ -1513,7 +1605,10 @@ __global__ __launch_bounds__(FBS) void k_kge_combine_snap(KgeArgs a, SnapArgs sn
   const TrajScal &S = a.scal[cm.x];
   __shared__ double cv[NSEQ][kMaxKrylov];
   __shared__ double cw[NSEQ];
+  __shared__ double sho[NSEQ * (FBS / kWave)];
   const int m = a.m;
+  const bool orth = ORTH && m >= 3;
+  double o[NSEQ] = {0.0, 0.0};
   if (threadIdx.x < NSEQ) {
     const int s = threadIdx.x;
     const double ib0 = S.b0[s] > 0.0 ? 1.0 / S.b0[s] : 0.0;
@@ -1549,6 +1644,159 @@ __global__ __launch_bounds__(FBS) void k_kge_combine_snap(KgeArgs a, SnapArgs sn
     const double ui = src == i ? us : unew(gi);
     snap.u[gi] = us;
     snap.v[gi] = (ui - a.u[gi]) / T.tau;
+    if (orth) orth_partial(a, gi, o);
+  }
+  if (orth) {
+    block_sum_n<NSEQ>(o, sho);
+    if (threadIdx.x == 0) store_orth(a, T, S, cm.y, o);
+  }
+}
+
+// ---------------------------------------------------------------- full reorthogonalisation
+// Classical Gram-Schmidt of the Lanczos residual against the earlier basis vectors, with
+// FP64 dot products and coefficients (the reference makes one such pass in every
+// iteration, lanczos.hpp:249-250).  Pass j has stored w'_j = A v_j - b_j v_{j-1}; the
+// next pass forms v_{j+1} from r = w'_j - alpha_j v_j.  Here r is made orthogonal to
+// v_0 .. v_{j-1}:
+//   h_k = <r, v_k> = <w'_j, v_k> - alpha_j <v_j, v_k>,   w'_j <- w'_j - sum_{k<j} h_k v_k,
+// and the pass-j partials are recomputed from the corrected w'_j, so the next pass's
+// alpha_j and b_{j+1} (and the combine's folded v_{m-1}) follow it.  v_0 = x / b0 is formed
+// from the seed x.  Both kernels run over the combine's flat ranges and return at once
+// for trajectories where reorthogonalisation is not active.
+
+// Per-chunk partials <w'_j, v_k> and <v_j, v_k> for k in [4 blockIdx.y, 4 blockIdx.y + 4),
+// k < j.  The first workgroup of a trajectory also records alpha_j from the pass partials.
+__global__ __launch_bounds__(FBS) void k_kge_reorth_dots(KgeArgs a, int j) {
+  const int2 cm = a.cmap[blockIdx.x];
+  if (cm.x < 0) return;
+  TrajScal &S = a.scal[cm.x];
+  if (!reorth_active(a, S)) return;
+  const TrajDev T = a.traj[cm.x];
+  __shared__ double sh[NSEQ * 8 * (FBS / kWave)];
+  if (cm.y == 0 && blockIdx.y == 0 && threadIdx.x < kWave) {
+    const int lane = threadIdx.x;
+    const double *p = a.part + ((size_t)(j & 1) * a.total_chunks + T.chunk0) * (NSEQ * PN);
+#pragma unroll
+    for (int s = 0; s < NSEQ; ++s) {
+      double q0 = 0.0, q2 = 0.0;
+      for (int g = lane; g < T.nchunks; g += kWave) {
+        q0 += p[(size_t)g * (NSEQ * PN) + s * PN];
+        q2 += p[(size_t)g * (NSEQ * PN) + s * PN + 2];
+      }
+      q0 = wave_sum(q0);
+      q2 = wave_sum(q2);
+      if (lane == 0) S.ralpha[s] = q2 > 0.0 ? q0 / q2 : 0.0;
+    }
+  }
+  const int k0 = blockIdx.y * 4;
+  const int64_t cells = a.cells;
+  const size_t m2 = (size_t)(a.m - 2);
+  const float *W0 = a.W + ((size_t)0 * 2 + (j & 1)) * cells + T.off;
+  const float *W1 = a.W + ((size_t)1 * 2 + (j & 1)) * cells + T.off;
+  const float *Vj0 = a.V + (0 * m2 + (size_t)(j - 1)) * cells + T.off;
+  const float *Vj1 = a.V + (1 * m2 + (size_t)(j - 1)) * cells + T.off;
+  double acc[NSEQ * 8]; // [s][q][w' or v_j]
+#pragma unroll
+  for (int k = 0; k < NSEQ * 8; ++k) acc[k] = 0.0;
+  const int64_t cbeg = (int64_t)cm.y * T.range;
+  const int64_t cend = cbeg + T.range < T.N ? cbeg + T.range : T.N;
+  for (int64_t i = cbeg + threadIdx.x; i < cend; i += FBS) {
+    const double w0 = W0[i], w1 = W1[i], v0 = Vj0[i], v1 = Vj1[i];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = k0 + q;
+      if (k < j) {
+        double x0, x1;
+        if (k == 0) {
+          basis0(a, S, T.off + i, x0, x1);
+        } else {
+          x0 = a.V[(0 * m2 + (size_t)(k - 1)) * cells + T.off + i];
+          x1 = a.V[(1 * m2 + (size_t)(k - 1)) * cells + T.off + i];
+        }
+        acc[(0 * 4 + q) * 2 + 0] += w0 * x0;
+        acc[(0 * 4 + q) * 2 + 1] += v0 * x0;
+        acc[(1 * 4 + q) * 2 + 0] += w1 * x1;
+        acc[(1 * 4 + q) * 2 + 1] += v1 * x1;
+      }
+    }
+  }
+  block_sum_n<NSEQ * 8>(acc, sh);
+  if (threadIdx.x == 0) {
+    double *dst = a.part2 + (size_t)(T.chunk0 + cm.y) * NSEQ * 2 * kMaxKrylov;
+#pragma unroll
+    for (int s = 0; s < NSEQ; ++s)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (k0 + q < j) {
+          dst[(s * 2 + 0) * kMaxKrylov + k0 + q] = acc[(s * 4 + q) * 2 + 0];
+          dst[(s * 2 + 1) * kMaxKrylov + k0 + q] = acc[(s * 4 + q) * 2 + 1];
+        }
+  }
+}
+
+// w'_j <- w'_j - sum_{k<j} h_k v_k and the pass-j partials <w',v_j>, <w',w'>, <v_j,v_j>
+// of the corrected w'_j (written over the pass's own).
+__global__ __launch_bounds__(FBS) void k_kge_reorth_apply(KgeArgs a, int j) {
+  const int2 cm = a.cmap[blockIdx.x];
+  if (cm.x < 0) return;
+  const TrajScal &S = a.scal[cm.x];
+  if (!reorth_active(a, S)) return;
+  const TrajDev T = a.traj[cm.x];
+  __shared__ double hk[NSEQ][kMaxKrylov];
+  __shared__ double shn[NSEQ * PN * (FBS / kWave)];
+  if (threadIdx.x < kWave) {
+    const int lane = threadIdx.x;
+    const double *p2 = a.part2 + (size_t)T.chunk0 * NSEQ * 2 * kMaxKrylov;
+    for (int s = 0; s < NSEQ; ++s) {
+      const double al = S.ralpha[s];
+      for (int k = 0; k < j; ++k) {
+        double d1 = 0.0, d2 = 0.0;
+        for (int g = lane; g < T.nchunks; g += kWave) {
+          d1 += p2[((size_t)g * NSEQ + s) * 2 * kMaxKrylov + k];
+          d2 += p2[((size_t)g * NSEQ + s) * 2 * kMaxKrylov + kMaxKrylov + k];
+        }
+        d1 = wave_sum(d1);
+        d2 = wave_sum(d2);
+        if (lane == 0) hk[s][k] = d1 - al * d2;
+      }
+    }
+  }
+  __syncthreads();
+  const int64_t cells = a.cells;
+  const size_t m2 = (size_t)(a.m - 2);
+  float *W0 = a.W + ((size_t)0 * 2 + (j & 1)) * cells + T.off;
+  float *W1 = a.W + ((size_t)1 * 2 + (j & 1)) * cells + T.off;
+  const float *Vj0 = a.V + (0 * m2 + (size_t)(j - 1)) * cells + T.off;
+  const float *Vj1 = a.V + (1 * m2 + (size_t)(j - 1)) * cells + T.off;
+  double acc[NSEQ * PN];
+#pragma unroll
+  for (int k = 0; k < NSEQ * PN; ++k) acc[k] = 0.0;
+  const int64_t cbeg = (int64_t)cm.y * T.range;
+  const int64_t cend = cbeg + T.range < T.N ? cbeg + T.range : T.N;
+  for (int64_t i = cbeg + threadIdx.x; i < cend; i += FBS) {
+    double x0, x1;
+    basis0(a, S, T.off + i, x0, x1);
+    double c0 = hk[0][0] * x0, c1 = hk[1][0] * x1;
+    for (int k = 1; k < j; ++k) {
+      c0 += hk[0][k] * (double)a.V[(0 * m2 + (size_t)(k - 1)) * cells + T.off + i];
+      c1 += hk[1][k] * (double)a.V[(1 * m2 + (size_t)(k - 1)) * cells + T.off + i];
+    }
+    const float w0 = (float)((double)W0[i] - c0), w1 = (float)((double)W1[i] - c1);
+    W0[i] = w0;
+    W1[i] = w1;
+    const float v0 = Vj0[i], v1 = Vj1[i];
+    acc[0] += (double)w0 * v0;
+    acc[1] += (double)w0 * w0;
+    acc[2] += (double)v0 * v0;
+    acc[PN + 0] += (double)w1 * v1;
+    acc[PN + 1] += (double)w1 * w1;
+    acc[PN + 2] += (double)v1 * v1;
+  }
+  block_sum_n<NSEQ * PN>(acc, shn);
+  if (threadIdx.x == 0) {
+    double *dst = a.part + ((size_t)(j & 1) * a.total_chunks + T.chunk0 + cm.y) * (NSEQ * PN);
+#pragma unroll
+    for (int k = 0; k < NSEQ * PN; ++k) dst[k] = acc[k];
   }
 }
 
@@ -2148,6 +2396,12 @@ struct nls_batch {
   size_t cmap_cap = 0;
   TrajScal *d_scal = nullptr;
   double *d_part = nullptr;
+  double *d_part2 = nullptr, *d_part3 = nullptr; // reorthogonalisation partials
+  int *d_any = nullptr;                            // mode 2: any trajectory switched on
+  // The reorthogonalisation kernels are launched (mode 1 always; mode 2 once the device
+  // reports a trajectory that needs them, checked every kReorthCheck steps).
+  bool reorth_live = false;
+  static constexpr int kReorthCheck = 16;
   float *d_mf = nullptr, *d_cf = nullptr, *d_V = nullptr, *d_W = nullptr, *d_X = nullptr;
   double *d_state[3] = {nullptr, nullptr, nullptr}; // KGE: FP64 ring of 3; NLSE: complex64 pair
   double *d_snapU = nullptr, *d_snapV = nullptr;
@@ -2226,6 +2480,11 @@ struct nls_batch {
     a.m = m;
     a.q6 = p.q6;
     a.kind1 = p.method == NLS_KGE_SEWI ? F_SINC2_HALF : F_SINC2_SQRT;
+    a.reorth = p.reorth;
+    a.reorth_tol = p.reorth_tol > 0.0 ? p.reorth_tol : kReorthTol;
+    a.part2 = d_part2;
+    a.part3 = d_part3;
+    a.any_reorth = d_any;
     return a;
   }
 
@@ -2402,6 +2661,14 @@ struct nls_batch {
     hipLaunchKernelGGL((k_kge_rmarch2d<STAGE>), grid, blk, lds, s, a, j, eP, eV, alt);
   }
 
+  // Full reorthogonalisation of w'_j (modes 1 and 2; the kernels skip trajectories where
+  // it is not active).
+  void launch_reorth(const KgeArgs &a, int j, hipStream_t st, unsigned ngrid) {
+    if (!reorth_live) return;
+    hipLaunchKernelGGL(k_kge_reorth_dots, dim3(ngrid, (j + 3) / 4), dim3(FBS), 0, st, a, j);
+    hipLaunchKernelGGL(k_kge_reorth_apply, dim3(ngrid), dim3(FBS), 0, st, a, j);
+  }
+
   // One step of the trajectories in a block map (cmap, ngrid workgroups; tlist and
   // nt for the small stage) on stream st.
   template <int DIM>
@@ -2423,19 +2690,25 @@ struct nls_batch {
       }
     } else {
       launch_pass<DIM, 0>(a, 0, st, ngrid);
-      if (m - 2 >= 1) launch_pass<DIM, 1>(a, 1, st, ngrid);
+      if (m - 2 >= 1) {
+        launch_pass<DIM, 1>(a, 1, st, ngrid);
+        launch_reorth(a, 1, st, ngrid);
+      }
       for (int j = 2; j <= m - 2; ++j) {
         if (prof) NLS_HIP(hipEventRecord(next_event(), st));
         launch_pass<DIM, 2>(a, j, st, ngrid);
         if (prof) NLS_HIP(hipEventRecord(next_event(), st));
+        launch_reorth(a, j, st, ngrid);
       }
     }
     hipLaunchKernelGGL(k_kge_small, dim3(nt * NSEQ), dim3(kWave), 0, st, a, m - 2);
     if (snapshot) {
       SnapArgs sn{d_snapU + (size_t)slot * lay.cells, d_snapV + (size_t)slot * lay.cells};
-      hipLaunchKernelGGL((k_kge_combine_snap<DIM>), grid, blk, 0, st, a, sn);
+      if (p.reorth) hipLaunchKernelGGL((k_kge_combine_snap<DIM, true>), grid, blk, 0, st, a, sn);
+      else hipLaunchKernelGGL((k_kge_combine_snap<DIM, false>), grid, blk, 0, st, a, sn);
     } else {
-      hipLaunchKernelGGL((k_kge_combine_rt<DIM>), grid, blk, 0, st, a);
+      if (p.reorth) hipLaunchKernelGGL((k_kge_combine_rt<DIM, true>), grid, blk, 0, st, a);
+      else hipLaunchKernelGGL((k_kge_combine_rt<DIM, false>), grid, blk, 0, st, a);
     }
     check_launch("fused kge step");
   }
@@ -2485,7 +2758,7 @@ struct nls_batch {
              (kind == 2 && p.dim == 2 && 2 * nm <= MSUB && march2d_lds_bytes(nm) <= 65536 &&
               c >= 4 && c <= 64);
     // row-march kernels take any range length (tile = target cells per workgroup / 256)
-    if (kind == 3) return p.dim == 2 && 3 * nm <= RKS * FBS && 2 * nm <= MSUB &&
+    if (kind == 3) return p.reorth == 0 && p.dim == 2 && 3 * nm <= RKS * FBS && 2 * nm <= MSUB &&
                           rmarch2d_lds_bytes(nm) <= 65536 && c >= 4 && c <= 64;
     if (kind == 2) return p.dim == 2 && 2 * nm <= MSUB && march2d_lds_bytes(nm) <= 65536 &&
                           c >= 4 && c <= 64;
@@ -2529,8 +2802,10 @@ struct nls_batch {
     pass_kind = best_kind;
     lanes = best_lanes;
     set_layout(best_cpt);
-    // the timing steps were discarded: restart the convergence monitor
+    // the timing steps were discarded: restart the convergence monitors
     NLS_HIP(hipMemsetAsync(d_scal, 0, sizeof(TrajScal) * ntraj, s));
+    if (d_part3) NLS_HIP(hipMemsetAsync(d_part3, 0, sizeof(double) * lay.total_chunks * NSEQ, s));
+    if (d_any) NLS_HIP(hipMemsetAsync(d_any, 0, sizeof(int), s));
     tuned = true;
   }
 
@@ -2586,6 +2861,17 @@ struct nls_batch {
         NLS_HIP(hipGraphLaunch(graphs[cur], s));
       if (snap) nsnap = std::max(nsnap, (int)(idx / p.snapshot_every) + 1);
       ++step;
+      if (p.reorth == 2 && !reorth_live && step % kReorthCheck == 0) {
+        // has a trajectory lost semi-orthogonality?  Then launch the reorthogonalisation
+        // kernels from the next step on (they act on the trajectories that need it).
+        int any = 0;
+        NLS_HIP(hipMemcpyAsync(&any, d_any, sizeof(int), hipMemcpyDeviceToHost, s));
+        NLS_HIP(hipStreamSynchronize(s));
+        if (any) {
+          reorth_live = true;
+          if (p.use_graph) capture_graphs();
+        }
+      }
     }
     NLS_HIP(hipEventRecord(e1, s));
     NLS_HIP(hipEventSynchronize(e1));
@@ -2635,6 +2921,8 @@ static size_t estimate(const nls_batch_params &p, int ntraj, const nls_traj_desc
   b += sizeof(float) * cells * NSEQ * (size_t)(m - 2);    // Krylov basis (NLSE: complex64)
   b += sizeof(float) * cells * NSEQ * 2;                  // w' ping-pong
   b += sizeof(double) * 2 * chunks * NSEQ * PN;           // partials
+  if (p.reorth)
+    b += sizeof(double) * chunks * NSEQ * (2 * kMaxKrylov + 1); // reorthogonalisation partials
   b += sizeof(TrajScal) * ntraj;
   b += sizeof(double) * cells * 2 * (size_t)std::max(p.max_snapshots, 0) * (nl ? 1 : 1);
   return b;
@@ -2676,6 +2964,11 @@ int nls_batch_create(const nls_batch_params *p, int ntraj, const nls_traj_desc *
     NLS_REQUIRE(p->dim == 2 || p->dim == 3, "dim must be 2 or 3");
     NLS_REQUIRE(p->dt > 0, "dt must be positive");
     NLS_REQUIRE(p->pass_kernel >= 0 && p->pass_kernel <= 5, "pass_kernel must be in [0, 5]");
+    NLS_REQUIRE(p->reorth >= 0 && p->reorth <= 2, "reorth must be 0, 1 or 2");
+    NLS_REQUIRE(p->reorth == 0 || p->equation == NLS_EQ_KGE,
+                "reorthogonalisation is built for the Klein-Gordon batch");
+    NLS_REQUIRE(p->reorth == 0 || p->pass_kernel != 4,
+                "reorthogonalisation needs a pass kernel that stores w' (not kernel 3)");
     NLS_REQUIRE(p->lanes >= 0 && p->lanes <= nls_batch::MAXL, "lanes must be in [0, 4]");
     NLS_REQUIRE(p->cells_per_thread == 0 ||
                     (p->cells_per_thread >= 4 && p->cells_per_thread <= 64),
@@ -2710,6 +3003,14 @@ int nls_batch_create(const nls_batch_params *p, int ntraj, const nls_traj_desc *
       }
     b->d_scal = b->alloc<TrajScal>(ntraj);
     b->d_part = b->alloc<double>((size_t)2 * l4.total_chunks * NSEQ * PN);
+    if (p->reorth) {
+      b->d_part2 = b->alloc<double>((size_t)l4.total_chunks * NSEQ * 2 * kMaxKrylov);
+      b->d_part3 = b->alloc<double>((size_t)l4.total_chunks * NSEQ);
+      NLS_HIP(hipMemsetAsync(b->d_part3, 0, sizeof(double) * l4.total_chunks * NSEQ, b->s));
+      b->d_any = b->alloc<int>(1);
+      NLS_HIP(hipMemsetAsync(b->d_any, 0, sizeof(int), b->s));
+    }
+    b->reorth_live = p->reorth == 1;
     const int64_t cells = l4.cells;
     b->d_mf = b->alloc<float>(cells);
     b->d_cf = b->alloc<float>(cells);
@@ -2846,19 +3147,24 @@ int nls_batch_info(nls_batch *b, int *krylov_dim, int64_t *cells, size_t *device
   });
 }
 
-int nls_batch_monitor(nls_batch *b, int t, int *stalls, double *max_tail) {
+int nls_batch_monitor(nls_batch *b, int t, int *stalls, double *max_tail, double *max_orth,
+                      int *reorth_on) {
   return guard([&] {
     NLS_REQUIRE(b && t >= 0 && t < b->ntraj, "nls_batch_monitor: bad trajectory");
     TrajScal h;
+    NLS_HIP(hipStreamSynchronize(b->s));
     NLS_HIP(hipMemcpy(&h, b->d_scal + t, sizeof(TrajScal), hipMemcpyDeviceToHost));
     int st = 0;
-    double mt = 0.0;
+    double mt = 0.0, mo = 0.0;
     for (int s = 0; s < NSEQ; ++s) {
       st += h.stalls[s];
       mt = std::max(mt, h.max_tail[s]);
+      mo = std::max(mo, h.max_orth[s]);
     }
     if (stalls) *stalls = st;
     if (max_tail) *max_tail = mt;
+    if (max_orth) *max_orth = mo;
+    if (reorth_on) *reorth_on = b->p.reorth == 1 || (b->reorth_live && h.reorth_on);
   });
 }
 

@@ -41,7 +41,8 @@ class BatchParams(C.Structure):
                 ("krylov_dim", C.c_int), ("mu2", C.c_double), ("q6", C.c_int),
                 ("gamma", C.c_double), ("use_graph", C.c_int), ("snapshot_every", C.c_int),
                 ("max_snapshots", C.c_int), ("cells_per_thread", C.c_int),
-                ("pass_kernel", C.c_int), ("lanes", C.c_int)]
+                ("pass_kernel", C.c_int), ("lanes", C.c_int), ("reorth", C.c_int),
+                ("reorth_tol", C.c_double)]
 
 
 class ServerParams(C.Structure):
@@ -158,7 +159,7 @@ _SIGS = {
     "nls_batch_get_state": [_P, C.c_int, _D, _D],
     "nls_batch_info": [_P, C.POINTER(C.c_int), _I64, C.POINTER(C.c_size_t)],
     "nls_batch_set_profiling": [_P, C.c_int],
-    "nls_batch_monitor": [_P, C.c_int, C.POINTER(C.c_int), _D],
+    "nls_batch_monitor": [_P, C.c_int, C.POINTER(C.c_int), _D, _D, C.POINTER(C.c_int)],
     "nls_batch_tile": [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "nls_batch_pass_timing": [_P, _D, C.POINTER(C.c_int), _D],
     "nls_batch_estimate_bytes": [C.POINTER(BatchParams), C.c_int, C.POINTER(TrajDesc),

@@ -15,7 +15,7 @@ class FusedBatch:
     def __init__(self, ns_list, Ls, dt, equation="kge", method="gautschi", dim=2, krylov_dim=0,
                  mu2=0.0, q6=True, gamma=0.0, use_graph=True, snapshot_every=0,
                  max_snapshots=0, sigmas=None, seeds=None, cells_per_thread=0,
-                 pass_kernel=None, lanes=0):
+                 pass_kernel=None, lanes=0, reorth=0, reorth_tol=0.0):
         eq = {"kge": L.EQ_KGE, "nlse": L.EQ_NLSE, "sp4": L.EQ_SP4}[equation]
         meth = {"gautschi": L.KGE_GAUTSCHI, "sewi": L.KGE_SEWI, "sv": L.KGE_SV,
                 "strang": L.NLSE_STRANG, "reference": L.SP4_REFERENCE,
@@ -29,7 +29,9 @@ class FusedBatch:
         seeds = list(seeds) if seeds is not None else [0] * k
         self.p = L.BatchParams(eq, meth, dim, dt, krylov_dim, mu2, int(q6), gamma, int(use_graph),
                                snapshot_every, max_snapshots, cells_per_thread,
-                               0 if pass_kernel is None else pass_kernel + 1, lanes)
+                               0 if pass_kernel is None else pass_kernel + 1, lanes,
+                               {"never": 0, "always": 1, "auto": 2}.get(reorth, reorth),
+                               reorth_tol)
         self.td = (L.TrajDesc * k)(*[L.TrajDesc(n, l, s, sd)
                                      for n, l, s, sd in zip(self.ns, Ls, sigmas, seeds)])
         self.h = C.c_void_p()
@@ -41,7 +43,8 @@ class FusedBatch:
         k = len(ns_list)
         Ls = list(Ls) if np.ndim(Ls) else [Ls] * k
         eq = {"kge": L.EQ_KGE, "nlse": L.EQ_NLSE, "sp4": L.EQ_SP4}[equation]
-        p = L.BatchParams(eq, 0, dim, dt, krylov_dim, mu2, 1, 0.0, 1, 0, max_snapshots, 0, 0, 0)
+        p = L.BatchParams(eq, 0, dim, dt, krylov_dim, mu2, 1, 0.0, 1, 0, max_snapshots, 0, 0, 0,
+                          0, 0.0)
         td = (L.TrajDesc * k)(*[L.TrajDesc(n, l, 0.0, 0) for n, l in zip(ns_list, Ls)])
         out = C.c_size_t()
         call("nls_batch_estimate_bytes", C.byref(p), k, td, C.byref(out))
@@ -104,12 +107,17 @@ class FusedBatch:
         return {"cells_per_thread": cpt.value, "pass_kind": kind.value, "lanes": lanes.value}
 
     def monitor(self, t):
-        """Krylov convergence monitor of trajectory t: steps whose last-vector weight
-        |y_{m-1}|/|y| exceeded 1e-6, and the largest weight seen."""
+        """Krylov monitor of trajectory t: steps whose last-vector weight |y_{m-1}|/|y|
+        exceeded 1e-6, the largest weight seen, the largest loss of orthogonality
+        |<v_{m-2}, v_0>| seen (reorthogonalisation modes 1 and 2) and whether full
+        reorthogonalisation is active."""
         st = C.c_int()
         mt = C.c_double()
-        call("nls_batch_monitor", self.h, t, C.byref(st), C.byref(mt))
-        return {"stalls": st.value, "max_tail": mt.value}
+        mo = C.c_double()
+        on = C.c_int()
+        call("nls_batch_monitor", self.h, t, C.byref(st), C.byref(mt), C.byref(mo), C.byref(on))
+        return {"stalls": st.value, "max_tail": mt.value, "max_orth": mo.value,
+                "reorth_on": bool(on.value)}
 
     def set_profiling(self, on):
         call("nls_batch_set_profiling", self.h, int(bool(on)))

@@ -5,6 +5,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -75,3 +76,16 @@ def test_batch_estimate_is_host_only():
     assert 1.9 < b300 / b150 < 2.1
     # FP64 state ring + FP32 fields + 2 x (m-2) basis + 2 x 2 w' = 24 + 8 + 64 + 16 bytes/cell
     assert abs(b300 / (100 * 90000) - 112) < 2
+
+
+def test_batch_parameter_checks_fail_before_any_device_call():
+    """Invalid combinations are rejected by nls_batch_create's checks, which run before
+    the library touches the device (so this runs without a GPU)."""
+    from nls._lib import NLSError
+    from nls.batch import FusedBatch
+    bad = [dict(reorth=3), dict(reorth="always", pass_kernel=3),
+           dict(reorth="always", equation="nlse", method="strang"), dict(pass_kernel=5),
+           dict(lanes=5), dict(cells_per_thread=2)]
+    for kw in bad:
+        with pytest.raises(NLSError):
+            FusedBatch([32], 10.0, 0.005, krylov_dim=8, **kw)

@@ -264,3 +264,87 @@ def test_fused_krylov_monitor_vs_oracle(orc, krylov, stalls):
     assert mon["stalls"] == stalls == sum(t > 1e-6 for t in tails)
     b.run(4)  # counts accumulate per step
     assert b.monitor(0)["stalls"] == 5 * stalls
+
+
+# full reorthogonalisation (classical Gram-Schmidt, FP64 coefficients) in every iteration,
+# the reference's own Lanczos form (oracle variant 0)
+@pytest.mark.parametrize("kernel,cpt", [(0, 8), (2, 24)])
+def test_fused_reorth_always_vs_oracle(orc, kernel, cpt):
+    from nls.batch import FusedBatch
+    ns_list = [48, 64, 33, 130]
+    L, dt, nt, ns = 10.0, 0.005, 101, 2
+    b = FusedBatch(ns_list, L, dt, krylov_dim=10, mu2=1.0, snapshot_every=nt // ns,
+                   max_snapshots=ns, cells_per_thread=cpt, pass_kernel=kernel, reorth="always")
+    cases = [make_case(n, 9, t, ring=(t == 1)) for t, n in enumerate(ns_list)]
+    for t, cs in enumerate(cases):
+        b.set_initial(t, *cs)
+    b.run(nt - 1)
+    for t, n in enumerate(ns_list):
+        m, c, u0, v0 = cases[t]
+        ur, _ = orc.kge_run(n, L, dt * nt, nt, ns, m, c, u0, v0, krylov=10, mu2=1.0)
+        u, _ = b.snapshots(t)
+        assert rel_l2(u[1], ur[1]) < TOL, t
+        mon = b.monitor(t)
+        assert mon["reorth_on"] and mon["max_orth"] < 1e-5, mon
+
+
+def test_fused_reorth_always_3d_plane_march_vs_oracle(orc):
+    from nls.batch import FusedBatch
+    ns_list = [20, 33]
+    L, dt, nt, ns = 8.0, 0.004, 41, 2
+    b = FusedBatch(ns_list, L, dt, method="sewi", dim=3, krylov_dim=10, mu2=1.0,
+                   snapshot_every=nt // ns, max_snapshots=ns, cells_per_thread=4, pass_kernel=4,
+                   reorth="always")
+    cases = [make_case_3d(n, 4, t) for t, n in enumerate(ns_list)]
+    for t, cs in enumerate(cases):
+        b.set_initial(t, *cs)
+    b.run(nt - 1)
+    for t, n in enumerate(ns_list):
+        m, c, u0, v0 = cases[t]
+        ur, _ = orc.kge_run(n, L, dt * nt, nt, ns, m, c, u0, v0, krylov=10, mu2=1.0, dim=3,
+                            method=2)
+        u, _ = b.snapshots(t)
+        assert rel_l2(u[1], ur[1]) < TOL, t
+
+
+def test_fused_reorth_auto_restores_orthogonality(orc):
+    """A 6 x 6 grid with m = 24 exhausts the spectrum, so the FP32 basis loses
+    orthogonality (|<v_22, v_0>| ~ 1e-2).  Mode 2 measures it in every step and, at the
+    next check (every 16 steps), starts reorthogonalising; mode 1 keeps it below 1e-5 from
+    the start; with a threshold never reached the loss is only measured."""
+    from nls.batch import FusedBatch
+    n, L, dt, steps = 6, 10.0, 0.005, 24
+    m, c, u0, v0 = make_case(n, 8, 0)
+    res = {}
+    for name, mode, tol in (("measured", "auto", 1e30), ("auto", "auto", 0.0),
+                            ("always", "always", 0.0)):
+        b = FusedBatch([n], L, dt, krylov_dim=24, mu2=1.0, cells_per_thread=4, pass_kernel=0,
+                       reorth=mode, reorth_tol=tol)
+        b.set_initial(0, m, c, u0, v0)
+        b.run(steps)
+        res[name] = (b.monitor(0), b.state(0)[0])
+    assert not res["measured"][0]["reorth_on"] and res["measured"][0]["max_orth"] > 1e-3
+    assert res["auto"][0]["reorth_on"]
+    assert res["always"][0]["reorth_on"] and res["always"][0]["max_orth"] < 1e-5
+    # state after `steps` steps against the oracle's snapshot 1 of a 2 * steps run
+    ur, _ = orc.kge_run(n, L, dt * 2 * steps, 2 * steps, 2, m, c, u0, v0, krylov=24, mu2=1.0)
+    err = {k: rel_l2(v[1], ur[1]) for k, v in res.items()}
+    print("rel-L2 vs oracle:", err)
+    assert all(e < TOL for e in err.values()), err
+
+
+def test_fused_reorth_inactive_is_bitwise_off():
+    """Mode 2 below its threshold measures orthogonality but changes nothing."""
+    from nls.batch import FusedBatch
+    ns_list = [64, 40]
+    cases = [make_case(n, 3, t) for t, n in enumerate(ns_list)]
+    out = []
+    for mode, tol in ((0, 0.0), ("auto", 1e30)):
+        b = FusedBatch(ns_list, 10.0, 0.005, krylov_dim=8, mu2=1.0, cells_per_thread=24,
+                       pass_kernel=2, lanes=1, reorth=mode, reorth_tol=tol)
+        for t, cs in enumerate(cases):
+            b.set_initial(t, *cs)
+        b.run(15)
+        out.append([b.state(t)[0] for t in range(len(ns_list))])
+    for x, y in zip(*out):
+        assert np.array_equal(x, y)
