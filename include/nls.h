@@ -280,6 +280,23 @@ NLS_API int nls_batch_destroy(nls_batch *b);
  * stores snapshot 0. */
 NLS_API int nls_batch_set_initial(nls_batch *b, int t, const double *m, const double *c,
                                   const double *u0, const double *v0);
+/* Seeded initial-condition samplers of the benchmark configurations (the Python
+ * samplers nls/samplers.py gaussian_sum_2d, kink_ring_2d, nlse_packets_2d, bubbles_3d;
+ * field forms from util.hpp:94-125, real_sampler.py:464-467, nlse_sampler.py:43-100,
+ * real_sampler.py:1775-1792).  Draw s of trajectory traj is Philox4x32-10 at counter
+ * (s, traj, tag, 0) under the 64-bit seed. */
+enum { NLS_IC_GAUSSIANS_2D = 0, NLS_IC_KINK_RING_2D = 1, NLS_IC_PACKETS_2D = 2,
+       NLS_IC_BUBBLES_3D = 3 };
+/* The parameters drawn for one trajectory, in draw order (host only, no device use):
+ * gaussians [A, w, cx, cy] x 3; kink ring [R, w]; packets [K, (A, phi, vx, vy, cx, cy) x K];
+ * bubbles [K, (A, w, cx, cy, cz) x K].  *count receives their number (at most 32). */
+NLS_API int nls_ic_draw(int kind, uint64_t seed, int traj, double *params, int cap, int *count);
+/* Initial data of trajectory t drawn and evaluated on the GPU, in place of
+ * nls_batch_set_initial with host arrays: u0 (psi0 for NLS_IC_PACKETS_2D and an NLSE
+ * batch) of sampler `kind` with Philox trajectory counter traj_index, v0 = 0, m and c
+ * constant.  Stores snapshot 0. */
+NLS_API int nls_batch_set_initial_sampled(nls_batch *b, int t, int kind, uint64_t seed,
+                                          int traj_index, double m_const, double c_const);
 /* Advances every trajectory by nsteps; *ms receives the HIP-event time of the
  * region on the batch stream (NULL to skip). */
 NLS_API int nls_batch_run(nls_batch *b, int nsteps, float *ms);

@@ -32,6 +32,7 @@
 #include "nls_philox.hpp"
 #include "nls_plain.hpp"
 #include "nls_small.hpp"
+#include "nls_ic.hpp"
 
 #include <algorithm>
 #include <array>
@@ -1800,6 +1801,37 @@ __global__ __launch_bounds__(FBS) void k_kge_reorth_apply(KgeArgs a, int j) {
   }
 }
 
+// ---------------------------------------------------------------- sampled initial data
+// Initial condition of one trajectory evaluated at every cell (nls_ic.hpp) and written
+// where set_initial would put the host arrays: KGE: u0 into the state ring's current and
+// previous slots (v0 = 0), NLSE: psi0 as complex64; m, c constant; snapshot 0 if kept.
+__global__ __launch_bounds__(FBS) void k_ic_sample(ic::Params P, int64_t N, int nlse, float mval,
+                                                   float cval, double *u, double *up, float *psi,
+                                                   float *mf, float *cf, double *snapU,
+                                                   double *snapV) {
+  const int64_t g = (int64_t)blockIdx.x * FBS + threadIdx.x;
+  if (g >= N) return;
+  double re, im;
+  ic::eval(P, g, re, im);
+  mf[g] = mval;
+  cf[g] = cval;
+  if (nlse) {
+    psi[2 * g] = (float)re;
+    psi[2 * g + 1] = (float)im;
+    if (snapU) {
+      snapU[2 * g] = re;
+      snapU[2 * g + 1] = im;
+    }
+  } else {
+    u[g] = re;
+    up[g] = re;
+    if (snapU) {
+      snapU[g] = re;
+      snapV[g] = 0.0;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- batch object
 struct Layout {
   std::vector<TrajDev> traj;
@@ -3080,6 +3112,51 @@ int nls_batch_set_initial(nls_batch *b, int t, const double *m, const double *c,
       NLS_HIP(hipMemcpyAsync(b->d_snapV + T.off, v0, sizeof(double) * N, hipMemcpyHostToDevice, b->s));
       b->nsnap = std::max(b->nsnap, 1);
     }
+    NLS_HIP(hipStreamSynchronize(b->s));
+  });
+}
+
+int nls_ic_draw(int kind, uint64_t seed, int traj, double *params, int cap, int *count) {
+  return guard([&] {
+    NLS_REQUIRE(params && count && traj >= 0, "nls_ic_draw: bad arguments");
+    double p[ic::kMaxParams];
+    const int c = ic::draw(kind, seed, (uint32_t)traj, p);
+    NLS_REQUIRE(c >= 0, "nls_ic_draw: unknown sampler kind");
+    NLS_REQUIRE(c <= cap, "nls_ic_draw: parameter buffer too small");
+    std::copy(p, p + c, params);
+    *count = c;
+  });
+}
+
+int nls_batch_set_initial_sampled(nls_batch *b, int t, int kind, uint64_t seed, int traj_index,
+                                  double m_const, double c_const) {
+  return guard([&] {
+    NLS_REQUIRE(b && t >= 0 && t < b->ntraj, "nls_batch_set_initial_sampled: bad trajectory index");
+    NLS_REQUIRE(traj_index >= 0, "nls_batch_set_initial_sampled: bad trajectory counter");
+    NLS_REQUIRE(b->step == 0, "nls_batch_set_initial_sampled: the batch has already been advanced");
+    NLS_REQUIRE(kind >= ic::GAUSSIANS_2D && kind <= ic::BUBBLES_3D, "unknown sampler kind");
+    NLS_REQUIRE(ic::dim_of(kind) == b->p.dim, "sampler dimension differs from the batch's");
+    NLS_REQUIRE((kind == ic::PACKETS_2D) == b->nlse(),
+                "complex samplers (packets) go with the NLSE batch, real ones with Klein-Gordon");
+    const TrajDev &T = b->lay.traj[t];
+    ic::Params P{};
+    P.kind = kind;
+    P.n = T.n;
+    P.L = b->desc[t].L;
+    NLS_REQUIRE(ic::draw(kind, seed, (uint32_t)traj_index, P.p) >= 0, "sampler draw failed");
+    const bool keep = b->p.max_snapshots > 0;
+    const int64_t off = T.off;
+    float *psi = b->nlse() ? (float *)b->d_state[0] + 2 * off : nullptr;
+    double *u = b->nlse() ? nullptr : b->d_state[0] + off;
+    double *up = b->nlse() ? nullptr : b->d_state[2] + off;
+    double *su = keep ? b->d_snapU + (b->nlse() ? 2 * off : off) : nullptr;
+    double *sv = keep && !b->nlse() ? b->d_snapV + off : nullptr;
+    const unsigned grid = (unsigned)((T.N + FBS - 1) / FBS);
+    hipLaunchKernelGGL(k_ic_sample, dim3(grid), dim3(FBS), 0, b->s, P, T.N, (int)b->nlse(),
+                       (float)m_const, (float)c_const, u, up, psi, b->d_mf + off, b->d_cf + off,
+                       su, sv);
+    NLS_HIP(hipGetLastError());
+    if (keep) b->nsnap = std::max(b->nsnap, 1);
     NLS_HIP(hipStreamSynchronize(b->s));
   });
 }

@@ -160,6 +160,9 @@ _SIGS = {
     "nls_batch_info": [_P, C.POINTER(C.c_int), _I64, C.POINTER(C.c_size_t)],
     "nls_batch_set_profiling": [_P, C.c_int],
     "nls_batch_monitor": [_P, C.c_int, C.POINTER(C.c_int), _D, _D, C.POINTER(C.c_int)],
+    "nls_ic_draw": [C.c_int, C.c_uint64, C.c_int, _D, C.c_int, C.POINTER(C.c_int)],
+    "nls_batch_set_initial_sampled": [_P, C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_double,
+                                      C.c_double],
     "nls_batch_tile": [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "nls_batch_pass_timing": [_P, _D, C.POINTER(C.c_int), _D],
     "nls_batch_estimate_bytes": [C.POINTER(BatchParams), C.c_int, C.POINTER(TrajDesc),

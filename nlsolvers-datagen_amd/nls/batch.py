@@ -11,6 +11,18 @@ from . import _lib as L
 from ._lib import as_f64, call, dptr
 
 
+IC_KINDS = {"gaussians_2d": 0, "kink_ring_2d": 1, "packets_2d": 2, "bubbles_3d": 3}
+
+
+def ic_params(kind, seed, traj):
+    """The sampler parameters drawn for one trajectory, as the library draws them
+    (host only; see nls_ic_draw for the layout)."""
+    buf = np.empty(32)
+    cnt = C.c_int()
+    call("nls_ic_draw", IC_KINDS[kind], int(seed), int(traj), dptr(buf), 32, C.byref(cnt))
+    return buf[:cnt.value].copy()
+
+
 class FusedBatch:
     def __init__(self, ns_list, Ls, dt, equation="kge", method="gautschi", dim=2, krylov_dim=0,
                  mu2=0.0, q6=True, gamma=0.0, use_graph=True, snapshot_every=0,
@@ -60,6 +72,15 @@ class FusedBatch:
             u = as_f64(u0).ravel()
             v = as_f64(v0 if v0 is not None else np.zeros(N)).ravel()
         call("nls_batch_set_initial", self.h, t, dptr(mf), dptr(cf), dptr(u), dptr(v))
+
+    def sample_initial(self, t, kind, seed, traj_index=None, m=1.0, c=1.0):
+        """Initial data of trajectory t drawn and evaluated on the GPU (no host arrays):
+        kind one of IC_KINDS ("gaussians_2d", "kink_ring_2d", "packets_2d", "bubbles_3d",
+        the fields of nls.samplers' gaussian_sum_2d, kink_ring_2d, nlse_packets_2d,
+        bubbles_3d for the same seed and trajectory counter), v0 = 0, constant m and c.
+        traj_index (default t) is the sampler's trajectory counter."""
+        call("nls_batch_set_initial_sampled", self.h, t, IC_KINDS[kind], int(seed),
+             t if traj_index is None else int(traj_index), float(m), float(c))
 
     def run(self, nsteps):
         ms = C.c_float()
