@@ -261,8 +261,8 @@ typedef struct {
                         Lanczos pass (the reference's one CGS pass per iteration,
                         device/include/lanczos.hpp:249-250).  0 = never; 1 = in every
                         iteration; 2 = for a trajectory from the step after its basis lost
-                        semi-orthogonality, |<v_{m-2}, v_0>| > reorth_tol.  Klein-Gordon
-                        only; not with pass kernel 3 */
+                        semi-orthogonality, |<v_{m-2}, v_0>| > reorth_tol (complex
+                        Gram-Schmidt for NLSE).  Not with pass kernel 3 */
   double reorth_tol; /* threshold of mode 2; 0 = 2.5e-4 (about sqrt of the FP32 roundoff) */
 } nls_batch_params;
 typedef struct {

@@ -1962,6 +1962,11 @@ struct NlseArgs {
   int64_t cells;
   int m, q6;
   double dt;
+  int reorth; // as KgeArgs (complex classical Gram-Schmidt)
+  double reorth_tol;
+  double *part2;   // [total_chunks][4][kMaxKrylov]: <v_k, w'_j>, <v_k, v_j> (re, im)
+  double *part3;   // [total_chunks][2]: <v_{m-2}, v_0> (re, im) of the last combine
+  int *any_reorth;
 };
 
 __device__ inline c32 nl_half_phase(c32 p, float mval, double dt) {
@@ -2102,6 +2107,25 @@ __global__ __launch_bounds__(kWave) void k_nlse_small(NlseArgs a, int last_pass)
   __syncthreads();
   TrajScal &S = a.scal[t];
   const int lane = threadIdx.x;
+  if (a.reorth != 0 && m >= 3) {
+    // orthogonality of the previous step's basis, |<v_{m-2}, v_0>| (combine partials)
+    double o0 = 0.0, o1 = 0.0;
+    for (int g = lane; g < T.nchunks; g += kWave) {
+      o0 += a.part3[(size_t)(T.chunk0 + g) * 2];
+      o1 += a.part3[(size_t)(T.chunk0 + g) * 2 + 1];
+    }
+    o0 = wave_sum(o0);
+    o1 = wave_sum(o1);
+    if (lane == 0) {
+      const double o = sqrt(o0 * o0 + o1 * o1);
+      S.orth[0] = o;
+      if (o > S.max_orth[0]) S.max_orth[0] = o;
+      if (a.reorth == 2 && o > a.reorth_tol) {
+        S.reorth_on = 1;
+        *a.any_reorth = 1;
+      }
+    }
+  }
   double al, b;
   lanczos_scalars(red[0], red[1], red[2], al, b);
   if (lane == 0) {
@@ -2161,13 +2185,31 @@ struct SnapC {
   double *psi; // complex128 interleaved, slot base
 };
 
-template <int DIM, int CPT>
+// <v_{m-2}, x> at cell g (x the seed psi1 = b0 v_0), accumulated as (re, im)
+__device__ inline void nlse_orth_partial(const NlseArgs &a, int64_t g, double *o) {
+  const c32 v = a.V[(size_t)(a.m - 3) * a.cells + g], x = a.X[g];
+  o[0] += (double)v.re * x.re + (double)v.im * x.im;
+  o[1] += (double)v.re * x.im - (double)v.im * x.re;
+}
+
+__device__ inline void nlse_store_orth(const NlseArgs &a, const TrajDev &T, const TrajScal &S,
+                                       int chunk, const double *o) {
+  const double b0 = S.b0[0];
+  a.part3[(size_t)(T.chunk0 + chunk) * 2] = b0 > 0.0 ? o[0] / b0 : 0.0;
+  a.part3[(size_t)(T.chunk0 + chunk) * 2 + 1] = b0 > 0.0 ? o[1] / b0 : 0.0;
+}
+
+// ORTH: also the orthogonality partials (reorthogonalisation modes 1 and 2).
+template <int DIM, int CPT, bool ORTH>
 __global__ __launch_bounds__(FBS) void k_nlse_combine(NlseArgs a, SnapC snap) {
   const int2 cm = a.cmap[blockIdx.x];
   if (cm.x < 0) return;
   const TrajDev T = a.traj[cm.x];
   const TrajScal &S = a.scal[cm.x];
   __shared__ double ys[2][kMaxKrylov];
+  __shared__ double sho[2 * (FBS / kWave)];
+  const bool orth = ORTH && a.m >= 3;
+  double o[2] = {0.0, 0.0};
   if (threadIdx.x < a.m) {
     ys[0][threadIdx.x] = S.y[0][threadIdx.x];
     ys[1][threadIdx.x] = S.y[1][threadIdx.x];
@@ -2190,6 +2232,11 @@ __global__ __launch_bounds__(FBS) void k_nlse_combine(NlseArgs a, SnapC snap) {
       snap.psi[2 * gi] = ir;
       snap.psi[2 * gi + 1] = ii;
     }
+    if (orth) nlse_orth_partial(a, gi, o);
+  }
+  if (orth) {
+    block_sum_n<2>(o, sho);
+    if (threadIdx.x == 0) nlse_store_orth(a, T, S, cm.y, o);
   }
 }
 
@@ -2379,13 +2426,16 @@ __global__ __launch_bounds__(FBS) void k_nlse_march2d(NlseArgs a, int j, int ema
 
 // Combine over the workgroup's range (any layout): psi2 = sum_k y_k v_k, the second half
 // phase, the pre-copy snapshot and the ghost copy.
-template <int DIM>
+template <int DIM, bool ORTH>
 __global__ __launch_bounds__(FBS) void k_nlse_combine_rt(NlseArgs a, SnapC snap) {
   const int2 cm = a.cmap[blockIdx.x];
   if (cm.x < 0) return;
   const TrajDev T = a.traj[cm.x];
   const TrajScal &S = a.scal[cm.x];
   __shared__ double ys[2][kMaxKrylov];
+  __shared__ double sho[2 * (FBS / kWave)];
+  const bool orth = ORTH && a.m >= 3;
+  double o[2] = {0.0, 0.0};
   if (threadIdx.x < a.m) {
     ys[0][threadIdx.x] = S.y[0][threadIdx.x];
     ys[1][threadIdx.x] = S.y[1][threadIdx.x];
@@ -2406,6 +2456,133 @@ __global__ __launch_bounds__(FBS) void k_nlse_combine_rt(NlseArgs a, SnapC snap)
       snap.psi[2 * gi] = ir;
       snap.psi[2 * gi + 1] = ii;
     }
+    if (orth) nlse_orth_partial(a, gi, o);
+  }
+  if (orth) {
+    block_sum_n<2>(o, sho);
+    if (threadIdx.x == 0) nlse_store_orth(a, T, S, cm.y, o);
+  }
+}
+
+// ---------------------------------------------------------------- NLSE reorthogonalisation
+// Complex classical Gram-Schmidt of the residual r = w'_j - alpha_j v_j against
+// v_0 .. v_{j-1} with FP64 dot products and coefficients (lanczos_complex.hpp's CGS pass):
+// h_k = <v_k, r> = <v_k, w'_j> - alpha_j <v_k, v_j>, w'_j <- w'_j - sum_k h_k v_k, then the
+// pass partials of the corrected w'_j.  v_0 = X / b0.  As the KGE kernels otherwise.
+__global__ __launch_bounds__(FBS) void k_nlse_reorth_dots(NlseArgs a, int j) {
+  const int2 cm = a.cmap[blockIdx.x];
+  if (cm.x < 0) return;
+  TrajScal &S = a.scal[cm.x];
+  if (!(a.reorth == 1 || (a.reorth == 2 && S.reorth_on))) return;
+  const TrajDev T = a.traj[cm.x];
+  __shared__ double sh[16 * (FBS / kWave)];
+  if (cm.y == 0 && blockIdx.y == 0 && threadIdx.x < kWave) {
+    const int lane = threadIdx.x;
+    const double *p = a.part + ((size_t)(j & 1) * a.total_chunks + T.chunk0) * PN;
+    double q0 = 0.0, q2 = 0.0;
+    for (int g = lane; g < T.nchunks; g += kWave) {
+      q0 += p[(size_t)g * PN];
+      q2 += p[(size_t)g * PN + 2];
+    }
+    q0 = wave_sum(q0);
+    q2 = wave_sum(q2);
+    if (lane == 0) S.ralpha[0] = q2 > 0.0 ? q0 / q2 : 0.0;
+  }
+  const int k0 = blockIdx.y * 4;
+  const int64_t cells = a.cells;
+  const double ib0 = S.b0[0] > 0.0 ? 1.0 / S.b0[0] : 0.0;
+  double acc[16]; // [q][<v_k,w'> re, im, <v_k,v_j> re, im]
+#pragma unroll
+  for (int k = 0; k < 16; ++k) acc[k] = 0.0;
+  const int64_t cbeg = (int64_t)cm.y * T.range;
+  const int64_t cend = cbeg + T.range < T.N ? cbeg + T.range : T.N;
+  for (int64_t i = cbeg + threadIdx.x; i < cend; i += FBS) {
+    const int64_t g = T.off + i;
+    const c32 w = a.W[(size_t)(j & 1) * cells + g], v = a.V[(size_t)(j - 1) * cells + g];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = k0 + q;
+      if (k < j) {
+        double xr, xi;
+        if (k == 0) {
+          const c32 x = a.X[g];
+          xr = x.re * ib0;
+          xi = x.im * ib0;
+        } else {
+          const c32 x = a.V[(size_t)(k - 1) * cells + g];
+          xr = x.re;
+          xi = x.im;
+        }
+        acc[q * 4 + 0] += xr * w.re + xi * w.im;
+        acc[q * 4 + 1] += xr * w.im - xi * w.re;
+        acc[q * 4 + 2] += xr * v.re + xi * v.im;
+        acc[q * 4 + 3] += xr * v.im - xi * v.re;
+      }
+    }
+  }
+  block_sum_n<16>(acc, sh);
+  if (threadIdx.x == 0) {
+    double *dst = a.part2 + (size_t)(T.chunk0 + cm.y) * 4 * kMaxKrylov;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (k0 + q < j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) dst[r * kMaxKrylov + k0 + q] = acc[q * 4 + r];
+  }
+}
+
+__global__ __launch_bounds__(FBS) void k_nlse_reorth_apply(NlseArgs a, int j) {
+  const int2 cm = a.cmap[blockIdx.x];
+  if (cm.x < 0) return;
+  const TrajScal &S = a.scal[cm.x];
+  if (!(a.reorth == 1 || (a.reorth == 2 && S.reorth_on))) return;
+  const TrajDev T = a.traj[cm.x];
+  __shared__ double hr[kMaxKrylov], hi[kMaxKrylov];
+  __shared__ double shn[PN * (FBS / kWave)];
+  if (threadIdx.x < kWave) {
+    const int lane = threadIdx.x;
+    const double *p2 = a.part2 + (size_t)T.chunk0 * 4 * kMaxKrylov;
+    const double al = S.ralpha[0];
+    for (int k = 0; k < j; ++k) {
+      double d[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int g = lane; g < T.nchunks; g += kWave)
+        for (int r = 0; r < 4; ++r) d[r] += p2[((size_t)g * 4 + r) * kMaxKrylov + k];
+      for (int r = 0; r < 4; ++r) d[r] = wave_sum(d[r]);
+      if (lane == 0) {
+        hr[k] = d[0] - al * d[2];
+        hi[k] = d[1] - al * d[3];
+      }
+    }
+  }
+  __syncthreads();
+  const int64_t cells = a.cells;
+  const double ib0 = S.b0[0] > 0.0 ? 1.0 / S.b0[0] : 0.0;
+  double acc[PN] = {0.0, 0.0, 0.0};
+  const int64_t cbeg = (int64_t)cm.y * T.range;
+  const int64_t cend = cbeg + T.range < T.N ? cbeg + T.range : T.N;
+  for (int64_t i = cbeg + threadIdx.x; i < cend; i += FBS) {
+    const int64_t g = T.off + i;
+    const c32 x = a.X[g];
+    double xr = x.re * ib0, xi = x.im * ib0;
+    double cr = hr[0] * xr - hi[0] * xi, ci = hr[0] * xi + hi[0] * xr;
+    for (int k = 1; k < j; ++k) {
+      const c32 v = a.V[(size_t)(k - 1) * cells + g];
+      cr += hr[k] * v.re - hi[k] * v.im;
+      ci += hr[k] * v.im + hi[k] * v.re;
+    }
+    c32 &wr = a.W[(size_t)(j & 1) * cells + g];
+    const c32 w = {(float)((double)wr.re - cr), (float)((double)wr.im - ci)};
+    wr = w;
+    const c32 v = a.V[(size_t)(j - 1) * cells + g];
+    acc[0] += (double)w.re * v.re + (double)w.im * v.im;
+    acc[1] += (double)w.re * w.re + (double)w.im * w.im;
+    acc[2] += (double)v.re * v.re + (double)v.im * v.im;
+  }
+  block_sum_n<PN>(acc, shn);
+  if (threadIdx.x == 0) {
+    double *dst = a.part + ((size_t)(j & 1) * a.total_chunks + T.chunk0 + cm.y) * PN;
+#pragma unroll
+    for (int k = 0; k < PN; ++k) dst[k] = acc[k];
   }
 }
 
@@ -2541,6 +2718,11 @@ struct nls_batch {
     a.m = m;
     a.q6 = p.q6;
     a.dt = p.dt;
+    a.reorth = p.reorth;
+    a.reorth_tol = p.reorth_tol > 0.0 ? p.reorth_tol : kReorthTol;
+    a.part2 = d_part2;
+    a.part3 = d_part3;
+    a.any_reorth = d_any;
     return a;
   }
 
@@ -2554,6 +2736,11 @@ struct nls_batch {
     const bool march = pass_kind == 2 && DIM == 2;
     const size_t lds = march2d_lds_bytes(nmax);
     const int emax = MSUB + 2 * nmax;
+    auto reorth = [&](int j) {
+      if (!reorth_live) return;
+      hipLaunchKernelGGL(k_nlse_reorth_dots, dim3(ngrid, (j + 3) / 4), blk, 0, s, a, j);
+      hipLaunchKernelGGL(k_nlse_reorth_apply, grid, blk, 0, s, a, j);
+    };
     if (march) {
       hipLaunchKernelGGL(k_nlse_march2d<0>, grid, blk, lds, s, a, 0, emax, 1);
       hipLaunchKernelGGL(k_nlse_march2d<1>, grid, blk, lds, s, a, 1, emax, 1);
@@ -2561,17 +2748,24 @@ struct nls_batch {
       hipLaunchKernelGGL((k_nlse_pass<DIM, 0, CPT>), grid, blk, 0, s, a, 0);
       hipLaunchKernelGGL((k_nlse_pass<DIM, 1, CPT>), grid, blk, 0, s, a, 1);
     }
+    reorth(1);
     for (int j = 2; j <= m - 2; ++j) {
       if (prof) NLS_HIP(hipEventRecord(next_event(), s));
       if (march) hipLaunchKernelGGL(k_nlse_march2d<2>, grid, blk, lds, s, a, j, emax, 1);
       else hipLaunchKernelGGL((k_nlse_pass<DIM, 2, CPT>), grid, blk, 0, s, a, j);
       if (prof) NLS_HIP(hipEventRecord(next_event(), s));
+      reorth(j);
     }
     hipLaunchKernelGGL(k_nlse_small, dim3(nt), dim3(kWave), 0, s, a, m - 2);
     SnapC sn{nullptr};
     if (snapshot) sn.psi = d_snapU + (size_t)slot * lay.cells * 2;
-    if (march) hipLaunchKernelGGL((k_nlse_combine_rt<DIM>), grid, blk, 0, s, a, sn);
-    else hipLaunchKernelGGL((k_nlse_combine<DIM, CPT>), grid, blk, 0, s, a, sn);
+    if (march) {
+      if (p.reorth) hipLaunchKernelGGL((k_nlse_combine_rt<DIM, true>), grid, blk, 0, s, a, sn);
+      else hipLaunchKernelGGL((k_nlse_combine_rt<DIM, false>), grid, blk, 0, s, a, sn);
+    } else {
+      if (p.reorth) hipLaunchKernelGGL((k_nlse_combine<DIM, CPT, true>), grid, blk, 0, s, a, sn);
+      else hipLaunchKernelGGL((k_nlse_combine<DIM, CPT, false>), grid, blk, 0, s, a, sn);
+    }
     check_launch("fused nlse step");
   }
 
@@ -2997,8 +3191,6 @@ int nls_batch_create(const nls_batch_params *p, int ntraj, const nls_traj_desc *
     NLS_REQUIRE(p->dt > 0, "dt must be positive");
     NLS_REQUIRE(p->pass_kernel >= 0 && p->pass_kernel <= 5, "pass_kernel must be in [0, 5]");
     NLS_REQUIRE(p->reorth >= 0 && p->reorth <= 2, "reorth must be 0, 1 or 2");
-    NLS_REQUIRE(p->reorth == 0 || p->equation == NLS_EQ_KGE,
-                "reorthogonalisation is built for the Klein-Gordon batch");
     NLS_REQUIRE(p->reorth == 0 || p->pass_kernel != 4,
                 "reorthogonalisation needs a pass kernel that stores w' (not kernel 3)");
     NLS_REQUIRE(p->lanes >= 0 && p->lanes <= nls_batch::MAXL, "lanes must be in [0, 4]");

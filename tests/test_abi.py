@@ -83,8 +83,7 @@ def test_batch_parameter_checks_fail_before_any_device_call():
     the library touches the device (so this runs without a GPU)."""
     from nls._lib import NLSError
     from nls.batch import FusedBatch
-    bad = [dict(reorth=3), dict(reorth="always", pass_kernel=3),
-           dict(reorth="always", equation="nlse", method="strang"), dict(pass_kernel=5),
+    bad = [dict(reorth=3), dict(reorth="always", pass_kernel=3), dict(pass_kernel=5),
            dict(lanes=5), dict(cells_per_thread=2)]
     for kw in bad:
         with pytest.raises(NLSError):

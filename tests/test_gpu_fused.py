@@ -348,3 +348,47 @@ def test_fused_reorth_inactive_is_bitwise_off():
         out.append([b.state(t)[0] for t in range(len(ns_list))])
     for x, y in zip(*out):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("kernel,cpt", [(0, 8), (2, 16)])
+def test_fused_nlse_reorth_always_vs_oracle(orc, kernel, cpt):
+    """Complex Gram-Schmidt in every iteration (the reference's NLSE Lanczos form)."""
+    from nls.batch import FusedBatch
+    from nls.samplers import nlse_packets_2d
+    ns_list = [48, 40, 130]
+    L, dt, nt, ns = 10.0, 0.005, 41, 2
+    b = FusedBatch(ns_list, L, dt, equation="nlse", method="strang", krylov_dim=12,
+                   snapshot_every=nt // ns, max_snapshots=ns, cells_per_thread=cpt,
+                   pass_kernel=kernel, reorth="always")
+    psis = []
+    for t, n in enumerate(ns_list):
+        psi, _ = nlse_packets_2d(n, L, 1, t)
+        psis.append(psi)
+        b.set_initial(t, np.ones((n, n)), np.ones((n, n)), psi)
+    b.run(nt - 1)
+    for t, n in enumerate(ns_list):
+        out, _ = b.snapshots(t)
+        ref = orc.nlse_run(n, L, dt * nt, nt, ns, np.ones((n, n)), np.ones((n, n)), psis[t],
+                           krylov=12, method=0, variant=0)
+        assert rel_l2(out[1], ref[1]) < 1e-4, t
+        mon = b.monitor(t)
+        assert mon["reorth_on"] and mon["max_orth"] < 1e-5, mon
+
+
+def test_fused_nlse_reorth_auto_restores_orthogonality(orc):
+    from nls.batch import FusedBatch
+    from nls.samplers import nlse_packets_2d
+    n, L, dt, steps = 6, 10.0, 0.005, 24
+    psi, _ = nlse_packets_2d(n, L, 1, 0)
+    res = {}
+    for name, mode, tol in (("measured", "auto", 1e30), ("auto", "auto", 0.0),
+                            ("always", "always", 0.0)):
+        b = FusedBatch([n], L, dt, equation="nlse", method="strang", krylov_dim=24,
+                       cells_per_thread=4, pass_kernel=0, reorth=mode, reorth_tol=tol)
+        b.set_initial(0, np.ones((n, n)), np.ones((n, n)), psi)
+        b.run(steps)
+        res[name] = b.monitor(0)
+    print(res)
+    assert not res["measured"]["reorth_on"] and res["measured"]["max_orth"] > 1e-3
+    assert res["auto"]["reorth_on"]
+    assert res["always"]["reorth_on"] and res["always"]["max_orth"] < 1e-5
