@@ -388,28 +388,20 @@ struct nls_server {
         for (auto &r : act) K = std::min(K, steps_to_event(*r));
         const auto t0 = std::chrono::steady_clock::now();
         if (K > 0) {
-          // XCD-aware block map: request's chunks on one XCD, largest first
-          std::vector<int> order(act.size());
-          std::iota(order.begin(), order.end(), 0);
-          std::vector<int> nch(act.size());
-          for (size_t k = 0; k < act.size(); ++k) nch[k] = (int)((act[k]->N + SBS * SCPT - 1) / (SBS * SCPT));
-          std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nch[a] > nch[b]; });
-          std::array<std::vector<int2>, NXCD> bins;
-          std::array<int64_t, NXCD> load{};
-          for (int t : order) {
-            int best = 0;
-            for (int x = 1; x < NXCD; ++x)
-              if (load[x] < load[best]) best = x;
-            for (int c = 0; c < nch[t]; ++c) bins[best].push_back(int2{t, c});
-            load[best] += nch[t];
+          // XCD-aware block map (workgroup b runs on XCD b % 8): the chunks, request by
+          // request, cut into 8 contiguous runs of equal length, one per XCD, so a
+          // request's neighbouring chunks share an XCD's L2 and no XCD holds more
+          // workgroups than another (a large request is split across XCDs)
+          std::vector<int2> all;
+          for (size_t k = 0; k < act.size(); ++k) {
+            const int nch = (int)((act[k]->N + SBS * SCPT - 1) / (SBS * SCPT));
+            for (int c = 0; c < nch; ++c) all.push_back(int2{(int)k, c});
           }
-          size_t maxlen = 0;
-          for (auto &b : bins) maxlen = std::max(maxlen, b.size());
-          const size_t nblk = maxlen * NXCD;
+          const size_t per = (all.size() + NXCD - 1) / NXCD;
+          const size_t nblk = per * NXCD;
           ensure_capacity(act.size(), nblk);
           for (size_t k = 0; k < nblk; ++k) h_cmap[k] = int2{-1, -1};
-          for (int x = 0; x < NXCD; ++x)
-            for (size_t k = 0; k < bins[x].size(); ++k) h_cmap[k * NXCD + x] = bins[x][k];
+          for (size_t k = 0; k < all.size(); ++k) h_cmap[(k % per) * NXCD + k / per] = all[k];
           double cells = 0;
           for (size_t k = 0; k < act.size(); ++k) {
             h_reqs[k] = desc(*act[k]);
