@@ -44,7 +44,11 @@ namespace nls {
 namespace serve {
 
 constexpr int SBS = 256;
-constexpr int SCPT = 4;
+// One cell per thread: a step of a small group is latency-bound (dependent loads of the
+// block map, the request, the state; Philox and the FP64 Box-Muller), so short threads
+// and many workgroups finish it sooner (4 cells per thread: p50 62 ms, 1 cell: 30 ms on
+// the Poisson 50 req/s workload).
+constexpr int SCPT = 1;
 constexpr int NXCD = 8;
 
 struct ReqDev {
