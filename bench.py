@@ -206,6 +206,76 @@ def _timed_batch(b, warmup, steps):
     return ms, pt
 
 
+def run_kge2d_uneven(args):
+    """Uneven batch (KGE-2D, phi^4 nonlinearity, mu2 = 1): 100 trajectories with n drawn
+    from {128, 192, 256, 300, 384} (Philox, seed 2), side 20, ring/kink initial data of
+    seed 2 drawn and evaluated on the GPU, dt = 0.005, Krylov m from the autotuner,
+    a snapshot every 150 steps kept on the device.  Timed over 300 steps (two snapshot
+    steps included); parity of the smallest trajectory against the FP64 oracle at the
+    first snapshot."""
+    from nls.batch import FusedBatch
+    from nls.samplers import ParamStream, kink_ring_2d
+    ntraj, L, dt, every = args.ntraj or 100, 10.0, 0.005, 150
+    choices = (128, 192, 256, 300, 384)
+    sizes = [choices[ParamStream(2, t, tag=0x5153).integer(0, 4)] for t in range(ntraj)]
+    b = FusedBatch(sizes, L, dt, krylov_dim=args.krylov, mu2=1.0, snapshot_every=every,
+                   max_snapshots=args.steps // every + 2, cells_per_thread=args.cpt,
+                   lanes=args.lanes, use_graph=not args.no_graph, reorth=args.reorth,
+                   pass_kernel=None if args.kernel < 0 else args.kernel)
+    for t in range(ntraj):
+        b.sample_initial(t, "kink_ring_2d", 2, m=1.0, c=1.0)
+    ms, pt = _timed_batch(b, args.warmup, args.steps)
+    info = b.info()
+    info.update(b.tile())
+    cells = info["cells"]
+    value = cells * args.steps / (ms / 1e3)
+    achieved = pt["bytes_per_launch"] / (pt["avg_ms"] / 1e3) / 1e9 if pt["avg_ms"] > 0 else None
+    log(f"[bench] kge2d uneven: {value:.3e} cell-steps/s ({ms / args.steps:.3f} ms/step), "
+        f"m={info['krylov_dim']}, tile {info['cells_per_thread']} kind={info['pass_kind']} "
+        f"lanes={info['lanes']}, pass {pt['avg_ms'] * 1e3:.1f} us -> {achieved or 0:.0f} GB/s")
+    err = None
+    if not args.skip_parity:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        t = sizes.index(min(sizes))
+        n = sizes[t]
+        S = args.parity_steps
+        p = FusedBatch([n], L, dt, krylov_dim=info["krylov_dim"], mu2=1.0, snapshot_every=S,
+                       max_snapshots=2, cells_per_thread=info["cells_per_thread"],
+                       pass_kernel=info["pass_kind"])
+        p.sample_initial(0, "kink_ring_2d", 2, traj_index=t)
+        p.run(S)
+        u, _ = p.snapshots(0)
+        u0, v0, _ = kink_ring_2d(n, L, 2, t)
+        one = np.ones((n, n))
+        ur, _ = orc.kge_run(n, L, dt * 2 * S, 2 * S, 2, one, one, u0, v0,
+                            krylov=info["krylov_dim"], mu2=1.0)
+        err = float(np.linalg.norm(u[1] - ur[1]) / np.linalg.norm(ur[1]))
+        log(f"[bench] kge2d uneven rel-L2 ({n}^2, {S} steps) {err:.2e}")
+    b.close()
+    return {"metric": "KGE-2D uneven batch cell-steps/s (fused, snapshots on the device)",
+            "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms / args.steps, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None,
+            "dtype": "fp32 Krylov vectors / fp64 state and small stage",
+            "data": "synthetic (ring/kink initial data of seed 2 evaluated on the GPU; grid "
+                    "sizes drawn from seed 2; m = c = 1)",
+            "config": {"workload": f"KGE-2D batched uneven, {ntraj} trajectories, n in "
+                                   f"{{128,192,256,300,384}}, side 20, mu2=1, dt={dt}, "
+                                   f"snapshot every {every} steps", "ntraj": ntraj,
+                       "sizes": {str(c): sizes.count(c) for c in choices}, "cells": cells,
+                       "krylov_dim": info["krylov_dim"],
+                       "cells_per_thread": info["cells_per_thread"],
+                       "pass_kind": info["pass_kind"], "lanes": info["lanes"],
+                       "reorth": args.reorth},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                         "traffic": None, "kernel": "dominant Lanczos pass (stage 2)",
+                         "bytes_per_launch": pt["bytes_per_launch"],
+                         "avg_launch_ms": pt["avg_ms"]},
+            "rel_l2_vs_fp64_oracle": err, "cpu_baseline": None}
+
+
 def run_nlse2d(args):
     """Secondary workload: 2D cubic NLSE, 128 x 200^2 on [-10, 10]^2, complex64 fields,
     Strang with Krylov exp(i dt L), m = 12, dt = 0.005, Gaussian packets from seed 1;
@@ -372,7 +442,8 @@ def run_kge3d(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="kge2d", choices=["kge2d", "nlse2d", "kge3d"],
+    ap.add_argument("--workload", default="kge2d",
+                    choices=["kge2d", "kge2d_uneven", "nlse2d", "kge3d"],
                     help="kge2d: the headline batch; nlse2d, kge3d: secondary workloads")
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=20)
@@ -418,10 +489,13 @@ def main():
         if "--n" not in sys.argv:
             args.n = 0
         if "--steps" not in sys.argv:
-            args.steps = 40 if args.workload == "nlse2d" else 5
+            args.steps = {"nlse2d": 40, "kge3d": 5, "kge2d_uneven": 300}[args.workload]
         if "--warmup" not in sys.argv:
-            args.warmup = 5 if args.workload == "nlse2d" else 2
-        line = run_nlse2d(args) if args.workload == "nlse2d" else run_kge3d(args)
+            args.warmup = {"nlse2d": 5, "kge3d": 2, "kge2d_uneven": 20}[args.workload]
+        if "--krylov" not in sys.argv and args.workload == "kge2d_uneven":
+            args.krylov = 0  # the autotuner picks m
+        run = {"nlse2d": run_nlse2d, "kge3d": run_kge3d, "kge2d_uneven": run_kge2d_uneven}
+        line = run[args.workload](args)
         print(json.dumps(line), flush=True)
         return
     log(f"[bench] building inputs: {args.ntraj} x {args.n}^2")
