@@ -396,8 +396,8 @@ def test_fused_nlse_reorth_auto_restores_orthogonality(orc):
 
 def test_fused_long_horizon_vs_oracle(orc):
     """3000 steps (a fifth of the uneven-batch configuration's nt = 15000) on a 128^2 ring:
-    the FP32-Krylov / FP64-state error grows about linearly (7.8e-7 at step 3000 when
-    recorded) and stays below the 1e-5 bar at every snapshot."""
+    the FP32-Krylov / FP64-state error grows about linearly (7.8e-7 at the step-2500
+    snapshot when recorded) and stays below the 1e-5 bar at every snapshot."""
     from nls.batch import FusedBatch
     n, L, dt, nt, ns = 128, 10.0, 0.005, 3001, 6
     m, c, u0, v0 = make_case(n, 2, 1, ring=True)
