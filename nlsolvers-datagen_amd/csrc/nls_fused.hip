@@ -2090,7 +2090,6 @@ __global__ __launch_bounds__(FBS) void k_nlse_pass(NlseArgs a, int j) {
 __global__ __launch_bounds__(kWave) void k_nlse_small(NlseArgs a, int last_pass) {
   const int t = a.tlist ? a.tlist[blockIdx.x] : (int)blockIdx.x;
   const TrajDev T = a.traj[t];
-  __shared__ SmallWS ws;
   __shared__ double red[PN];
   const int m = a.m;
   {
@@ -2134,12 +2133,11 @@ __global__ __launch_bounds__(kWave) void k_nlse_small(NlseArgs a, int last_pass)
   }
   double dk = lane < m - 2 ? S.alpha[0][lane] : (lane == m - 2 ? al : 0.0); // d_{m-1} = 0 (Q1)
   double ek = lane < m - 2 ? S.beta[0][lane + 1] : (lane == m - 2 ? b : 0.0);
-  if (small_ql_wave(m, dk, ek, ws.Z)) {
-    if (lane == 0) S.status = 1;
-    return;
-  }
+  // exp(i dt T) e1 is analytic in T: a scaled Taylor series gives it without the QL
+  // iteration (the eigen-decomposition route, small_ql_wave + small_apply_f_wave, takes
+  // ~84 us for m = 12 against a few us here)
   double yr, yi;
-  small_apply_f_wave(m, dk, ws.Z, F_EXP, 0.0, a.dt, S.b0[0], 1.0, 0.0, yr, yi);
+  small_exp_series_wave(m, dk, ek, 0.0, a.dt, S.b0[0], yr, yi);
   if (lane < m) {
     S.y[0][lane] = yr;
     S.y[1][lane] = yi;

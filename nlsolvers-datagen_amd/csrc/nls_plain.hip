@@ -314,6 +314,17 @@ __global__ __launch_bounds__(kWave) void k_small_de(int m, const double *__restr
                                                     double t_re, double t_im, double beta0,
                                                     double *yr, double *yi, int *status) {
   __shared__ SmallWS ws;
+  if (kind == F_EXP_SERIES) { // exp(t T) e1 without the eigen-decomposition
+    const int lane = threadIdx.x;
+    double r, i;
+    small_exp_series_wave(m, lane < m ? d[lane] : 0.0, lane < m - 1 ? e[lane] : 0.0, t_re, t_im,
+                          beta0, r, i);
+    if (lane < m) {
+      yr[lane] = r;
+      yi[lane] = i;
+    }
+    return;
+  }
   if (threadIdx.x == 0) {
     for (int i = 0; i < m; ++i) { ws.d[i] = d[i]; ws.phr[i] = 1.0; ws.phi[i] = 0.0; }
     for (int i = 0; i < m - 1; ++i) ws.e[i] = e[i];

@@ -28,6 +28,7 @@ enum FuncKind : int {
   F_COSM1_SQRT = 4, // cos(x) - 1 = -2 sin^2(x/2)    (increment form used by the fused path)
   F_EXP = 8,        // exp(dt l), dt complex         (matfunc_complex.hpp:281-287)
   F_SINC = 9,       // sin(dt l)/(dt l), 1 if |.| < 1e-8  (matfunc_complex.hpp:293-300)
+  F_EXP_SERIES = 10, // exp(dt T) e1 by a scaled Taylor series (no eigen-decomposition)
 };
 
 struct cplx {
@@ -251,6 +252,60 @@ __device__ inline void small_apply_f_wave(int m, double lam, double (*Z)[kMaxKry
   }
   yr = beta0 * (phr * ar - phi * ai);
   yi = beta0 * (phr * ai + phi * ar);
+}
+
+// y = beta0 exp(t T) e1 (t = t_re + i t_im) for the symmetric tridiagonal T of the
+// wavefront layout of small_ql_wave (lane k: d_k, e_k = T[k][k+1]; lanes >= m ignored),
+// lane k < m receiving y_k.  exp is analytic, so the series gives what the
+// eigen-decomposition gives (the same function of T) without the QL iteration: the step
+// t is cut into s pieces with |t| ||T||_inf / s <= 1 and each piece applies
+// sum_k (t/s)^k T^k / k! to the running vector, summed until a term drops below 1e-17 of
+// the sum (at most 40 terms; the loop length is uniform across the wavefront).
+__device__ inline void small_exp_series_wave(int m, double dk, double ek, double t_re,
+                                             double t_im, double beta0, double &yr,
+                                             double &yi) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const bool in = lane < m;
+  const double d = in ? dk : 0.0;
+  const double e = lane < m - 1 ? ek : 0.0;      // T[lane][lane+1]
+  const double el = __shfl_up(e, 1, kWave);      // T[lane-1][lane]
+  const double eu = lane > 0 ? el : 0.0;
+  double nu = in ? fabs(d) + fabs(eu) + fabs(e) : 0.0;
+  for (int off = 32; off >= 1; off >>= 1) nu = fmax(nu, __shfl_xor(nu, off, kWave));
+  const double tabs = sqrt(t_re * t_re + t_im * t_im);
+  const int s = (int)fmax(1.0, ceil(tabs * nu));
+  const double cr = t_re / s, ci = t_im / s;
+  double vr = lane == 0 ? 1.0 : 0.0, vi = 0.0; // e1
+  for (int sub = 0; sub < s; ++sub) {
+    double ar = vr, ai = vi, tr = vr, ti = vi;
+    for (int k = 1; k <= 40; ++k) {
+      // T * term
+      const double tr_dn = __shfl_up(tr, 1, kWave), ti_dn = __shfl_up(ti, 1, kWave);
+      const double tr_up = __shfl_down(tr, 1, kWave), ti_up = __shfl_down(ti, 1, kWave);
+      double xr = d * tr + e * (lane + 1 < m ? tr_up : 0.0);
+      double xi = d * ti + e * (lane + 1 < m ? ti_up : 0.0);
+      if (lane > 0) {
+        xr += eu * tr_dn;
+        xi += eu * ti_dn;
+      }
+      // term <- (c / k) * (T term)
+      const double inv = 1.0 / (double)k;
+      tr = in ? (cr * xr - ci * xi) * inv : 0.0;
+      ti = in ? (cr * xi + ci * xr) * inv : 0.0;
+      ar += tr;
+      ai += ti;
+      double tn = fabs(tr) + fabs(ti), an = fabs(ar) + fabs(ai);
+      for (int off = 32; off >= 1; off >>= 1) {
+        tn = fmax(tn, __shfl_xor(tn, off, kWave));
+        an = fmax(an, __shfl_xor(an, off, kWave));
+      }
+      if (tn <= 1e-17 * an) break;
+    }
+    vr = ar;
+    vi = ai;
+  }
+  yr = beta0 * vr;
+  yi = beta0 * vi;
 }
 
 } // namespace nls

@@ -74,6 +74,12 @@ def test_small_matfunc_gpu_vs_oracle(orc):
         Tc = Tcm.astype(np.complex128)
         ref = orc.small_matfunc_complex(Tc, 0, 0.005j)
         assert np.abs(y - ref).max() <= 1e-13
+        # exp(dt T) e1 by the scaled Taylor series (the fused NLSE small stage), dt up to
+        # |dt| ||T|| ~ 30 substeps; complex dt as well
+        for dt in (0.005j, 0.05j, 0.2j, 0.01 + 0.02j):
+            y = small_matfunc(d, e, 10, dt)
+            ref = orc.small_matfunc_complex(Tc, 0, dt)
+            assert np.abs(y - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), (m, dt)
 
 
 def test_matfunc_real_plain_vs_oracle(orc):
