@@ -410,3 +410,29 @@ def test_fused_long_horizon_vs_oracle(orc):
     errs = [rel_l2(u[k], ur[k]) for k in range(1, ns)]
     assert max(errs) < TOL, errs
     assert errs[-1] < 2e-6, errs
+
+
+def test_fused_reorth_auto_acts_per_trajectory():
+    """Mode 2 in a batch of several lanes: only the trajectory whose basis lost
+    orthogonality (6 x 6 grid, m = 24) switches on; the graphs are recaptured with the
+    reorthogonalisation kernels mid-run, and the other trajectories' results stay bitwise
+    those of a run without reorthogonalisation."""
+    from nls.batch import FusedBatch
+    sizes = [40, 6, 64, 33]
+    cases = [make_case(n, 8, t) for t, n in enumerate(sizes)]
+    out = {}
+    for mode in (0, "auto"):
+        b = FusedBatch(sizes, 10.0, 0.005, krylov_dim=24, mu2=1.0, cells_per_thread=8,
+                       pass_kernel=0, lanes=2, reorth=mode, snapshot_every=10, max_snapshots=5)
+        for t, cs in enumerate(cases):
+            b.set_initial(t, *cs)
+        b.run(20)
+        b.run(20)
+        out[mode] = ([b.state(t)[0] for t in range(len(sizes))],
+                     [b.monitor(t) for t in range(len(sizes))],
+                     [b.snapshots(t)[0] for t in range(len(sizes))])
+    mons = out["auto"][1]
+    assert [m["reorth_on"] for m in mons] == [False, True, False, False], mons
+    for t in (0, 2, 3):
+        assert np.array_equal(out[0][0][t], out["auto"][0][t]), t
+        assert np.array_equal(out[0][2][t], out["auto"][2][t]), t
