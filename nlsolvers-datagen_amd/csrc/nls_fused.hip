@@ -1864,8 +1864,10 @@ static std::vector<int2> xcd_cmap(const std::vector<TrajDev> &traj, const std::v
 // chunking depends only on its own size and the tile.
 // Plane march (3D): in-plane ranges of at most 256 * cpt cells (equal, multiples of 64),
 // and the planes split into equal segments of at most kPlaneSeg planes (each segment
-// re-reads the two planes beside it).  Both depend only on the trajectory's own size.
-constexpr int kPlaneSeg = 64;
+// re-reads the two planes beside it), more of them (down to kPlaneSegMin planes) when a
+// trajectory has fewer than kPlaneChunks chunks.  Both depend only on the trajectory's
+// own size.
+constexpr int kPlaneSeg = 64, kPlaneSegMin = 16, kPlaneChunks = 96;
 
 static Layout make_layout(const nls_traj_desc *td, int ntraj, int dim, int cpt, double dt,
                           double mu2, bool balanced = false, bool plane3d = false) {
@@ -1886,7 +1888,8 @@ static Layout make_layout(const nls_traj_desc *td, int ntraj, int dim, int cpt, 
       const int r = (n2 + T.npen - 1) / T.npen;
       T.prange = std::min(CH, (r + 63) / 64 * 64);
       T.npen = (n2 + T.prange - 1) / T.prange;
-      const int nseg = (T.n + kPlaneSeg - 1) / kPlaneSeg;
+      const int nseg = std::max((T.n + kPlaneSeg - 1) / kPlaneSeg,
+                                std::min((kPlaneChunks + T.npen - 1) / T.npen, T.n / kPlaneSegMin));
       T.zseg = (T.n + nseg - 1) / nseg;
     }
   }
@@ -2422,6 +2425,256 @@ __global__ __launch_bounds__(FBS) void k_nlse_march2d(NlseArgs a, int j, int ema
   }
 }
 
+// The plane march of k_kge_march3d for the complex64 NLSE sequence: real and imaginary
+// parts of v_j share the (real) Lanczos scalars, L v = s (sum_nb w v_nb - wsum v), and the
+// partials add both parts; STAGE 0 forms the seed psi1 = psi exp(i dt/2 m|psi|^2) at each
+// cell it loads and stores it (X) with W[0] = L psi1, STAGE 1 forms v_1 from W[0] and X,
+// STAGE 2 is the general pass.  Same window, registers and cross-plane couplings as the
+// Klein-Gordon kernel.
+template <int STAGE, int CPT>
+__global__ __launch_bounds__(FBS) void k_nlse_march3d(NlseArgs a, int j, int emax) {
+  extern __shared__ float lds[];
+  const int2 cm = a.cmap[blockIdx.x];
+  if (cm.x < 0) return;
+  const TrajDev T = a.traj[cm.x];
+  __shared__ double red[PN];
+  __shared__ double shn[PN * (FBS / kWave)];
+  float cA = 0.f, cB = 0.f, cC = 0.f;
+  if (STAGE >= 1) {
+    reduce_partials3(a, T, (j - 1) & 1, red);
+    const double q0 = red[0], q1 = red[1], q2 = red[2];
+    if (STAGE == 1) {
+      const double b0 = sqrt(q0);
+      const double al = q0 > 0.0 ? q1 / q0 : 0.0;
+      const double b1sq = q0 > 0.0 ? (q2 - q1 * q1 / q0) / q0 : 0.0;
+      const double b1 = (b1sq > 1e-14 * (q0 > 0.0 ? q2 / q0 : 0.0) && b1sq > 0.0) ? sqrt(b1sq) : 0.0;
+      cA = (float)al;
+      cB = (b0 > 0.0 && b1 > 0.0) ? (float)(1.0 / (b0 * b1)) : 0.0f;
+      cC = b0 > 0.0 ? (float)(b1 / b0) : 0.0f;
+      if (cm.y == 0 && threadIdx.x == 0) {
+        a.scal[cm.x].b0[0] = b0;
+        a.scal[cm.x].alpha[0][0] = al;
+        a.scal[cm.x].beta[0][1] = b1;
+      }
+    } else {
+      double al, b;
+      lanczos_scalars(q0, q1, q2, al, b);
+      cA = (float)al;
+      cB = b > 0.0 ? (float)(1.0 / b) : 0.0f;
+      cC = (float)b;
+      if (cm.y == 0 && threadIdx.x == 0) {
+        a.scal[cm.x].alpha[0][j - 1] = al;
+        a.scal[cm.x].beta[0][j] = b;
+      }
+    }
+  }
+  const int n = T.n, n2 = n * n, N = (int)T.N;
+  const int64_t cells = a.cells;
+  float *sc = lds, *sv0 = lds + emax, *sv1 = lds + 2 * emax;
+  const c32 *psiT = a.psi + T.off;
+  const float *cT = a.cf + T.off, *mT = a.mf + T.off;
+  const c32 *Xin = a.W + (STAGE == 1 ? 0 : (size_t)((j - 1) & 1) * cells) + T.off; // z or w'
+  const c32 *Pin = (STAGE == 1 ? a.X : a.V + (STAGE >= 2 ? (size_t)(j - 2) * cells : 0)) + T.off;
+  c32 *Vout = a.V + (STAGE >= 1 ? (size_t)(j - 1) * cells : 0) + T.off;
+  c32 *Wout = a.W + (size_t)(j & 1) * cells + T.off;
+  c32 *Xout = a.X + T.off;
+  const float scl = T.s;
+  const int q6 = a.q6;
+  double acc[PN] = {0.0, 0.0, 0.0};
+  auto store2 = [](c32 *p, float re, float im) {
+    __builtin_nontemporal_store(f2v{re, im}, reinterpret_cast<f2v *>(p));
+  };
+
+  const int pen = cm.y % T.npen, zs = cm.y / T.npen;
+  const int pa = pen * T.prange;
+  const int pb = pa + T.prange < n2 ? pa + T.prange : n2;
+  const int z0 = zs * T.zseg;
+  const int z1 = z0 + T.zseg < n ? z0 + T.zseg : n;
+  const int S = pb - pa;
+
+  struct Raw {
+    float c;
+    c32 x, p; // STAGE 0: x = psi, p.re = m
+  };
+  auto fetch = [&](int g, Raw &r) {
+    r.c = cT[g];
+    if (STAGE == 0) {
+      r.x = psiT[g];
+      r.p.re = mT[g];
+    } else {
+      r.x = Xin[g];
+      r.p = Pin[g];
+    }
+  };
+  // v_j (re, im) and the previous vector P from the raw inputs
+  auto form = [&](const Raw &r, float &v0, float &v1, float &q0, float &q1) {
+    if (STAGE == 0) {
+      const c32 v = nl_half_phase(r.x, r.p.re, a.dt);
+      v0 = v.re;
+      v1 = v.im;
+      q0 = q1 = 0.0f;
+    } else {
+      v0 = cB * (r.x.re - cA * r.p.re);
+      v1 = cB * (r.x.im - cA * r.p.im);
+      q0 = r.p.re;
+      q1 = r.p.im;
+    }
+  };
+
+  int po[CPT];
+  bool own[CPT];
+#pragma unroll
+  for (int q = 0; q < CPT; ++q) {
+    po[q] = pa + threadIdx.x + q * FBS;
+    own[q] = threadIdx.x + q * FBS < S;
+  }
+  float vm0[CPT], vm1[CPT], vc0[CPT], vc1[CPT], vn0[CPT], vn1[CPT];
+  float cmz[CPT], ccz[CPT], cnz[CPT], pc0[CPT], pc1[CPT], pn0[CPT], pn1[CPT];
+  Raw ro[CPT];
+  Raw rh[M3_HPT];
+  auto hwin = [&](int h) { return h < n ? h : S + h; };
+  auto hcell = [&](int h, int z) { return z * n2 + pa - n + hwin(h); };
+  auto fetch_own = [&](int z) {
+#pragma unroll
+    for (int q = 0; q < CPT; ++q)
+      if (own[q]) fetch(z * n2 + po[q], ro[q]);
+  };
+  auto fetch_halo = [&](int z) {
+#pragma unroll
+    for (int k = 0; k < M3_HPT; ++k) {
+      const int h = threadIdx.x + k * FBS;
+      if (h < 2 * n) {
+        const int g = hcell(h, z);
+        if (g >= 0 && g < N) fetch(g, rh[k]);
+      }
+    }
+  };
+
+  if (z0 < z1) {
+    if (z0 > 0) {
+      fetch_own(z0 - 1);
+#pragma unroll
+      for (int q = 0; q < CPT; ++q)
+        if (own[q]) {
+          float t0, t1;
+          form(ro[q], vm0[q], vm1[q], t0, t1);
+          cmz[q] = ro[q].c;
+        }
+    }
+    fetch_own(z0);
+#pragma unroll
+    for (int q = 0; q < CPT; ++q)
+      if (own[q]) {
+        form(ro[q], vc0[q], vc1[q], pc0[q], pc1[q]);
+        ccz[q] = ro[q].c;
+      }
+    if (z0 + 1 < n) fetch_own(z0 + 1);
+    fetch_halo(z0);
+  }
+  for (int z = z0; z < z1; ++z) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < CPT; ++q)
+      if (own[q]) {
+        const int e = po[q] - pa + n;
+        sc[e] = ccz[q];
+        sv0[e] = vc0[q];
+        sv1[e] = vc1[q];
+      }
+#pragma unroll
+    for (int k = 0; k < M3_HPT; ++k) {
+      const int h = threadIdx.x + k * FBS;
+      if (h < 2 * n) {
+        const int g = hcell(h, z);
+        if (g >= 0 && g < N) {
+          const int e = hwin(h);
+          float t0, t1;
+          sc[e] = rh[k].c;
+          form(rh[k], sv0[e], sv1[e], t0, t1);
+        }
+      }
+    }
+    const bool up = z + 1 < n;
+    if (up) {
+#pragma unroll
+      for (int q = 0; q < CPT; ++q)
+        if (own[q]) {
+          form(ro[q], vn0[q], vn1[q], pn0[q], pn1[q]);
+          cnz[q] = ro[q].c;
+        }
+    }
+    if (z + 1 < z1) {
+      if (z + 2 < n) fetch_own(z + 2);
+      fetch_halo(z + 1);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+      if (!own[q]) continue;
+      const int p = po[q], e = p - pa + n, g = z * n2 + p;
+      const int col = p % n;
+      const bool hzm = z > 0, hzp = up;
+      const bool hym = q6 ? g >= n : p >= n;
+      const bool hyp = q6 ? g + n < N : p < n2 - n;
+      const bool hxm = col > 0, hxp = col < n - 1;
+      const float ci = ccz[q];
+      const float wzm = hzm ? 0.5f * (ci + cmz[q]) : 0.0f;
+      const float wym = hym ? 0.5f * (ci + sc[e - n]) : 0.0f;
+      const float wxm = hxm ? 0.5f * (ci + sc[e - 1]) : 0.0f;
+      const float wxp = hxp ? 0.5f * (ci + sc[e + 1]) : 0.0f;
+      const float wyp = hyp ? 0.5f * (ci + sc[e + n]) : 0.0f;
+      const float wzp = hzp ? 0.5f * (ci + cnz[q]) : 0.0f;
+      const float wsum = ((((wzm + wym) + wxm) + wxp) + wyp) + wzp;
+      float lv[2];
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const float *sv = s == 0 ? sv0 : sv1;
+        const float vi = s == 0 ? vc0[q] : vc1[q];
+        float nb = 0.0f;
+        if (hzm) nb += wzm * (s == 0 ? vm0[q] : vm1[q]);
+        if (hym) nb += wym * sv[e - n];
+        if (hxm) nb += wxm * sv[e - 1];
+        if (hxp) nb += wxp * sv[e + 1];
+        if (hyp) nb += wyp * sv[e + n];
+        if (hzp) nb += wzp * (s == 0 ? vn0[q] : vn1[q]);
+        lv[s] = scl * (nb - wsum * vi);
+      }
+      const float vr = vc0[q], vi = vc1[q];
+      if (STAGE == 0) {
+        store2(Xout + g, vr, vi);
+        store2(Wout + g, lv[0], lv[1]);
+        acc[0] += (double)vr * vr + (double)vi * vi;
+        acc[1] += (double)vr * lv[0] + (double)vi * lv[1];
+        acc[2] += (double)lv[0] * lv[0] + (double)lv[1] * lv[1];
+      } else {
+        const float wr = lv[0] - cC * pc0[q], wi = lv[1] - cC * pc1[q];
+        store2(Vout + g, vr, vi);
+        store2(Wout + g, wr, wi);
+        acc[0] += (double)wr * vr + (double)wi * vi;
+        acc[1] += (double)wr * wr + (double)wi * wi;
+        acc[2] += (double)vr * vr + (double)vi * vi;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+      vm0[q] = vc0[q];
+      vm1[q] = vc1[q];
+      cmz[q] = ccz[q];
+      vc0[q] = vn0[q];
+      vc1[q] = vn1[q];
+      ccz[q] = cnz[q];
+      pc0[q] = pn0[q];
+      pc1[q] = pn1[q];
+    }
+  }
+  block_sum_n<PN>(acc, shn);
+  if (threadIdx.x == 0) {
+    double *dst = a.part + ((size_t)(j & 1) * a.total_chunks + T.chunk0 + cm.y) * PN;
+#pragma unroll
+    for (int k = 0; k < PN; ++k) dst[k] = acc[k];
+  }
+}
+
 // Combine over the workgroup's range (any layout): psi2 = sum_k y_k v_k, the second half
 // phase, the pre-copy snapshot and the ghost copy.
 template <int DIM, bool ORTH>
@@ -2732,8 +2985,19 @@ struct nls_batch {
     a.tlist = tlist;
     const dim3 grid(ngrid), blk(FBS);
     const bool march = pass_kind == 2 && DIM == 2;
+    const bool plane = pass_kind == 4 && DIM == 3 && (CPT == 4 || CPT == 8);
     const size_t lds = march2d_lds_bytes(nmax);
     const int emax = MSUB + 2 * nmax;
+    const size_t lds3 = march3d_lds_bytes(CPT, nmax);
+    const int emax3 = FBS * CPT + 2 * nmax;
+    // pass j of the 3D plane march (tiles 4 and 8 only)
+    auto plane_pass = [&](int stage, int j) {
+      if constexpr (CPT == 4 || CPT == 8) {
+        if (stage == 0) hipLaunchKernelGGL((k_nlse_march3d<0, CPT>), grid, blk, lds3, s, a, j, emax3);
+        else if (stage == 1) hipLaunchKernelGGL((k_nlse_march3d<1, CPT>), grid, blk, lds3, s, a, j, emax3);
+        else hipLaunchKernelGGL((k_nlse_march3d<2, CPT>), grid, blk, lds3, s, a, j, emax3);
+      }
+    };
     auto reorth = [&](int j) {
       if (!reorth_live) return;
       hipLaunchKernelGGL(k_nlse_reorth_dots, dim3(ngrid, (j + 3) / 4), blk, 0, s, a, j);
@@ -2742,6 +3006,9 @@ struct nls_batch {
     if (march) {
       hipLaunchKernelGGL(k_nlse_march2d<0>, grid, blk, lds, s, a, 0, emax, 1);
       hipLaunchKernelGGL(k_nlse_march2d<1>, grid, blk, lds, s, a, 1, emax, 1);
+    } else if (plane) {
+      plane_pass(0, 0);
+      plane_pass(1, 1);
     } else {
       hipLaunchKernelGGL((k_nlse_pass<DIM, 0, CPT>), grid, blk, 0, s, a, 0);
       hipLaunchKernelGGL((k_nlse_pass<DIM, 1, CPT>), grid, blk, 0, s, a, 1);
@@ -2750,6 +3017,7 @@ struct nls_batch {
     for (int j = 2; j <= m - 2; ++j) {
       if (prof) NLS_HIP(hipEventRecord(next_event(), s));
       if (march) hipLaunchKernelGGL(k_nlse_march2d<2>, grid, blk, lds, s, a, j, emax, 1);
+      else if (plane) plane_pass(2, j);
       else hipLaunchKernelGGL((k_nlse_pass<DIM, 2, CPT>), grid, blk, 0, s, a, j);
       if (prof) NLS_HIP(hipEventRecord(next_event(), s));
       reorth(j);
@@ -2757,7 +3025,7 @@ struct nls_batch {
     hipLaunchKernelGGL(k_nlse_small, dim3(nt), dim3(kWave), 0, s, a, m - 2);
     SnapC sn{nullptr};
     if (snapshot) sn.psi = d_snapU + (size_t)slot * lay.cells * 2;
-    if (march) {
+    if (march || plane) {
       if (p.reorth) hipLaunchKernelGGL((k_nlse_combine_rt<DIM, true>), grid, blk, 0, s, a, sn);
       else hipLaunchKernelGGL((k_nlse_combine_rt<DIM, false>), grid, blk, 0, s, a, sn);
     } else {
@@ -2977,10 +3245,11 @@ struct nls_batch {
   // timing runs leave the trajectory untouched) and keeps the fastest.
   bool kind_ok(int kind, int c) const {
     const int nm = nmax_of();
-    if (nlse()) // k_nlse_pass tiles, or the NLSE row march (2D)
+    if (nlse()) // k_nlse_pass tiles, the NLSE row march (2D) or plane march (3D)
       return (kind == 0 && (c == 4 || c == 8 || c == 16)) ||
              (kind == 2 && p.dim == 2 && 2 * nm <= MSUB && march2d_lds_bytes(nm) <= 65536 &&
-              c >= 4 && c <= 64);
+              c >= 4 && c <= 64) ||
+             (kind == 4 && p.dim == 3 && (c == 4 || c == 8) && 2 * nm <= FBS * M3_HPT);
     // row-march kernels take any range length (tile = target cells per workgroup / 256)
     if (kind == 3) return p.reorth == 0 && p.dim == 2 && 3 * nm <= RKS * FBS && 2 * nm <= MSUB &&
                           rmarch2d_lds_bytes(nm) <= 65536 && c >= 4 && c <= 64;

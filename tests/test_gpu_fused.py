@@ -86,8 +86,8 @@ _ORACLE_3D = {}
 
 
 # direct 7-point stencil, plane march with 1024- and 2048-cell in-plane ranges; the
-# sizes give one range (20^3), ranges that start mid-row (50^3: 896 cells) and two
-# plane segments (70^3: planes 0-34, 35-69)
+# sizes give one range (20^3), ranges that start mid-row (50^3: 896 cells) and several
+# plane segments (50^3: 3 of 17 planes; 70^3: 4 of 18)
 @pytest.mark.parametrize("kernel,cpt", [(0, 8), (4, 4), (4, 8)])
 @pytest.mark.parametrize("q6", [True, False])
 def test_fused_kge_3d_uneven_vs_oracle(orc, kernel, cpt, q6):
@@ -436,3 +436,40 @@ def test_fused_reorth_auto_acts_per_trajectory():
     for t in (0, 2, 3):
         assert np.array_equal(out[0][0][t], out["auto"][0][t]), t
         assert np.array_equal(out[0][2][t], out["auto"][2][t]), t
+
+
+def nlse_case_3d(n, L, t):
+    x = np.linspace(-L, L, n)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    rng = np.random.default_rng(40 + t)
+    cx, cy, cz = rng.uniform(-2, 2, 3)
+    psi = np.exp(-((X - cx) ** 2 + (Y - cy) ** 2 + (Z - cz) ** 2) / 4) * np.exp(1j * (0.7 * X - 0.3 * Z))
+    m = 0.8 + 0.4 * rng.random((n, n, n))
+    c = 0.8 + 0.4 * rng.random((n, n, n))
+    return m, c, psi
+
+
+# NLSE in 3D: direct stencil and plane march (one range at 20^3, ranges starting mid-row
+# and 4 plane segments at 70^3), both operator variants
+@pytest.mark.parametrize("kernel,cpt", [(0, 8), (4, 4)])
+@pytest.mark.parametrize("q6", [True, False])
+def test_fused_nlse_3d_vs_oracle(orc, kernel, cpt, q6):
+    from nls.batch import FusedBatch
+    ns_list = [20, 70]
+    L, dt, nt, ns = 8.0, 0.002, 21, 2
+    b = FusedBatch(ns_list, L, dt, equation="nlse", method="strang", dim=3, krylov_dim=12,
+                   q6=q6, snapshot_every=nt // ns, max_snapshots=ns, cells_per_thread=cpt,
+                   pass_kernel=kernel)
+    cases = [nlse_case_3d(n, L, t) for t, n in enumerate(ns_list)]
+    for t, (m, c, psi) in enumerate(cases):
+        b.set_initial(t, m, c, psi)
+    b.run(nt - 1)
+    assert b.tile()["pass_kind"] == kernel
+    for t, n in enumerate(ns_list):
+        key = ("nlse", n, q6)
+        if key not in _ORACLE_3D:  # independent of kernel and tile
+            m, c, psi = cases[t]
+            _ORACLE_3D[key] = orc.nlse_run(n, L, dt * nt, nt, ns, m, c, psi, krylov=12, dim=3,
+                                           q6=q6)
+        out, _ = b.snapshots(t)
+        assert rel_l2(out[1], _ORACLE_3D[key][1]) < 1e-4, (t, n)
