@@ -105,22 +105,20 @@ def downsample_interpolation(u, target_shape, Lx, Ly):
     nt, nx, ny = u.shape
     tx, ty = target_shape
 
-    def axis(n, t, L):
+    def weights(n, t, L):
+        # linear interpolation is separable: target point k takes (1 - w) of node i and
+        # w of node i + 1, as one row of a (t, n) matrix with two non-zeros
         x = np.linspace(-L, L, n)
         xn = np.linspace(-L, L, t)
         i = np.clip(np.searchsorted(x, xn, side="right") - 1, 0, n - 2)
         w = (xn - x[i]) / (x[i + 1] - x[i])
-        return i, w
+        W = np.zeros((t, n))
+        W[np.arange(t), i] = 1 - w
+        W[np.arange(t), i + 1] += w
+        return W
 
-    ix, wx = axis(nx, tx, Lx)
-    iy, wy = axis(ny, ty, Ly)
-    a = u[:, ix][:, :, iy]
-    b = u[:, ix + 1][:, :, iy]
-    c = u[:, ix][:, :, iy + 1]
-    d = u[:, ix + 1][:, :, iy + 1]
-    WX, WY = wx[:, None], wy[None, :]
-    out = (a * (1 - WX) * (1 - WY) + b * WX * (1 - WY) + c * (1 - WX) * WY + d * WX * WY)
-    return out.astype(u.dtype)
+    Wx, Wy = weights(nx, tx, Lx), weights(ny, ty, Ly)
+    return np.matmul(np.matmul(Wx, u), Wy.T).astype(u.dtype)
 
 
 def downsample_fft(u, target_shape):
@@ -203,27 +201,46 @@ class BatchLauncher:
         if self.kind != "nlse":
             kw.update(v0=v0, v=downsample(v, cfg))
         writer.submit(self.path(idx), **kw)
-        self.files.append(self.path(idx))
 
     def run(self, log=print):
+        """Runs every group.  Host work overlaps the GPU: the next group is sampled and
+        the previous one downsampled and handed to the HDF5 writer on worker threads
+        while the current group runs (the library releases the interpreter lock inside
+        its calls)."""
+        from concurrent.futures import ThreadPoolExecutor
+
         from .h5 import AsyncWriter
         cfg = self.cfg
         t_all = time.perf_counter()
         cells = 0
-        with AsyncWriter() as writer:
-            for g0 in range(0, cfg.num_runs, cfg.batch):
-                idxs = list(range(g0, min(cfg.num_runs, g0 + cfg.batch)))
-                samples = [self.sample(i) for i in idxs]
+        n_cells = cfg.nx * cfg.ny * (cfg.nz if cfg.dim == 3 else 1)
+        groups = [list(range(g0, min(cfg.num_runs, g0 + cfg.batch)))
+                  for g0 in range(0, cfg.num_runs, cfg.batch)]
+        self.files = [self.path(i) for g in groups for i in g]
+
+        def write_group(writer, idxs, samples, out, wall):
+            for i, s, (u, v) in zip(idxs, samples, out):
+                self._write(writer, i, s, u, v, wall / len(idxs))
+
+        with AsyncWriter() as writer, ThreadPoolExecutor(max_workers=2) as pool:
+            nxt = pool.submit(lambda g: [self.sample(i) for i in g], groups[0]) if groups else None
+            pending = None
+            for gi, idxs in enumerate(groups):
+                samples = nxt.result()
+                if gi + 1 < len(groups):
+                    nxt = pool.submit(lambda g: [self.sample(i) for i in g], groups[gi + 1])
                 self._first = idxs[0]
                 t0 = time.perf_counter()
                 out = self._run_group(samples)
                 wall = time.perf_counter() - t0
-                n_cells = cfg.nx * cfg.ny * (cfg.nz if cfg.dim == 3 else 1)
                 cells += len(idxs) * n_cells * (cfg.nt - 1)
-                for i, s, (u, v) in zip(idxs, samples, out):
-                    self._write(writer, i, s, u, v, wall / len(idxs))
+                if pending is not None:
+                    pending.result()
+                pending = pool.submit(write_group, writer, idxs, samples, out, wall)
                 log(f"[launcher] runs {idxs[0]}..{idxs[-1]}: {wall:.2f} s "
                     f"({len(idxs) * n_cells * (cfg.nt - 1) / wall:.3e} cell-steps/s)")
+            if pending is not None:
+                pending.result()
             written = writer.drain()
         total = time.perf_counter() - t_all
         return {"files": written, "wall_s": total, "cell_steps": cells, "run_id": self.run_id}
@@ -248,12 +265,18 @@ class BatchLauncher:
             return out
         from .batch import FusedBatch
         method = cfg.integrator or ("strang" if self.kind == "nlse" else "gautschi")
+        tile = getattr(self, "_tile", None)  # the first group's autotuned choice, reused
         b = FusedBatch([n] * len(samples), cfg.Lx, dt, equation=self.kind, method=method,
-                       dim=cfg.dim, krylov_dim=cfg.krylov_dim, mu2=cfg.mu2,
-                       snapshot_every=every, max_snapshots=cfg.num_snapshots)
+                       dim=cfg.dim, krylov_dim=getattr(self, "_m", cfg.krylov_dim), mu2=cfg.mu2,
+                       snapshot_every=every, max_snapshots=cfg.num_snapshots,
+                       cells_per_thread=tile["cells_per_thread"] if tile else 0,
+                       pass_kernel=tile["pass_kind"] if tile else None,
+                       lanes=tile["lanes"] if tile else 0)
         for k, s in enumerate(samples):
             b.set_initial(k, s[2], s[3], s[0], s[1])
         b.run(cfg.nt - 1)  # reference cadence: nt - 1 steps (kg_driver_dev_2d.cpp:144)
+        if tile is None and len(samples) == cfg.batch:
+            self._tile, self._m = b.tile(), b.info()["krylov_dim"]
         out = [b.snapshots(k) for k in range(len(samples))]
         b.close()
         return out
