@@ -9,8 +9,9 @@
 // emitted directly in ascending column order (Eigen's compressed layout; the
 // matrix is symmetric, so the drivers pass CSC as CSR, kg_driver_dev_2d.cpp:132-137).
 // negate flips the sign (Gautschi drivers build -L, kg_driver_dev_2d.cpp:105-108);
-// shift is then added to the diagonal (the spectral shift mu2 of the north-star
-// Klein-Gordon equations, no reference counterpart).
+// shift is then added to the diagonal (the spectral shift mu2 of the Klein-Gordon
+// equations u_tt = Lu - mu2 u - m u^3 of the benchmark workloads, no reference
+// counterpart).
 // q6 keeps the reference's 3D "y" couplings across plane boundaries
 // (laplacians.hpp:192-198); q6 = 0 gives the plane-respecting 7-point operator.
 #include "nls_common.hpp"

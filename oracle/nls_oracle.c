@@ -20,8 +20,8 @@
  *   Q9  u_{-1} = u0 - dt*v0, v = (u+ - u)/dt                 (device/include/kg_dev.hpp:55-58)
  *   Q14 zero input vector: the reference divides by 0; here the result is 0 (guard)
  *
- * Build-defined extensions (no reference counterpart, used by the north-star
- * configurations): a spectral shift mu2 (operator A = -L + mu2*I), the sEWI
+ * Extensions with no reference counterpart, used by the benchmark workloads:
+ * a spectral shift mu2 (operator A = -L + mu2*I), the sEWI
  * filter sinc^2(t*sqrt(l)/2), a Q6-free 3D operator, Philox4x32-10 noise with a
  * fixed-operation Box-Muller transform, and a damped stochastic phi^4 stepper.
  *
