@@ -102,21 +102,43 @@ def plain_run(args, inputs):
 
 
 def parity_check(args, inputs, krylov):
-    """rel-L2 of a fused 300^2 trajectory vs the FP64 oracle at step S: both take
-    the reference cadence with nt = 2S, num_snapshots = 2 (snapshot 1 = step S)."""
+    """A fused 300^2 trajectory vs the FP64 oracle over S steps with a snapshot every
+    S/10 steps (the reference cadence with nt = 11 S/10, num_snapshots = 11, so snapshot
+    k is step k S/10): rel-L2 at the last snapshot and the largest over the snapshots
+    (compare_utils_real_2d.py:661-665), and the relative drift of the discrete energy
+    E = 1/2 |v|^2 + 1/2 u.A u + 1/4 sum m u^4 (A = -L_c + mu2, compare_utils_real_2d.py:
+    112-131 plus the mass term) of both runs, largest over the snapshots."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     from nls.batch import FusedBatch
-    n, L, S = args.n, args.L, args.parity_steps
+    n, L = args.n, args.L
+    every = max(1, args.parity_steps // 10)
+    S, ns = 10 * every, 11
     m, c, u0, v0 = inputs[0]
-    b = FusedBatch([n], L, args.dt, krylov_dim=krylov, mu2=1.0, snapshot_every=S,
-                   max_snapshots=2, cells_per_thread=8)
+    b = FusedBatch([n], L, args.dt, krylov_dim=krylov, mu2=1.0, snapshot_every=every,
+                   max_snapshots=ns, cells_per_thread=8)
     b.set_initial(0, m, c, u0, v0)
     b.run(S)
-    u, _ = b.snapshots(0)
-    ur, _ = orc.kge_run(n, L, args.dt * 2 * S, 2 * S, 2, m, c, u0, v0, krylov=krylov, mu2=1.0)
-    err = float(np.linalg.norm(u[1] - ur[1]) / np.linalg.norm(ur[1]))
-    return err, S
+    u, v = b.snapshots(0)
+    ur, vr = orc.kge_run(n, L, args.dt * ns * every, ns * every, ns, m, c, u0, v0,
+                         krylov=krylov, mu2=1.0)
+    errs = [float(np.linalg.norm(u[k] - ur[k]) / np.linalg.norm(ur[k])) for k in range(1, ns)]
+    dx = 2 * L / (n - 1)
+    op = orc.laplacian_2d(n, dx, dx, np.ascontiguousarray(c, dtype=np.float64).ravel(),
+                          negate=True, shift=1.0)
+    mf = np.asarray(m, dtype=np.float64).ravel()
+
+    def energy(uu, vv):
+        uu, vv = uu.ravel(), vv.ravel()
+        return 0.5 * vv @ vv + 0.5 * uu @ op.matvec(uu) + 0.25 * np.sum(mf * uu ** 4)
+
+    def drift(us, vs):
+        # snapshot 0 holds v0, not the first-order velocity the later snapshots hold
+        E = [energy(us[k], vs[k]) for k in range(1, ns)]
+        return float(max(abs(e - E[0]) for e in E) / abs(E[0]))
+    return {"rel_l2": errs[-1], "rel_l2_max_over_snapshots": max(errs), "steps": S,
+            "snapshots": ns, "energy_drift_fused": drift(u, v),
+            "energy_drift_oracle": drift(ur, vr)}
 
 
 def cpu_baseline(args, inputs, krylov):
@@ -522,11 +544,14 @@ def main():
         log("[bench] plain FP64 CSR drop-in")
         plain_rate, plain_wall, plain_steps = plain_run(args, inputs)
         log(f"[bench] plain: {plain_rate:.3e} cell-steps/s")
-    err = None
+    err, par = None, None
     if not args.skip_parity:
         log("[bench] parity vs FP64 oracle")
-        err, psteps = parity_check(args, inputs, info["krylov_dim"])
-        log(f"[bench] rel-L2 after {psteps} steps: {err:.2e}")
+        par = parity_check(args, inputs, info["krylov_dim"])
+        err = par["rel_l2"]
+        log(f"[bench] rel-L2 after {par['steps']} steps: {err:.2e} (max over snapshots "
+            f"{par['rel_l2_max_over_snapshots']:.2e}); energy drift fused "
+            f"{par['energy_drift_fused']:.3e}, oracle {par['energy_drift_oracle']:.3e}")
     serve = None
     if not args.skip_serve:
         log(f"[bench] serving: {args.serve_requests} phi^4 requests at {args.serve_rate:g}/s")
@@ -591,6 +616,7 @@ def main():
                           "at a time", "steps": plain_steps},
         "speedup_vs_plain": (value / plain_rate) if plain_rate else None,
         "rel_l2_vs_fp64_oracle": err,
+        "parity": par,
         "krylov_monitor": info["krylov_monitor"],
         "latency": serve,
         "cpu_baseline": cpu,

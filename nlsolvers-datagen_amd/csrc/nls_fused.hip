@@ -1533,6 +1533,7 @@ __global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a) {
   const double t2 = T.tau * T.tau;
   const float *Wl0 = a.W + ((size_t)0 * 2 + ((m - 2) & 1)) * cells;
   const float *Wl1 = a.W + ((size_t)1 * 2 + ((m - 2) & 1)) * cells;
+  // every stream except u is read once per step: non-temporal loads (u is read twice)
   for (int64_t gb = cbeg; gb < cend; gb += G * FBS) {
     int64_t gs[G];
     bool ok[G];
@@ -1544,9 +1545,9 @@ __global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a) {
       ok[kk] = i < cend;
       gs[kk] = T.off + (ok[kk] ? clamp_src<DIM>(i, n) : 0);
       const double uu = a.u[gs[kk]];
-      const double g1 = -(double)a.mf[gs[kk]] * uu * uu * uu;
-      d0[kk] = cv[0][0] * (double)(float)uu + cw[0] * (double)Wl0[gs[kk]];
-      d1[kk] = cv[1][0] * (double)(float)g1 + cw[1] * (double)Wl1[gs[kk]];
+      const double g1 = -(double)__builtin_nontemporal_load(a.mf + gs[kk]) * uu * uu * uu;
+      d0[kk] = cv[0][0] * (double)(float)uu + cw[0] * (double)__builtin_nontemporal_load(Wl0 + gs[kk]);
+      d1[kk] = cv[1][0] * (double)(float)g1 + cw[1] * (double)__builtin_nontemporal_load(Wl1 + gs[kk]);
       xs0[kk] = (float)uu;
       xs1[kk] = (float)g1;
     }
@@ -1558,8 +1559,8 @@ __global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a) {
       const double y0 = cv[0][k], y1 = cv[1][k];
 #pragma unroll
       for (int kk = 0; kk < G; ++kk) {
-        d0[kk] += y0 * (double)Vk0[gs[kk]];
-        d1[kk] += y1 * (double)Vk1[gs[kk]];
+        d0[kk] += y0 * (double)__builtin_nontemporal_load(Vk0 + gs[kk]);
+        d1[kk] += y1 * (double)__builtin_nontemporal_load(Vk1 + gs[kk]);
       }
     }
     if (orth) {
@@ -1568,7 +1569,8 @@ __global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a) {
       const double y0 = cv[0][m - 2], y1 = cv[1][m - 2];
 #pragma unroll
       for (int kk = 0; kk < G; ++kk) {
-        const float l0 = VL0[gs[kk]], l1 = VL1[gs[kk]];
+        const float l0 = __builtin_nontemporal_load(VL0 + gs[kk]),
+                    l1 = __builtin_nontemporal_load(VL1 + gs[kk]);
         d0[kk] += y0 * (double)l0;
         d1[kk] += y1 * (double)l1;
         const int64_t gi = T.off + gb + threadIdx.x + (int64_t)kk * FBS;
@@ -1586,7 +1588,8 @@ __global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a) {
     for (int kk = 0; kk < G; ++kk) {
       const int64_t i = gb + threadIdx.x + (int64_t)kk * FBS;
       if (ok[kk])
-        __builtin_nontemporal_store(2.0 * a.u[gs[kk]] - a.up[gs[kk]] + 2.0 * d0[kk] + t2 * d1[kk],
+        __builtin_nontemporal_store(2.0 * a.u[gs[kk]] - __builtin_nontemporal_load(a.up + gs[kk]) +
+                                        2.0 * d0[kk] + t2 * d1[kk],
                                     &a.un[T.off + i]);
     }
   }

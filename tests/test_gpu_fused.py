@@ -412,6 +412,44 @@ def test_fused_long_horizon_vs_oracle(orc):
     assert errs[-1] < 2e-6, errs
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_fused_kge_energy_tracks_oracle(orc, dim):
+    """Discrete energy E = 1/2 |v|^2 + 1/2 u.A u + 1/4 sum m u^4 (A = -L_c + mu2,
+    compare_utils_real_2d.py:112-131 plus the mass term) of the fused run against the
+    FP64 oracle's at every snapshot: both carry the same O(tau) oscillation of the
+    first-order velocity, so their difference is the fused path's own error: 1.4e-8 (2D)
+    and 7e-9 (3D) of E when recorded, bar 1e-6.  The fused drift itself (about 6e-3 here,
+    the O(tau) oscillation) must equal the oracle's to the same bar."""
+    from nls.batch import FusedBatch
+    if dim == 2:
+        n, L, dt, nt, ns, krylov = 96, 10.0, 0.005, 801, 9, 8
+        m, c, u0, v0 = make_case(n, 2, 1, ring=True)
+        op = orc.laplacian_2d(n, 2 * L / (n - 1), 2 * L / (n - 1), c.ravel(), negate=True,
+                              shift=1.0)
+    else:
+        n, L, dt, nt, ns, krylov = 24, 8.0, 0.004, 401, 5, 12
+        m, c, u0, v0 = make_case_3d(n, 3, 0)
+        op = orc.laplacian_3d(n, 2 * L / (n - 1), c.ravel(), negate=True, shift=1.0)
+    b = FusedBatch([n], L, dt, dim=dim, krylov_dim=krylov, mu2=1.0, snapshot_every=nt // ns,
+                   max_snapshots=ns, cells_per_thread=8)
+    b.set_initial(0, m, c, u0, v0)
+    b.run(nt - 1)
+    u, v = b.snapshots(0)
+    ur, vr = orc.kge_run(n, L, dt * nt, nt, ns, m, c, u0, v0, krylov=krylov, mu2=1.0, dim=dim)
+
+    def energy(uu, vv):
+        uu, vv = uu.ravel(), vv.ravel()
+        return 0.5 * vv @ vv + 0.5 * uu @ op.matvec(uu) + 0.25 * np.sum(m.ravel() * uu ** 4)
+    Ef = np.array([energy(u[k], v[k]) for k in range(1, ns)])
+    Eo = np.array([energy(ur[k], vr[k]) for k in range(1, ns)])
+    gap = np.abs(Ef - Eo).max() / abs(Eo[0])
+    drift_f = np.abs(Ef - Ef[0]).max() / abs(Ef[0])
+    drift_o = np.abs(Eo - Eo[0]).max() / abs(Eo[0])
+    print(f"dim {dim}: energy gap {gap:.2e}, drift fused {drift_f:.3e}, oracle {drift_o:.3e}")
+    assert gap < 1e-6, (gap, drift_f, drift_o)
+    assert abs(drift_f - drift_o) < 1e-6, (drift_f, drift_o)
+
+
 def test_fused_reorth_auto_acts_per_trajectory():
     """Mode 2 in a batch of several lanes: only the trajectory whose basis lost
     orthogonality (6 x 6 grid, m = 24) switches on; the graphs are recaptured with the
