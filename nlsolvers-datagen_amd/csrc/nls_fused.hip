@@ -2151,6 +2151,13 @@ __global__ __launch_bounds__(kWave) void k_nlse_small(NlseArgs a, int last_pass)
   monitor_tail(S, 0, m, yr, yi, 0.0);
 }
 
+// Non-temporal load of a complex64 value (the combine reads each basis vector once).
+__device__ inline c32 ld_stream(const c32 *p) {
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  const f2 v = __builtin_nontemporal_load(reinterpret_cast<const f2 *>(p));
+  return c32{v.x, v.y};
+}
+
 __device__ inline void nlse_psi3(const NlseArgs &a, const TrajScal &S, const double *yr,
                                  const double *yi, int64_t g, double &ore, double &oim) {
   const int m = a.m;
@@ -2158,27 +2165,27 @@ __device__ inline void nlse_psi3(const NlseArgs &a, const TrajScal &S, const dou
   const double b0 = S.b0[0];
   double re = 0.0, im = 0.0;
   if (b0 > 0.0) {
-    const c32 x = a.X[g];
+    const c32 x = ld_stream(a.X + g);
     const double xr = x.re / b0, xi = x.im / b0;
     re += yr[0] * xr - yi[0] * xi;
     im += yr[0] * xi + yi[0] * xr;
   }
   for (int k = 1; k <= m - 2; ++k) {
-    const c32 v = a.V[(size_t)(k - 1) * cells + g];
+    const c32 v = ld_stream(a.V + (size_t)(k - 1) * cells + g);
     re += yr[k] * v.re - yi[k] * v.im;
     im += yr[k] * v.im + yi[k] * v.re;
   }
   const double bl = S.beta[0][m - 1];
   if (bl > 0.0) {
-    const c32 wl = a.W[(size_t)((m - 2) & 1) * cells + g];
-    const c32 vl = a.V[(size_t)(m - 3) * cells + g];
+    const c32 wl = ld_stream(a.W + (size_t)((m - 2) & 1) * cells + g);
+    const c32 vl = ld_stream(a.V + (size_t)(m - 3) * cells + g);
     const float al = (float)S.alpha[0][m - 2];
     const double vr = (double)(wl.re - al * vl.re) / bl, vim = (double)(wl.im - al * vl.im) / bl;
     re += yr[m - 1] * vr - yi[m - 1] * vim;
     im += yr[m - 1] * vim + yi[m - 1] * vr;
   }
   // second half phase with |psi2|^2 (nlse_dev.hpp:192-193)
-  const double rho = (double)a.mf[g] * (re * re + im * im);
+  const double rho = (double)__builtin_nontemporal_load(a.mf + g) * (re * re + im * im);
   double sn, cs;
   sincos(0.5 * a.dt * rho, &sn, &cs);
   ore = re * cs - im * sn;
