@@ -308,7 +308,8 @@ def run_nlse2d(args):
     one = np.ones((n, n))
     psis = [nlse_packets_2d(n, L, 1, t)[0] for t in range(ntraj)]
     b = FusedBatch([n] * ntraj, L, dt, equation="nlse", method="strang", krylov_dim=m,
-                   cells_per_thread=args.cpt)
+                   cells_per_thread=args.cpt, lanes=args.lanes,
+                   pass_kernel=None if args.kernel < 0 else args.kernel)
     for t, psi in enumerate(psis):
         b.set_initial(t, one, one, psi)
     ms, pt = _timed_batch(b, args.warmup, args.steps)

@@ -3273,10 +3273,51 @@ struct nls_batch {
   // Tile autotuner: times steps per (pass kernel, tile) candidate on the live
   // buffers (a step only writes the next-state buffer and Krylov scratch, so
   // timing runs leave the trajectory untouched) and keeps the fastest.
+  // Workgroups of a row-march pass (kinds 2, 3) resident on the whole chip at once.
+  int march_slots(int kind) const {
+    const int nm = nmax_of();
+    int per_cu = 0, dev = 0, ncu = 0;
+    if (kind == 2 && nlse())
+      NLS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nlse_march2d<2>, FBS,
+                                                           march2d_lds_bytes(nm)));
+    else if (kind == 2)
+      NLS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_kge_march2d<2>, FBS,
+                                                           march2d_lds_bytes(nm)));
+    else if (kind == 3)
+      NLS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_kge_rmarch2d<3>, FBS,
+                                                           rmarch2d_lds_bytes(nm)));
+    NLS_HIP(hipGetDevice(&dev));
+    NLS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    return per_cu * ncu;
+  }
+
+  int64_t chunks_at(int c) const {
+    int64_t k = 0;
+    for (auto &d : desc) {
+      const int64_t N = (int64_t)d.n * d.n * (p.dim == 3 ? d.n : 1);
+      k += (N + (int64_t)FBS * c - 1) / ((int64_t)FBS * c);
+    }
+    return k;
+  }
+
   void autotune() {
-    const int cands[][2] = {{3, 12}, {3, 16}, {3, 24}, {3, 32}, {3, 36}, {3, 48}, {3, 64},
-                            {2, 12}, {2, 16}, {2, 24}, {2, 32}, {2, 36}, {2, 48}, {2, 64},
-                            {1, 4},  {1, 8},  {4, 4},  {4, 8},  {0, 4},  {0, 8},  {0, 16}};
+    std::vector<std::array<int, 2>> cands = {
+        {3, 12}, {3, 16}, {3, 24}, {3, 32}, {3, 36}, {3, 48}, {3, 64}, {2, 12}, {2, 16},
+        {2, 24}, {2, 32}, {2, 36}, {2, 48}, {2, 64}, {1, 4},  {1, 8},  {4, 4},  {4, 8},
+        {0, 4},  {0, 8},  {0, 16}};
+    // Row-march tiles whose workgroup count just fits one or two rounds of resident
+    // workgroups: a pass whose last round is nearly empty takes a whole round longer.
+    for (int kind : {2, 3}) {
+      if (!kind_ok(kind, 64)) continue;
+      const int64_t slots = march_slots(kind);
+      for (int rounds = 1; rounds <= 2 && slots > 0; ++rounds)
+        for (int c = 4; c <= 64; ++c)
+          if (chunks_at(c) <= rounds * slots) {
+            if (std::find(cands.begin(), cands.end(), std::array<int, 2>{kind, c}) == cands.end())
+              cands.push_back({kind, c});
+            break;
+          }
+    }
     float best = 1e30f;
     int best_cpt = 4, best_kind = 0, best_lanes = 1;
     hipEvent_t e0, e1;
