@@ -412,6 +412,50 @@ def test_fused_long_horizon_vs_oracle(orc):
     assert errs[-1] < 2e-6, errs
 
 
+def test_fused_nlse_full_horizon_vs_oracle(orc):
+    """The NLSE workload's own size and horizon (200^2 on [-10, 10]^2, m = 12,
+    dt = 0.005, nt = 2000, packets of seed 1): complex64 fused Strang against the
+    complex128 oracle at 5 snapshots: rel-L2 1.5e-6 at the last snapshot when recorded
+    (bar 1e-5 at every snapshot, the KGE field bar), the mass equal to the oracle's (which
+    carries the reference's own ghost-copy mass loss) to 2.4e-8 (bar 1e-6)."""
+    from nls.batch import FusedBatch
+    from nls.samplers import nlse_packets_2d
+    n, L, dt, nt, ns = 200, 10.0, 0.005, 2000, 5
+    psi0, _ = nlse_packets_2d(n, L, 1, 0)
+    one = np.ones((n, n))
+    b = FusedBatch([n], L, dt, equation="nlse", method="strang", krylov_dim=12,
+                   snapshot_every=nt // ns, max_snapshots=ns, cells_per_thread=8, pass_kernel=2)
+    b.set_initial(0, one, one, psi0)
+    b.run(nt - 1)
+    out, _ = b.snapshots(0)
+    ref = orc.nlse_run(n, L, dt * nt, nt, ns, one, one, psi0, krylov=12, method=0, variant=0)
+    errs = [rel_l2(out[k], ref[k]) for k in range(ns)]
+    m0 = np.sum(np.abs(ref[0]) ** 2)
+    dm = [abs(np.sum(np.abs(out[k]) ** 2) - np.sum(np.abs(ref[k]) ** 2)) / m0 for k in range(ns)]
+    print(f"nlse 200^2 x {nt}: rel-L2 {['%.2e' % e for e in errs]}, mass gap {max(dm):.2e}")
+    assert max(errs) < 1e-5, errs
+    assert max(dm) < 1e-6, dm
+
+
+def test_fused_kge_3d_long_horizon_vs_oracle(orc):
+    """1000 sEWI steps of a 32^3 trajectory (m = 12, dt = 0.004, bubbles of seed 3, varying
+    m and c): the FP32-Krylov / FP64-state error stays below the 1e-5 bar at every
+    snapshot (2.2e-7 at the last one when recorded)."""
+    from nls.batch import FusedBatch
+    n, L, dt, nt, ns = 32, 8.0, 0.004, 1000, 5
+    m, c, u0, v0 = make_case_3d(n, 3, 2)
+    b = FusedBatch([n], L, dt, method="sewi", dim=3, krylov_dim=12, mu2=1.0,
+                   snapshot_every=nt // ns, max_snapshots=ns, cells_per_thread=4, pass_kernel=4)
+    b.set_initial(0, m, c, u0, v0)
+    b.run(nt - 1)
+    u, _ = b.snapshots(0)
+    ur, _ = orc.kge_run(n, L, dt * nt, nt, ns, m, c, u0, v0, krylov=12, mu2=1.0, dim=3,
+                        method=2)
+    errs = [rel_l2(u[k], ur[k]) for k in range(1, ns)]
+    print(f"kge 32^3 x {nt}: rel-L2 {['%.2e' % e for e in errs]}")
+    assert max(errs) < TOL, errs
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 def test_fused_kge_energy_tracks_oracle(orc, dim):
     """Discrete energy E = 1/2 |v|^2 + 1/2 u.A u + 1/4 sum m u^4 (A = -L_c + mu2,
