@@ -1014,37 +1014,6 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax,
   __shared__ double shn[NSEQ * PN * (FBS / kWave)];
   // v = (X - cA P) * cB at deposit;  w' = A v - cC P
   float cA[NSEQ] = {0.f, 0.f}, cB[NSEQ] = {0.f, 0.f}, cC[NSEQ] = {0.f, 0.f};
-  if (STAGE >= 1) {
-    reduce_partials(a, T, (j - 1) & 1, red);
-#pragma unroll
-    for (int s = 0; s < NSEQ; ++s) {
-      const double q0 = red[s * PN], q1 = red[s * PN + 1], q2 = red[s * PN + 2];
-      if (STAGE == 1) {
-        const double b0 = sqrt(q0);
-        const double al = q0 > 0.0 ? q1 / q0 : 0.0;
-        const double b1sq = q0 > 0.0 ? (q2 - q1 * q1 / q0) / q0 : 0.0;
-        const double b1 = (b1sq > 1e-14 * (q0 > 0.0 ? q2 / q0 : 0.0) && b1sq > 0.0) ? sqrt(b1sq) : 0.0;
-        cA[s] = (float)al;
-        cB[s] = (b0 > 0.0 && b1 > 0.0) ? (float)(1.0 / (b0 * b1)) : 0.0f;
-        cC[s] = b0 > 0.0 ? (float)(b1 / b0) : 0.0f;
-        if (cm.y == 0 && threadIdx.x == 0) {
-          a.scal[cm.x].b0[s] = b0;
-          a.scal[cm.x].alpha[s][0] = al;
-          a.scal[cm.x].beta[s][1] = b1;
-        }
-      } else {
-        double al, b;
-        lanczos_scalars(q0, q1, q2, al, b);
-        cA[s] = (float)al;
-        cB[s] = b > 0.0 ? (float)(1.0 / b) : 0.0f;
-        cC[s] = (float)b;
-        if (cm.y == 0 && threadIdx.x == 0) {
-          a.scal[cm.x].alpha[s][j - 1] = al;
-          a.scal[cm.x].beta[s][j] = b;
-        }
-      }
-    }
-  }
   const int n = T.n;
   const int N = (int)T.N;
   const int64_t cells = a.cells;
@@ -1117,6 +1086,41 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax,
       sp1[e] = p1;
     }
   };
+  // Lanczos scalars of the previous pass; formed after the first window loads are
+  // issued, so the reduction's latency overlaps theirs
+  auto coefs = [&]() {
+    if (STAGE >= 1) {
+      reduce_partials(a, T, (j - 1) & 1, red);
+#pragma unroll
+      for (int s = 0; s < NSEQ; ++s) {
+        const double q0 = red[s * PN], q1 = red[s * PN + 1], q2 = red[s * PN + 2];
+        if (STAGE == 1) {
+          const double b0 = sqrt(q0);
+          const double al = q0 > 0.0 ? q1 / q0 : 0.0;
+          const double b1sq = q0 > 0.0 ? (q2 - q1 * q1 / q0) / q0 : 0.0;
+          const double b1 = (b1sq > 1e-14 * (q0 > 0.0 ? q2 / q0 : 0.0) && b1sq > 0.0) ? sqrt(b1sq) : 0.0;
+          cA[s] = (float)al;
+          cB[s] = (b0 > 0.0 && b1 > 0.0) ? (float)(1.0 / (b0 * b1)) : 0.0f;
+          cC[s] = b0 > 0.0 ? (float)(b1 / b0) : 0.0f;
+          if (cm.y == 0 && threadIdx.x == 0) {
+            a.scal[cm.x].b0[s] = b0;
+            a.scal[cm.x].alpha[s][0] = al;
+            a.scal[cm.x].beta[s][1] = b1;
+          }
+        } else {
+          double al, b;
+          lanczos_scalars(q0, q1, q2, al, b);
+          cA[s] = (float)al;
+          cB[s] = b > 0.0 ? (float)(1.0 / b) : 0.0f;
+          cC[s] = (float)b;
+          if (cm.y == 0 && threadIdx.x == 0) {
+            a.scal[cm.x].alpha[s][j - 1] = al;
+            a.scal[cm.x].beta[s][j] = b;
+          }
+        }
+      }
+    }
+  };
   // first window [-n, SUB + n), MSPT loads per thread in flight before their deposits
   for (int e0 = 0; e0 < MSUB + 2 * n; e0 += MSPT * FBS) {
     float rc[MSPT], x0[MSPT], x1[MSPT], p0[MSPT], p1[MSPT], rm[MSPT];
@@ -1127,6 +1131,7 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax,
       if (e < MSUB + 2 * n && i >= 0 && i < N)
         fetch(i, rc[q], x0[q], x1[q], p0[q], p1[q], ru[q], rm[q]);
     }
+    if (e0 == 0) coefs();
 #pragma unroll
     for (int q = 0; q < MSPT; ++q) {
       const int e = e0 + threadIdx.x + q * FBS, i = cell(e - n);
@@ -2270,34 +2275,6 @@ __global__ __launch_bounds__(FBS) void k_nlse_march2d(NlseArgs a, int j, int ema
   __shared__ double red[PN];
   __shared__ double shn[PN * (FBS / kWave)];
   float cA = 0.f, cB = 0.f, cC = 0.f;
-  if (STAGE >= 1) {
-    reduce_partials3(a, T, (j - 1) & 1, red);
-    const double q0 = red[0], q1 = red[1], q2 = red[2];
-    if (STAGE == 1) {
-      const double b0 = sqrt(q0);
-      const double al = q0 > 0.0 ? q1 / q0 : 0.0;
-      const double b1sq = q0 > 0.0 ? (q2 - q1 * q1 / q0) / q0 : 0.0;
-      const double b1 = (b1sq > 1e-14 * (q0 > 0.0 ? q2 / q0 : 0.0) && b1sq > 0.0) ? sqrt(b1sq) : 0.0;
-      cA = (float)al;
-      cB = (b0 > 0.0 && b1 > 0.0) ? (float)(1.0 / (b0 * b1)) : 0.0f;
-      cC = b0 > 0.0 ? (float)(b1 / b0) : 0.0f;
-      if (cm.y == 0 && threadIdx.x == 0) {
-        a.scal[cm.x].b0[0] = b0;
-        a.scal[cm.x].alpha[0][0] = al;
-        a.scal[cm.x].beta[0][1] = b1;
-      }
-    } else {
-      double al, b;
-      lanczos_scalars(q0, q1, q2, al, b);
-      cA = (float)al;
-      cB = b > 0.0 ? (float)(1.0 / b) : 0.0f;
-      cC = (float)b;
-      if (cm.y == 0 && threadIdx.x == 0) {
-        a.scal[cm.x].alpha[0][j - 1] = al;
-        a.scal[cm.x].beta[0][j] = b;
-      }
-    }
-  }
   const int n = T.n;
   const int N = (int)T.N;
   const int64_t cells = a.cells;
@@ -2358,6 +2335,38 @@ __global__ __launch_bounds__(FBS) void k_nlse_march2d(NlseArgs a, int j, int ema
     if (w.hd) nb += w.wd * v[e + oD];
     return scl * (nb - w.wsum * vi);
   };
+  // Lanczos scalars of the previous pass; formed after the first window loads are
+  // issued, so the reduction's latency overlaps theirs
+  auto coefs = [&]() {
+    if (STAGE >= 1) {
+      reduce_partials3(a, T, (j - 1) & 1, red);
+      const double q0 = red[0], q1 = red[1], q2 = red[2];
+      if (STAGE == 1) {
+        const double b0 = sqrt(q0);
+        const double al = q0 > 0.0 ? q1 / q0 : 0.0;
+        const double b1sq = q0 > 0.0 ? (q2 - q1 * q1 / q0) / q0 : 0.0;
+        const double b1 = (b1sq > 1e-14 * (q0 > 0.0 ? q2 / q0 : 0.0) && b1sq > 0.0) ? sqrt(b1sq) : 0.0;
+        cA = (float)al;
+        cB = (b0 > 0.0 && b1 > 0.0) ? (float)(1.0 / (b0 * b1)) : 0.0f;
+        cC = b0 > 0.0 ? (float)(b1 / b0) : 0.0f;
+        if (cm.y == 0 && threadIdx.x == 0) {
+          a.scal[cm.x].b0[0] = b0;
+          a.scal[cm.x].alpha[0][0] = al;
+          a.scal[cm.x].beta[0][1] = b1;
+        }
+      } else {
+        double al, b;
+        lanczos_scalars(q0, q1, q2, al, b);
+        cA = (float)al;
+        cB = b > 0.0 ? (float)(1.0 / b) : 0.0f;
+        cC = (float)b;
+        if (cm.y == 0 && threadIdx.x == 0) {
+          a.scal[cm.x].alpha[0][j - 1] = al;
+          a.scal[cm.x].beta[0][j] = b;
+        }
+      }
+    }
+  };
   for (int e0 = 0; e0 < MSUB + 2 * n; e0 += MSPT * FBS) {
     float rc[MSPT];
     c32 rx[MSPT], rp[MSPT];
@@ -2366,6 +2375,7 @@ __global__ __launch_bounds__(FBS) void k_nlse_march2d(NlseArgs a, int j, int ema
       const int e = e0 + threadIdx.x + q * FBS, i = cell(e - n);
       if (e < MSUB + 2 * n && i >= 0 && i < N) fetch(i, rc[q], rx[q], rp[q]);
     }
+    if (e0 == 0) coefs();
 #pragma unroll
     for (int q = 0; q < MSPT; ++q) {
       const int e = e0 + threadIdx.x + q * FBS, i = cell(e - n);
