@@ -26,11 +26,28 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant pass, measured with rocprofv3 counters on this
 # workload: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB copy kernel in the
 # same runs).  Keyed by (kernel, cells per launch); reported only when the tile matches.
-MEASURED_TRAFFIC = {("k_kge_march2d<2>", 9_000_000): 325.0e6}
+MEASURED_TRAFFIC = {("k_kge_march2d<2, false>", 9_000_000): 325.0e6}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def kge_step_bytes(m, pass_kind, uniform):
+    """Algorithmic bytes per cell-step of the fused KGE step with the stored-w' pass kernels
+    (FP32 Krylov vectors of two sequences, FP64 state): pass 0 reads u 8, m 4, c 4 and
+    writes z 8; pass 1 reads z 8, u 8, m 4, c 4 and writes v, w' 16; the m - 3 general
+    passes read w', v 16 and c 4 and write v, w' 16; the combine reads the m - 2 stored
+    basis vectors 8 (m - 2), w' 8, u 8, u_prev 8, m 4 and writes u 8.  With uniform m and
+    c fields the row and plane marches (kinds 2, 4) drop the 4 B of c from every pass and
+    of m from passes 0 and 1, and the combine the 4 B of m.  Kind 3 (recomputed w') as
+    the kernel's own accounting."""
+    if pass_kind == 3:
+        return 16 + 24 + 32 + 28 * (m - 4) + 8 + 8 * (m - 2) + 36
+    cu = uniform and pass_kind in (2, 4)
+    fc = 0 if cu else 4
+    fmc = 0 if uniform else 4
+    return (16 + fc + fc) + (32 + fc + fc) + (32 + fc) * (m - 3) + 8 * (m - 2) + 32 + fmc
 
 
 def make_inputs(ntraj, n, L, seed):
@@ -48,7 +65,7 @@ def fused_run(args, inputs):
     b = FusedBatch([n] * args.ntraj, L, args.dt, krylov_dim=args.krylov, mu2=1.0,
                    use_graph=not args.no_graph, cells_per_thread=args.cpt,
                    pass_kernel=None if args.kernel < 0 else args.kernel, lanes=args.lanes,
-                   reorth=args.reorth)
+                   reorth=args.reorth, always_load_fields=args.load_fields)
     for t, (m, c, u0, v0) in enumerate(inputs):
         b.set_initial(t, m, c, u0, v0)
     b.run(1)  # autotune + graph capture
@@ -381,6 +398,7 @@ def run_kge3d(args):
     ntraj, n, L, dt, m = args.ntraj or 40, args.n or 220, 8.0, 0.004, args.krylov or 12
     one = np.ones((n, n, n))
     b = FusedBatch([n] * ntraj, L, dt, method="sewi", dim=3, krylov_dim=m, mu2=1.0,
+                   always_load_fields=args.load_fields,
                    cells_per_thread=args.cpt, lanes=args.lanes, use_graph=not args.no_graph,
                    pass_kernel=None if args.kernel < 0 else args.kernel)
     for t in range(ntraj):  # Gaussian bubbles of seed 3, drawn and evaluated on the GPU
@@ -391,9 +409,8 @@ def run_kge3d(args):
     cells = info["cells"]
     value = cells * args.steps / (ms / 1e3)
     achieved = pt["bytes_per_launch"] / (pt["avg_ms"] / 1e3) / 1e9 if pt["avg_ms"] > 0 else None
-    # algorithmic bytes per cell-step: pass 0 (24), pass 1 (40), m - 3 general passes (36),
-    # combine (read 2(m-2) basis vectors, w', u, u-, m; write u+: 8(m-2) + 36)
-    step_bytes = 24 + 40 + 36 * (m - 3) + 8 * (m - 2) + 36
+    # m = c = 1 everywhere (sampled on the GPU with constant fields)
+    step_bytes = kge_step_bytes(m, info["pass_kind"], not args.load_fields)
     b.close()
     log(f"[bench] kge3d fused: {value:.3e} cell-steps/s ({ms / args.steps:.1f} ms/step), "
         f"pass {pt['avg_ms'] * 1e3:.1f} us -> {achieved or 0:.0f} GB/s")
@@ -481,6 +498,8 @@ def main():
                          "3 row march storing only the basis")
     ap.add_argument("--lanes", type=int, default=0, help="concurrent parts of the batch, "
                     "0 = autotuner")
+    ap.add_argument("--load-fields", action="store_true",
+                    help="stream the m and c fields in every pass even though they are uniform")
     ap.add_argument("--reorth", type=int, default=2, choices=(0, 1, 2),
                     help="full reorthogonalisation: 0 never, 1 every iteration, 2 (default) "
                          "for trajectories whose basis lost semi-orthogonality")
@@ -565,16 +584,15 @@ def main():
         cpu = cpu_baseline(args, inputs, info["krylov_dim"])
         log(f"[bench] cpu: {cpu['value']:.3e} cell-steps/s on {cpu['cores']} threads")
     m = info["krylov_dim"]
+    uniform = not args.load_fields
     kname = {3: "k_kge_rmarch2d<3>",
-             2: "k_kge_march2d<2>",
+             2: "k_kge_march2d<2, true>" if uniform else "k_kge_march2d<2, false>",
              1: f"k_kge_pass2d<2, {info['cells_per_thread']}>",
              0: f"k_kge_pass<2, 2, {info['cells_per_thread']}>"}[info["pass_kind"]]
+    # the workload's m and c are 1 everywhere: the row march and the combine use the
+    # value instead of streaming the fields unless --load-fields
     traffic = MEASURED_TRAFFIC.get((kname, cells))
-    # algorithmic bytes of a whole step per cell (passes + combine) for the stored-w'
-    # kernels: 24 + 40 + 36 (m - 3) + (16 + 8 (m - 2) + 8 + 4 + 8) with FP32 Krylov
-    # vectors of two sequences and FP64 state
-    step_bytes = (24 + 40 + 36 * (m - 3) + 8 * (m - 2) + 36) if info["pass_kind"] in (0, 1, 2) \
-        else (16 + 24 + 32 + 28 * (m - 4) + 8 + 8 * (m - 2) + 36)
+    step_bytes = kge_step_bytes(m, info["pass_kind"], uniform)
     step_gbs = step_bytes * cells / (ms / args.steps / 1e3) / 1e9
     line = {
         "metric": METRIC,
@@ -594,6 +612,8 @@ def main():
                    "ntraj": args.ntraj, "n": args.n, "dt": args.dt, "krylov_dim": m,
                    "cells_per_thread": info["cells_per_thread"], "pass_kind": info["pass_kind"],
                    "lanes": info["lanes"], "reorth": args.reorth, "cells": cells,
+                   "fields": "streamed every pass" if args.load_fields else
+                             "uniform m = c = 1: value used, fields not streamed",
                    "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,

@@ -264,6 +264,10 @@ typedef struct {
                         semi-orthogonality, |<v_{m-2}, v_0>| > reorth_tol (complex
                         Gram-Schmidt for NLSE).  Not with pass kernel 3 */
   double reorth_tol; /* threshold of mode 2; 0 = 2.5e-4 (about sqrt of the FP32 roundoff) */
+  int always_load_fields; /* 0: a trajectory whose m or c is spatially uniform (found at
+                        upload) has the row- and plane-march passes and the combines use
+                        the value instead of streaming the field (the same numbers, 4 B per
+                        cell and pass less); 1: always stream the fields */
 } nls_batch_params;
 typedef struct {
   int n;             /* full side */

@@ -41,6 +41,7 @@
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 namespace nls {
@@ -60,7 +61,13 @@ struct TrajDev {
   int64_t off, N;
   float s, mu2;
   double tau;
+  // spatially uniform coefficient fields (bit 0: c, bit 1: m) and their FP32 values; a
+  // batch whose trajectories all have both runs the UNI kernel variants, which use
+  // cval / mval instead of streaming the fields (the same numbers)
+  int uni;
+  float cval, mval;
 };
+constexpr int kUniC = 1, kUniM = 2;
 
 struct TrajScal {
   double b0[NSEQ];
@@ -626,8 +633,9 @@ __device__ inline void block_sum_n(double (&v)[NV], double *sh /* NV * (FBS/64) 
 constexpr int MSPT = 4;               // cells per thread per sub-chunk
 constexpr int MSUB = FBS * MSPT;      // sub-chunk cells
 
-inline size_t march2d_lds_bytes(int nmax) {
-  return sizeof(float) * 5 * (size_t)(MSUB + 2 * nmax);
+// c, v_j (two arrays) and the previous vector (two arrays); without c for uniform fields
+inline size_t march2d_lds_bytes(int nmax, bool uni = false) {
+  return sizeof(float) * (uni ? 4 : 5) * (size_t)(MSUB + 2 * nmax);
 }
 
 // ---------------------------------------------------------------- 2D row march, recomputed w'
@@ -675,6 +683,23 @@ __device__ inline Sten5 sten5(const float *sc, int e, int oU, int oL, int oR, in
   w.wl = w.hl ? 0.5f * (ci + sc[e + oL]) : 0.0f;
   w.wr = w.hr ? 0.5f * (ci + sc[e + oR]) : 0.0f;
   w.wd = w.hd ? 0.5f * (ci + sc[e + oD]) : 0.0f;
+  w.wsum = ((w.wu + w.wl) + w.wr) + w.wd;
+  return w;
+}
+
+// sten5 for a uniform c: every weight (c + c) / 2 is c, formed as the field path forms it
+__device__ inline Sten5 sten5_uniform(float cu, int i, int n, float rn) {
+  const int r = row_of(i, n, rn), col = i - r * n;
+  Sten5 w;
+  w.hu = r > 0;
+  w.hl = col > 0;
+  w.hr = col < n - 1;
+  w.hd = r < n - 1;
+  const float wc = 0.5f * (cu + cu);
+  w.wu = w.hu ? wc : 0.0f;
+  w.wl = w.hl ? wc : 0.0f;
+  w.wr = w.hr ? wc : 0.0f;
+  w.wd = w.hd ? wc : 0.0f;
   w.wsum = ((w.wu + w.wl) + w.wr) + w.wd;
   return w;
 }
@@ -1004,7 +1029,8 @@ inline size_t rmarch2d_lds_bytes(int nmax) {
 // w'_j = A v_j - b_j v_{j-1}.  Same local coordinates as k_kge_rmarch2d: blocks with an
 // odd range index march downward (alt != 0), so shared halo rows come from L2.
 // STAGE 0: j = 0 on the seeds x (stores z = A x); 1: j = 1 (from z and x); 2: j >= 2.
-template <int STAGE>
+// UNI: every trajectory of the batch has uniform m and c (values from the descriptor).
+template <int STAGE, bool UNI = false>
 __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax, int alt) {
   extern __shared__ float lds[];
   const int2 cm = a.cmap[blockIdx.x];
@@ -1017,8 +1043,9 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax,
   const int n = T.n;
   const int N = (int)T.N;
   const int64_t cells = a.cells;
-  float *sc = lds, *sv0 = lds + emax, *sv1 = lds + 2 * emax, *sp0 = lds + 3 * emax,
-        *sp1 = lds + 4 * emax;
+  // no c window for uniform fields
+  float *sc = lds, *sv0 = lds + (UNI ? 0 : emax), *sv1 = sv0 + emax, *sp0 = sv1 + emax,
+        *sp1 = sp0 + emax;
   const size_t m2 = (size_t)(a.m - 2);
   const float *cT = a.cf + T.off, *mT = a.mf + T.off;
   const double *uT = a.u + T.off;
@@ -1049,10 +1076,10 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax,
   // deposit() uses them
   auto fetch = [&](int i, float &c, float &x0, float &x1, float &p0, float &p1, double &uu,
                    float &mm) {
-    c = cT[i];
+    c = UNI ? T.cval : cT[i];
     if (STAGE <= 1) {
       uu = uT[i];
-      mm = mT[i];
+      mm = UNI ? T.mval : mT[i];
     }
     if (STAGE >= 1) {
       x0 = X0[i];
@@ -1065,7 +1092,7 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax,
   };
   auto deposit = [&](int e, float c, float x0, float x1, float p0, float p1, double uu,
                      float mm) {
-    sc[e] = c;
+    if (!UNI) sc[e] = c;
     if (STAGE <= 1) { // seeds x0 = u, x1 = -m u^3 (FP64 -> FP32)
       const float s0 = (float)uu, s1 = (float)(-(double)mm * uu * uu * uu);
       if (STAGE == 0) {
@@ -1158,7 +1185,7 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax,
       const int l = cb + threadIdx.x + kk * FBS;
       if (l < ce) {
         const int i = cell(l), e = l - cb + n;
-        const Sten5 w = sten5(sc, e, oU, oL, oR, oD, i, n, rn);
+        const Sten5 w = UNI ? sten5_uniform(T.cval, i, n, rn) : sten5(sc, e, oU, oL, oR, oD, i, n, rn);
 #pragma unroll
         for (int s = 0; s < NSEQ; ++s) {
           const float *sv = s == 0 ? sv0 : sv1;
@@ -1184,7 +1211,7 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax,
     __syncthreads(); // every read of this window is done
     // keep [cb' - n, cb' + n) (disjoint source and destination: 2n <= SUB)
     for (int e = threadIdx.x; e < 2 * n; e += FBS) {
-      sc[e] = sc[MSUB + e];
+      if (!UNI) sc[e] = sc[MSUB + e];
       sv0[e] = sv0[MSUB + e];
       sv1[e] = sv1[MSUB + e];
       if (STAGE >= 1) {
@@ -1226,11 +1253,12 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax,
 // STAGE 0: j = 0 on the seeds x (stores z = A x); 1: j = 1 (from z and x); 2: j >= 2.
 constexpr int M3_HPT = 2; // window-end cells per thread: 2n <= 256 * M3_HPT
 
-inline size_t march3d_lds_bytes(int cpt, int nmax) {
-  return sizeof(float) * 3 * (size_t)(FBS * cpt + 2 * nmax);
+// c and v_j (two arrays); without c for uniform fields
+inline size_t march3d_lds_bytes(int cpt, int nmax, bool uni = false) {
+  return sizeof(float) * (uni ? 2 : 3) * (size_t)(FBS * cpt + 2 * nmax);
 }
 
-template <int STAGE, int CPT>
+template <int STAGE, int CPT, bool UNI = false>
 __global__ __launch_bounds__(FBS) void k_kge_march3d(KgeArgs a, int j, int emax) {
   extern __shared__ float lds[];
   const int2 cm = a.cmap[blockIdx.x];
@@ -1274,7 +1302,7 @@ __global__ __launch_bounds__(FBS) void k_kge_march3d(KgeArgs a, int j, int emax)
   const int n = T.n, n2 = n * n, N = (int)T.N;
   const int64_t cells = a.cells;
   const size_t m2 = (size_t)(a.m - 2);
-  float *sc = lds, *sv0 = lds + emax, *sv1 = lds + 2 * emax;
+  float *sc = lds, *sv0 = lds + (UNI ? 0 : emax), *sv1 = sv0 + emax; // no c window if UNI
   const float *cT = a.cf + T.off, *mT = a.mf + T.off;
   const double *uT = a.u + T.off;
   const float *X0 = a.W + ((size_t)0 * 2 + (STAGE == 1 ? 0 : ((j - 1) & 1))) * cells + T.off;
@@ -1304,10 +1332,10 @@ __global__ __launch_bounds__(FBS) void k_kge_march3d(KgeArgs a, int j, int emax)
     double uu;
   };
   auto fetch = [&](int g, Raw &r) {
-    r.c = cT[g];
+    r.c = UNI ? T.cval : cT[g];
     if (STAGE <= 1) {
       r.uu = uT[g];
-      r.mm = mT[g];
+      r.mm = UNI ? T.mval : mT[g];
     }
     if (STAGE >= 1) {
       r.x0 = X0[g];
@@ -1404,7 +1432,7 @@ __global__ __launch_bounds__(FBS) void k_kge_march3d(KgeArgs a, int j, int emax)
     for (int q = 0; q < CPT; ++q)
       if (own[q]) {
         const int e = po[q] - pa + n;
-        sc[e] = ccz[q];
+        if (!UNI) sc[e] = ccz[q];
         sv0[e] = vc0[q];
         sv1[e] = vc1[q];
       }
@@ -1416,7 +1444,7 @@ __global__ __launch_bounds__(FBS) void k_kge_march3d(KgeArgs a, int j, int emax)
         if (g >= 0 && g < N) {
           const int e = hwin(h);
           float t0, t1;
-          sc[e] = rh[k].c;
+          if (!UNI) sc[e] = rh[k].c;
           form(rh[k], sv0[e], sv1[e], t0, t1);
         }
       }
@@ -1448,10 +1476,10 @@ __global__ __launch_bounds__(FBS) void k_kge_march3d(KgeArgs a, int j, int emax)
       const bool hxm = col > 0, hxp = col < n - 1;
       const float ci = ccz[q];
       const float wzm = hzm ? 0.5f * (ci + cmz[q]) : 0.0f;
-      const float wym = hym ? 0.5f * (ci + sc[e - n]) : 0.0f;
-      const float wxm = hxm ? 0.5f * (ci + sc[e - 1]) : 0.0f;
-      const float wxp = hxp ? 0.5f * (ci + sc[e + 1]) : 0.0f;
-      const float wyp = hyp ? 0.5f * (ci + sc[e + n]) : 0.0f;
+      const float wym = hym ? 0.5f * (ci + (UNI ? ci : sc[e - n])) : 0.0f;
+      const float wxm = hxm ? 0.5f * (ci + (UNI ? ci : sc[e - 1])) : 0.0f;
+      const float wxp = hxp ? 0.5f * (ci + (UNI ? ci : sc[e + 1])) : 0.0f;
+      const float wyp = hyp ? 0.5f * (ci + (UNI ? ci : sc[e + n])) : 0.0f;
       const float wzp = hzp ? 0.5f * (ci + cnz[q]) : 0.0f;
       const float wsum = ((((wzm + wym) + wxm) + wxp) + wyp) + wzp;
 #pragma unroll
@@ -1507,7 +1535,7 @@ __global__ __launch_bounds__(FBS) void k_kge_march3d(KgeArgs a, int j, int emax)
 // coefficients: d = y0 x/b0 + sum_{k<m-2} y_k v_k + (y_{m-2} - y_{m-1} alpha/b) v_{m-2}
 // + (y_{m-1}/b) w'_{m-2}, so each stored vector is read exactly once.
 // ORTH: also the orthogonality partials <v_{m-2}, v_0> (reorthogonalisation modes 1, 2).
-template <int DIM, bool ORTH>
+template <int DIM, bool ORTH, bool UNI = false>
 __global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a) {
   const int2 cm = a.cmap[blockIdx.x];
   if (cm.x < 0) return;
@@ -1550,7 +1578,8 @@ __global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a) {
       ok[kk] = i < cend;
       gs[kk] = T.off + (ok[kk] ? clamp_src<DIM>(i, n) : 0);
       const double uu = a.u[gs[kk]];
-      const double g1 = -(double)__builtin_nontemporal_load(a.mf + gs[kk]) * uu * uu * uu;
+      const double mm = UNI ? (double)T.mval : (double)__builtin_nontemporal_load(a.mf + gs[kk]);
+      const double g1 = -mm * uu * uu * uu;
       d0[kk] = cv[0][0] * (double)(float)uu + cw[0] * (double)__builtin_nontemporal_load(Wl0 + gs[kk]);
       d1[kk] = cv[1][0] * (double)(float)g1 + cw[1] * (double)__builtin_nontemporal_load(Wl1 + gs[kk]);
       xs0[kk] = (float)uu;
@@ -1606,7 +1635,7 @@ __global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a) {
 
 // Snapshot combine over the workgroup's range: also the velocity (u+ - u)/tau before
 // the ghost copy and both copies of u+ (same folded coefficients as above).
-template <int DIM, bool ORTH>
+template <int DIM, bool ORTH, bool UNI = false>
 __global__ __launch_bounds__(FBS) void k_kge_combine_snap(KgeArgs a, SnapArgs snap) {
   const int2 cm = a.cmap[blockIdx.x];
   if (cm.x < 0) return;
@@ -1634,7 +1663,7 @@ __global__ __launch_bounds__(FBS) void k_kge_combine_snap(KgeArgs a, SnapArgs sn
   const double t2 = T.tau * T.tau;
   auto unew = [&](int64_t g) {
     const double uu = a.u[g];
-    const double g1 = -(double)a.mf[g] * uu * uu * uu;
+    const double g1 = -(UNI ? (double)T.mval : (double)a.mf[g]) * uu * uu * uu;
     double d0 = cv[0][0] * (double)(float)uu + cw[0] * (double)a.W[((size_t)0 * 2 + ((m - 2) & 1)) * cells + g];
     double d1 = cv[1][0] * (double)(float)g1 + cw[1] * (double)a.W[((size_t)1 * 2 + ((m - 2) & 1)) * cells + g];
     for (int k = 1; k <= m - 2; ++k) {
@@ -2163,7 +2192,8 @@ __device__ inline c32 ld_stream(const c32 *p) {
   return c32{v.x, v.y};
 }
 
-__device__ inline void nlse_psi3(const NlseArgs &a, const TrajScal &S, const double *yr,
+template <bool UNI>
+__device__ inline void nlse_psi3(const NlseArgs &a, const TrajDev &T, const TrajScal &S, const double *yr,
                                  const double *yi, int64_t g, double &ore, double &oim) {
   const int m = a.m;
   const int64_t cells = a.cells;
@@ -2190,7 +2220,8 @@ __device__ inline void nlse_psi3(const NlseArgs &a, const TrajScal &S, const dou
     im += yr[m - 1] * vim + yi[m - 1] * vr;
   }
   // second half phase with |psi2|^2 (nlse_dev.hpp:192-193)
-  const double rho = (double)__builtin_nontemporal_load(a.mf + g) * (re * re + im * im);
+  const double mm = UNI ? (double)T.mval : (double)__builtin_nontemporal_load(a.mf + g);
+  const double rho = mm * (re * re + im * im);
   double sn, cs;
   sincos(0.5 * a.dt * rho, &sn, &cs);
   ore = re * cs - im * sn;
@@ -2216,7 +2247,7 @@ __device__ inline void nlse_store_orth(const NlseArgs &a, const TrajDev &T, cons
 }
 
 // ORTH: also the orthogonality partials (reorthogonalisation modes 1 and 2).
-template <int DIM, int CPT, bool ORTH>
+template <int DIM, int CPT, bool ORTH, bool UNI = false>
 __global__ __launch_bounds__(FBS) void k_nlse_combine(NlseArgs a, SnapC snap) {
   const int2 cm = a.cmap[blockIdx.x];
   if (cm.x < 0) return;
@@ -2240,11 +2271,11 @@ __global__ __launch_bounds__(FBS) void k_nlse_combine(NlseArgs a, SnapC snap) {
     const int64_t src = clamp_src<DIM>(i, n);
     const int64_t gi = T.off + i, gs = T.off + src;
     double sr, si;
-    nlse_psi3(a, S, ys[0], ys[1], gs, sr, si);
+    nlse_psi3<UNI>(a, T, S, ys[0], ys[1], gs, sr, si);
     a.psi_next[gi] = c32{(float)sr, (float)si};
     if (snap.psi) { // snapshot before the ghost copy (nlse_dev.hpp:195-197)
       double ir = sr, ii = si;
-      if (src != i) nlse_psi3(a, S, ys[0], ys[1], gi, ir, ii);
+      if (src != i) nlse_psi3<UNI>(a, T, S, ys[0], ys[1], gi, ir, ii);
       snap.psi[2 * gi] = ir;
       snap.psi[2 * gi + 1] = ii;
     }
@@ -2266,7 +2297,7 @@ __global__ __launch_bounds__(FBS) void k_nlse_combine(NlseArgs a, SnapC snap) {
 // and X; STAGE 2 is the general pass.
 typedef float f2v __attribute__((ext_vector_type(2)));
 
-template <int STAGE>
+template <int STAGE, bool UNI = false>
 __global__ __launch_bounds__(FBS) void k_nlse_march2d(NlseArgs a, int j, int emax, int alt) {
   extern __shared__ float lds[];
   const int2 cm = a.cmap[blockIdx.x];
@@ -2278,8 +2309,9 @@ __global__ __launch_bounds__(FBS) void k_nlse_march2d(NlseArgs a, int j, int ema
   const int n = T.n;
   const int N = (int)T.N;
   const int64_t cells = a.cells;
-  float *sc = lds, *sv0 = lds + emax, *sv1 = lds + 2 * emax, *sp0 = lds + 3 * emax,
-        *sp1 = lds + 4 * emax;
+  // no c window for uniform fields
+  float *sc = lds, *sv0 = lds + (UNI ? 0 : emax), *sv1 = sv0 + emax, *sp0 = sv1 + emax,
+        *sp1 = sp0 + emax;
   const c32 *psiT = a.psi + T.off;
   const float *cT = a.cf + T.off, *mT = a.mf + T.off;
   const c32 *Xin = a.W + (STAGE == 1 ? 0 : (size_t)((j - 1) & 1) * cells) + T.off; // z or w'
@@ -2303,17 +2335,17 @@ __global__ __launch_bounds__(FBS) void k_nlse_march2d(NlseArgs a, int j, int ema
   };
   // raw inputs of cell i (no arithmetic here, so the loads stay in flight)
   auto fetch = [&](int i, float &c, c32 &x, c32 &pv) {
-    c = cT[i];
+    c = UNI ? T.cval : cT[i];
     if (STAGE == 0) {
       x = psiT[i];
-      pv.re = mT[i];
+      pv.re = UNI ? T.mval : mT[i];
     } else {
       x = Xin[i];
       pv = Pin[i];
     }
   };
   auto deposit = [&](int e, float c, c32 x, c32 pv) {
-    sc[e] = c;
+    if (!UNI) sc[e] = c;
     if (STAGE == 0) {
       const c32 v = nl_half_phase(x, pv.re, a.dt);
       sv0[e] = v.re;
@@ -2400,7 +2432,7 @@ __global__ __launch_bounds__(FBS) void k_nlse_march2d(NlseArgs a, int j, int ema
       const int l = cb + threadIdx.x + kk * FBS;
       if (l < ce) {
         const int i = cell(l), e = l - cb + n;
-        const Sten5 w = sten5(sc, e, oU, oL, oR, oD, i, n, rn);
+        const Sten5 w = UNI ? sten5_uniform(T.cval, i, n, rn) : sten5(sc, e, oU, oL, oR, oD, i, n, rn);
         const float vr = sv0[e], vi = sv1[e];
         const float lr = lap(w, sv0, e), li = lap(w, sv1, e);
         if (STAGE == 0) {
@@ -2422,7 +2454,7 @@ __global__ __launch_bounds__(FBS) void k_nlse_march2d(NlseArgs a, int j, int ema
     if (!more) break;
     __syncthreads(); // every read of this window is done
     for (int e = threadIdx.x; e < 2 * n; e += FBS) { // keep [cb' - n, cb' + n); 2n <= SUB
-      sc[e] = sc[MSUB + e];
+      if (!UNI) sc[e] = sc[MSUB + e];
       sv0[e] = sv0[MSUB + e];
       sv1[e] = sv1[MSUB + e];
       if (STAGE >= 1) {
@@ -2451,7 +2483,7 @@ __global__ __launch_bounds__(FBS) void k_nlse_march2d(NlseArgs a, int j, int ema
 // cell it loads and stores it (X) with W[0] = L psi1, STAGE 1 forms v_1 from W[0] and X,
 // STAGE 2 is the general pass.  Same window, registers and cross-plane couplings as the
 // Klein-Gordon kernel.
-template <int STAGE, int CPT>
+template <int STAGE, int CPT, bool UNI = false>
 __global__ __launch_bounds__(FBS) void k_nlse_march3d(NlseArgs a, int j, int emax) {
   extern __shared__ float lds[];
   const int2 cm = a.cmap[blockIdx.x];
@@ -2490,7 +2522,7 @@ __global__ __launch_bounds__(FBS) void k_nlse_march3d(NlseArgs a, int j, int ema
   }
   const int n = T.n, n2 = n * n, N = (int)T.N;
   const int64_t cells = a.cells;
-  float *sc = lds, *sv0 = lds + emax, *sv1 = lds + 2 * emax;
+  float *sc = lds, *sv0 = lds + (UNI ? 0 : emax), *sv1 = sv0 + emax; // no c window if UNI
   const c32 *psiT = a.psi + T.off;
   const float *cT = a.cf + T.off, *mT = a.mf + T.off;
   const c32 *Xin = a.W + (STAGE == 1 ? 0 : (size_t)((j - 1) & 1) * cells) + T.off; // z or w'
@@ -2517,10 +2549,10 @@ __global__ __launch_bounds__(FBS) void k_nlse_march3d(NlseArgs a, int j, int ema
     c32 x, p; // STAGE 0: x = psi, p.re = m
   };
   auto fetch = [&](int g, Raw &r) {
-    r.c = cT[g];
+    r.c = UNI ? T.cval : cT[g];
     if (STAGE == 0) {
       r.x = psiT[g];
-      r.p.re = mT[g];
+      r.p.re = UNI ? T.mval : mT[g];
     } else {
       r.x = Xin[g];
       r.p = Pin[g];
@@ -2597,7 +2629,7 @@ __global__ __launch_bounds__(FBS) void k_nlse_march3d(NlseArgs a, int j, int ema
     for (int q = 0; q < CPT; ++q)
       if (own[q]) {
         const int e = po[q] - pa + n;
-        sc[e] = ccz[q];
+        if (!UNI) sc[e] = ccz[q];
         sv0[e] = vc0[q];
         sv1[e] = vc1[q];
       }
@@ -2609,7 +2641,7 @@ __global__ __launch_bounds__(FBS) void k_nlse_march3d(NlseArgs a, int j, int ema
         if (g >= 0 && g < N) {
           const int e = hwin(h);
           float t0, t1;
-          sc[e] = rh[k].c;
+          if (!UNI) sc[e] = rh[k].c;
           form(rh[k], sv0[e], sv1[e], t0, t1);
         }
       }
@@ -2639,10 +2671,10 @@ __global__ __launch_bounds__(FBS) void k_nlse_march3d(NlseArgs a, int j, int ema
       const bool hxm = col > 0, hxp = col < n - 1;
       const float ci = ccz[q];
       const float wzm = hzm ? 0.5f * (ci + cmz[q]) : 0.0f;
-      const float wym = hym ? 0.5f * (ci + sc[e - n]) : 0.0f;
-      const float wxm = hxm ? 0.5f * (ci + sc[e - 1]) : 0.0f;
-      const float wxp = hxp ? 0.5f * (ci + sc[e + 1]) : 0.0f;
-      const float wyp = hyp ? 0.5f * (ci + sc[e + n]) : 0.0f;
+      const float wym = hym ? 0.5f * (ci + (UNI ? ci : sc[e - n])) : 0.0f;
+      const float wxm = hxm ? 0.5f * (ci + (UNI ? ci : sc[e - 1])) : 0.0f;
+      const float wxp = hxp ? 0.5f * (ci + (UNI ? ci : sc[e + 1])) : 0.0f;
+      const float wyp = hyp ? 0.5f * (ci + (UNI ? ci : sc[e + n])) : 0.0f;
       const float wzp = hzp ? 0.5f * (ci + cnz[q]) : 0.0f;
       const float wsum = ((((wzm + wym) + wxm) + wxp) + wyp) + wzp;
       float lv[2];
@@ -2697,7 +2729,7 @@ __global__ __launch_bounds__(FBS) void k_nlse_march3d(NlseArgs a, int j, int ema
 
 // Combine over the workgroup's range (any layout): psi2 = sum_k y_k v_k, the second half
 // phase, the pre-copy snapshot and the ghost copy.
-template <int DIM, bool ORTH>
+template <int DIM, bool ORTH, bool UNI = false>
 __global__ __launch_bounds__(FBS) void k_nlse_combine_rt(NlseArgs a, SnapC snap) {
   const int2 cm = a.cmap[blockIdx.x];
   if (cm.x < 0) return;
@@ -2719,11 +2751,11 @@ __global__ __launch_bounds__(FBS) void k_nlse_combine_rt(NlseArgs a, SnapC snap)
     const int64_t src = clamp_src<DIM>(i, n);
     const int64_t gi = T.off + i, gs = T.off + src;
     double sr, si;
-    nlse_psi3(a, S, ys[0], ys[1], gs, sr, si);
+    nlse_psi3<UNI>(a, T, S, ys[0], ys[1], gs, sr, si);
     a.psi_next[gi] = c32{(float)sr, (float)si};
     if (snap.psi) { // snapshot before the ghost copy (nlse_dev.hpp:195-197)
       double ir = sr, ii = si;
-      if (src != i) nlse_psi3(a, S, ys[0], ys[1], gi, ir, ii);
+      if (src != i) nlse_psi3<UNI>(a, T, S, ys[0], ys[1], gi, ir, ii);
       snap.psi[2 * gi] = ir;
       snap.psi[2 * gi + 1] = ii;
     }
@@ -2868,6 +2900,11 @@ struct nls_batch {
   int ntraj = 0;
   std::vector<nls_traj_desc> desc;
   Layout lay;
+  // per trajectory: uniform coefficient fields found at upload (TrajDev::uni, cval, mval)
+  std::vector<int> uni;
+  std::vector<float> uni_c, uni_m;
+  bool uni_all = false;  // every trajectory has uniform m and c: the UNI kernel variants
+  bool recapture = false; // the step graphs hold the other variants
   int m = 10, cpt = 8;
   bool tuned = false;
   hipStream_t s = nullptr;
@@ -2935,6 +2972,12 @@ struct nls_batch {
     }
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (s) (void)hipStreamDestroy(s);
+  }
+
+  // Calls f(std::true_type) for a batch of uniform fields, f(std::false_type) otherwise.
+  template <class F> void with_uni(F &&f) const {
+    if (uni_all) f(std::true_type{});
+    else f(std::false_type{});
   }
 
   void drop_graphs() {
@@ -3006,16 +3049,19 @@ struct nls_batch {
     const dim3 grid(ngrid), blk(FBS);
     const bool march = pass_kind == 2 && DIM == 2;
     const bool plane = pass_kind == 4 && DIM == 3 && (CPT == 4 || CPT == 8);
-    const size_t lds = march2d_lds_bytes(nmax);
+    const size_t lds = march2d_lds_bytes(nmax, uni_all);
     const int emax = MSUB + 2 * nmax;
-    const size_t lds3 = march3d_lds_bytes(CPT, nmax);
+    const size_t lds3 = march3d_lds_bytes(CPT, nmax, uni_all);
     const int emax3 = FBS * CPT + 2 * nmax;
     // pass j of the 3D plane march (tiles 4 and 8 only)
     auto plane_pass = [&](int stage, int j) {
       if constexpr (CPT == 4 || CPT == 8) {
-        if (stage == 0) hipLaunchKernelGGL((k_nlse_march3d<0, CPT>), grid, blk, lds3, s, a, j, emax3);
-        else if (stage == 1) hipLaunchKernelGGL((k_nlse_march3d<1, CPT>), grid, blk, lds3, s, a, j, emax3);
-        else hipLaunchKernelGGL((k_nlse_march3d<2, CPT>), grid, blk, lds3, s, a, j, emax3);
+        with_uni([&](auto u) {
+          constexpr bool U = decltype(u)::value;
+          if (stage == 0) hipLaunchKernelGGL((k_nlse_march3d<0, CPT, U>), grid, blk, lds3, s, a, j, emax3);
+          else if (stage == 1) hipLaunchKernelGGL((k_nlse_march3d<1, CPT, U>), grid, blk, lds3, s, a, j, emax3);
+          else hipLaunchKernelGGL((k_nlse_march3d<2, CPT, U>), grid, blk, lds3, s, a, j, emax3);
+        });
       }
     };
     auto reorth = [&](int j) {
@@ -3023,9 +3069,17 @@ struct nls_batch {
       hipLaunchKernelGGL(k_nlse_reorth_dots, dim3(ngrid, (j + 3) / 4), blk, 0, s, a, j);
       hipLaunchKernelGGL(k_nlse_reorth_apply, grid, blk, 0, s, a, j);
     };
+    auto march_pass = [&](int stage, int j) {
+      with_uni([&](auto u) {
+        constexpr bool U = decltype(u)::value;
+        if (stage == 0) hipLaunchKernelGGL((k_nlse_march2d<0, U>), grid, blk, lds, s, a, j, emax, 1);
+        else if (stage == 1) hipLaunchKernelGGL((k_nlse_march2d<1, U>), grid, blk, lds, s, a, j, emax, 1);
+        else hipLaunchKernelGGL((k_nlse_march2d<2, U>), grid, blk, lds, s, a, j, emax, 1);
+      });
+    };
     if (march) {
-      hipLaunchKernelGGL(k_nlse_march2d<0>, grid, blk, lds, s, a, 0, emax, 1);
-      hipLaunchKernelGGL(k_nlse_march2d<1>, grid, blk, lds, s, a, 1, emax, 1);
+      march_pass(0, 0);
+      march_pass(1, 1);
     } else if (plane) {
       plane_pass(0, 0);
       plane_pass(1, 1);
@@ -3036,7 +3090,7 @@ struct nls_batch {
     reorth(1);
     for (int j = 2; j <= m - 2; ++j) {
       if (prof) NLS_HIP(hipEventRecord(next_event(), s));
-      if (march) hipLaunchKernelGGL(k_nlse_march2d<2>, grid, blk, lds, s, a, j, emax, 1);
+      if (march) march_pass(2, j);
       else if (plane) plane_pass(2, j);
       else hipLaunchKernelGGL((k_nlse_pass<DIM, 2, CPT>), grid, blk, 0, s, a, j);
       if (prof) NLS_HIP(hipEventRecord(next_event(), s));
@@ -3046,11 +3100,17 @@ struct nls_batch {
     SnapC sn{nullptr};
     if (snapshot) sn.psi = d_snapU + (size_t)slot * lay.cells * 2;
     if (march || plane) {
-      if (p.reorth) hipLaunchKernelGGL((k_nlse_combine_rt<DIM, true>), grid, blk, 0, s, a, sn);
-      else hipLaunchKernelGGL((k_nlse_combine_rt<DIM, false>), grid, blk, 0, s, a, sn);
+      with_uni([&](auto u) {
+        constexpr bool U = decltype(u)::value;
+        if (p.reorth) hipLaunchKernelGGL((k_nlse_combine_rt<DIM, true, U>), grid, blk, 0, s, a, sn);
+        else hipLaunchKernelGGL((k_nlse_combine_rt<DIM, false, U>), grid, blk, 0, s, a, sn);
+      });
     } else {
-      if (p.reorth) hipLaunchKernelGGL((k_nlse_combine<DIM, CPT, true>), grid, blk, 0, s, a, sn);
-      else hipLaunchKernelGGL((k_nlse_combine<DIM, CPT, false>), grid, blk, 0, s, a, sn);
+      with_uni([&](auto u) {
+        constexpr bool U = decltype(u)::value;
+        if (p.reorth) hipLaunchKernelGGL((k_nlse_combine<DIM, CPT, true, U>), grid, blk, 0, s, a, sn);
+        else hipLaunchKernelGGL((k_nlse_combine<DIM, CPT, false, U>), grid, blk, 0, s, a, sn);
+      });
     }
     check_launch("fused nlse step");
   }
@@ -3077,6 +3137,7 @@ struct nls_batch {
     const double mu2 = p.mu2;
     lay = make_layout(desc.data(), ntraj, p.dim, cpt, p.dt, mu2, pass_kind >= 2 && pass_kind <= 3,
                       pass_kind == 4);
+    apply_uniform();
     nmax = 0;
     for (auto &T : lay.traj) nmax = std::max(nmax, T.n);
     NLS_REQUIRE(lay.cmap.size() <= cmap_cap, "internal: chunk map larger than allocated");
@@ -3085,6 +3146,42 @@ struct nls_batch {
     NLS_HIP(hipMemcpyAsync(d_cmap, lay.cmap.data(), sizeof(int2) * lay.cmap.size(),
                            hipMemcpyHostToDevice, s));
     set_lanes(lanes);
+  }
+
+  void apply_uniform() {
+    bool all = !uni.empty();
+    for (int t = 0; t < ntraj && t < (int)uni.size(); ++t) {
+      lay.traj[t].uni = uni[t];
+      lay.traj[t].cval = uni_c[t];
+      lay.traj[t].mval = uni_m[t];
+      all = all && uni[t] == (kUniC | kUniM);
+    }
+    if (all != uni_all && graphs[0]) {
+      drop_graphs();
+      recapture = true;
+    }
+    uni_all = all;
+  }
+
+  // Records whether trajectory t's FP32 fields are spatially uniform (unless the
+  // parameters ask for the fields to be streamed always) and uploads its descriptor.
+  void set_uniform(int t, const float *mf, const float *cf, int64_t N) {
+    if (uni.empty()) {
+      uni.assign(ntraj, 0);
+      uni_c.assign(ntraj, 0.0f);
+      uni_m.assign(ntraj, 0.0f);
+    }
+    auto same = [N](const float *f) {
+      for (int64_t i = 1; i < N; ++i)
+        if (f[i] != f[0]) return false;
+      return true;
+    };
+    const bool allow = p.always_load_fields == 0 && N > 0;
+    uni[t] = (allow && same(cf) ? kUniC : 0) | (allow && same(mf) ? kUniM : 0);
+    uni_c[t] = N > 0 ? cf[0] : 0.0f;
+    uni_m[t] = N > 0 ? mf[0] : 0.0f;
+    apply_uniform();
+    NLS_HIP(hipMemcpyAsync(d_traj + t, &lay.traj[t], sizeof(TrajDev), hipMemcpyHostToDevice, s));
   }
 
   // Splits the trajectories into L lanes (largest first onto the lane with fewest
@@ -3146,14 +3243,20 @@ struct nls_batch {
                                                  unsigned ngrid) {
     const dim3 grid(ngrid), blk(FBS);
     if (pass_kind == 4) {
-      const size_t lds = march3d_lds_bytes(cpt, nmax);
+      const size_t lds = march3d_lds_bytes(cpt, nmax, uni_all);
       const int emax = FBS * cpt + 2 * nmax;
-      if (cpt == 4) hipLaunchKernelGGL((k_kge_march3d<STAGE, 4>), grid, blk, lds, s, a, j, emax);
-      else hipLaunchKernelGGL((k_kge_march3d<STAGE, 8>), grid, blk, lds, s, a, j, emax);
+      with_uni([&](auto u) {
+        constexpr bool U = decltype(u)::value;
+        if (cpt == 4) hipLaunchKernelGGL((k_kge_march3d<STAGE, 4, U>), grid, blk, lds, s, a, j, emax);
+        else hipLaunchKernelGGL((k_kge_march3d<STAGE, 8, U>), grid, blk, lds, s, a, j, emax);
+      });
     } else if (pass_kind == 2) {
-      const size_t lds = march2d_lds_bytes(nmax);
+      const size_t lds = march2d_lds_bytes(nmax, uni_all);
       const int emax = MSUB + 2 * nmax;
-      hipLaunchKernelGGL((k_kge_march2d<STAGE>), grid, blk, lds, s, a, j, emax, 1);
+      with_uni([&](auto u) {
+        constexpr bool U = decltype(u)::value;
+        hipLaunchKernelGGL((k_kge_march2d<STAGE, U>), grid, blk, lds, s, a, j, emax, 1);
+      });
     } else if (pass_kind == 1) {
       const size_t lds = pass2d_lds_bytes(cpt, nmax);
       const int emax = FBS * cpt + 2 * nmax;
@@ -3216,11 +3319,17 @@ struct nls_batch {
     hipLaunchKernelGGL(k_kge_small, dim3(nt * NSEQ), dim3(kWave), 0, st, a, m - 2);
     if (snapshot) {
       SnapArgs sn{d_snapU + (size_t)slot * lay.cells, d_snapV + (size_t)slot * lay.cells};
-      if (p.reorth) hipLaunchKernelGGL((k_kge_combine_snap<DIM, true>), grid, blk, 0, st, a, sn);
-      else hipLaunchKernelGGL((k_kge_combine_snap<DIM, false>), grid, blk, 0, st, a, sn);
+      with_uni([&](auto u) {
+        constexpr bool U = decltype(u)::value;
+        if (p.reorth) hipLaunchKernelGGL((k_kge_combine_snap<DIM, true, U>), grid, blk, 0, st, a, sn);
+        else hipLaunchKernelGGL((k_kge_combine_snap<DIM, false, U>), grid, blk, 0, st, a, sn);
+      });
     } else {
-      if (p.reorth) hipLaunchKernelGGL((k_kge_combine_rt<DIM, true>), grid, blk, 0, st, a);
-      else hipLaunchKernelGGL((k_kge_combine_rt<DIM, false>), grid, blk, 0, st, a);
+      with_uni([&](auto u) {
+        constexpr bool U = decltype(u)::value;
+        if (p.reorth) hipLaunchKernelGGL((k_kge_combine_rt<DIM, true, U>), grid, blk, 0, st, a);
+        else hipLaunchKernelGGL((k_kge_combine_rt<DIM, false, U>), grid, blk, 0, st, a);
+      });
     }
     check_launch("fused kge step");
   }
@@ -3287,12 +3396,17 @@ struct nls_batch {
   int march_slots(int kind) const {
     const int nm = nmax_of();
     int per_cu = 0, dev = 0, ncu = 0;
+    const size_t lds2 = march2d_lds_bytes(nm, uni_all);
     if (kind == 2 && nlse())
-      NLS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_nlse_march2d<2>, FBS,
-                                                           march2d_lds_bytes(nm)));
+      with_uni([&](auto u) {
+        NLS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, k_nlse_march2d<2, decltype(u)::value>, FBS, lds2));
+      });
     else if (kind == 2)
-      NLS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_kge_march2d<2>, FBS,
-                                                           march2d_lds_bytes(nm)));
+      with_uni([&](auto u) {
+        NLS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, k_kge_march2d<2, decltype(u)::value>, FBS, lds2));
+      });
     else if (kind == 3)
       NLS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_kge_rmarch2d<3>, FBS,
                                                            rmarch2d_lds_bytes(nm)));
@@ -3398,6 +3512,11 @@ struct nls_batch {
         autotune();
       }
       if (p.use_graph) capture_graphs();
+      recapture = false;
+    }
+    if (recapture) {
+      if (p.use_graph) capture_graphs();
+      recapture = false;
     }
     hipEvent_t e0, e1;
     NLS_HIP(hipEventCreate(&e0));
@@ -3452,7 +3571,13 @@ struct nls_batch {
     // w'_{j-1}, v_{j-1} for both sequences and c (4 * 5 B), write v_j, w'_j for both
     // (4 * 4 B).  Recomputed w' (kind 3): read v_{j-1}, v_{j-2} for both sequences
     // and c (4 * 5 B), write v_j for both (4 * 2 B).
-    return (pass_kind == 3 ? 28.0 : 36.0) * (double)total_cells();
+    // With uniform fields (row and plane marches, UNI variants) the 4 B of c are not read.
+    double b = 0.0;
+    for (int t = 0; t < ntraj; ++t) {
+      const bool cu = uni_all && (pass_kind == 2 || pass_kind == 4);
+      b += ((pass_kind == 3 ? 28.0 : 36.0) - (cu ? 4.0 : 0.0)) * (double)lay.traj[t].N;
+    }
+    return b;
   }
   int64_t total_cells() const {
     int64_t c = 0;
@@ -3605,6 +3730,7 @@ int nls_batch_set_initial(nls_batch *b, int t, const double *m, const double *c,
       }
       NLS_HIP(hipMemcpyAsync(b->d_mf + T.off, mf.data(), sizeof(float) * N, hipMemcpyHostToDevice, b->s));
       NLS_HIP(hipMemcpyAsync(b->d_cf + T.off, cf.data(), sizeof(float) * N, hipMemcpyHostToDevice, b->s));
+      b->set_uniform(t, mf.data(), cf.data(), N);
       NLS_HIP(hipMemcpyAsync(b->d_state[0] + T.off, psi.data(), sizeof(float) * 2 * N,
                              hipMemcpyHostToDevice, b->s));
       if (b->p.max_snapshots > 0) {
@@ -3624,6 +3750,7 @@ int nls_batch_set_initial(nls_batch *b, int t, const double *m, const double *c,
     }
     NLS_HIP(hipMemcpyAsync(b->d_mf + T.off, mf.data(), sizeof(float) * N, hipMemcpyHostToDevice, b->s));
     NLS_HIP(hipMemcpyAsync(b->d_cf + T.off, cf.data(), sizeof(float) * N, hipMemcpyHostToDevice, b->s));
+    b->set_uniform(t, mf.data(), cf.data(), N);
     NLS_HIP(hipMemcpyAsync(b->d_state[0] + T.off, u0, sizeof(double) * N, hipMemcpyHostToDevice, b->s));
     NLS_HIP(hipMemcpyAsync(b->d_state[2] + T.off, upast.data(), sizeof(double) * N,
                            hipMemcpyHostToDevice, b->s));
@@ -3676,6 +3803,8 @@ int nls_batch_set_initial_sampled(nls_batch *b, int t, int kind, uint64_t seed, 
                        (float)m_const, (float)c_const, u, up, psi, b->d_mf + off, b->d_cf + off,
                        su, sv);
     NLS_HIP(hipGetLastError());
+    const float mc = (float)m_const, cc = (float)c_const;
+    b->set_uniform(t, &mc, &cc, 1);
     if (keep) b->nsnap = std::max(b->nsnap, 1);
     NLS_HIP(hipStreamSynchronize(b->s));
   });

@@ -27,7 +27,8 @@ class FusedBatch:
     def __init__(self, ns_list, Ls, dt, equation="kge", method="gautschi", dim=2, krylov_dim=0,
                  mu2=0.0, q6=True, gamma=0.0, use_graph=True, snapshot_every=0,
                  max_snapshots=0, sigmas=None, seeds=None, cells_per_thread=0,
-                 pass_kernel=None, lanes=0, reorth=0, reorth_tol=0.0):
+                 pass_kernel=None, lanes=0, reorth=0, reorth_tol=0.0,
+                 always_load_fields=False):
         eq = {"kge": L.EQ_KGE, "nlse": L.EQ_NLSE, "sp4": L.EQ_SP4}[equation]
         meth = {"gautschi": L.KGE_GAUTSCHI, "sewi": L.KGE_SEWI, "sv": L.KGE_SV,
                 "strang": L.NLSE_STRANG, "reference": L.SP4_REFERENCE,
@@ -43,7 +44,7 @@ class FusedBatch:
                                snapshot_every, max_snapshots, cells_per_thread,
                                0 if pass_kernel is None else pass_kernel + 1, lanes,
                                {"never": 0, "always": 1, "auto": 2}.get(reorth, reorth),
-                               reorth_tol)
+                               reorth_tol, int(always_load_fields))
         self.td = (L.TrajDesc * k)(*[L.TrajDesc(n, l, s, sd)
                                      for n, l, s, sd in zip(self.ns, Ls, sigmas, seeds)])
         self.h = C.c_void_p()

@@ -555,3 +555,65 @@ def test_fused_nlse_3d_vs_oracle(orc, kernel, cpt, q6):
                                            q6=q6)
         out, _ = b.snapshots(t)
         assert rel_l2(out[1], _ORACLE_3D[key][1]) < 1e-4, (t, n)
+
+
+# uniform-field path of the row march (2D) and plane march (3D), KGE and NLSE
+@pytest.mark.parametrize("equation,dim,kernel,cpt", [("kge", 2, 2, 16), ("kge", 3, 4, 4),
+                                                     ("nlse", 2, 2, 8), ("nlse", 3, 4, 4)])
+def test_fused_uniform_fields_equal_streamed_bitwise(equation, dim, kernel, cpt):
+    """A trajectory whose m and c are spatially uniform runs with the values from its
+    descriptor instead of streaming the fields: bitwise the result of streaming them
+    (always_load_fields), and its batch-mate with varying fields keeps its own result."""
+    from nls.batch import FusedBatch
+    from nls.samplers import nlse_packets_2d
+    sizes = [40, 56] if dim == 2 else [20, 28]
+    L = 10.0 if dim == 2 else 8.0
+    cases = []
+    for t, n in enumerate(sizes):
+        if equation == "kge":
+            m, c, u0, v0 = make_case(n, 6, t) if dim == 2 else make_case_3d(n, 6, t)
+        else:
+            if dim == 3:
+                m, c, u0 = nlse_case_3d(n, L, t)
+            else:
+                rng = np.random.default_rng(60 + t)
+                m, c = 0.8 + 0.4 * rng.random((n, n)), 0.8 + 0.4 * rng.random((n, n))
+                u0 = nlse_packets_2d(n, L, 1, t)[0]
+            v0 = None
+        if t == 0:  # uniform, values other than 1
+            m = np.full_like(np.real(m), 0.7)
+            c = np.full_like(np.real(c), 1.3)
+        cases.append((m, c, u0, v0))
+    out = []
+    for load in (False, True):
+        kw = dict(equation="nlse", method="strang", krylov_dim=12) if equation == "nlse" else \
+            dict(krylov_dim=8, mu2=1.0)
+        b = FusedBatch(sizes, L, 0.004, dim=dim, cells_per_thread=cpt, pass_kernel=kernel,
+                       always_load_fields=load, **kw)
+        for t, (m, c, u0, v0) in enumerate(cases):
+            if equation == "kge":
+                b.set_initial(t, m, c, u0, v0)
+            else:
+                b.set_initial(t, m, c, u0)
+        b.run(25)
+        assert b.tile()["pass_kind"] == kernel
+        out.append([b.state(t)[0] for t in range(len(sizes))])
+    for t in range(len(sizes)):
+        assert np.array_equal(out[0][t], out[1][t]), t
+
+
+def test_fused_uniform_fields_vs_oracle(orc):
+    """Uniform m = 0.7, c = 1.3 (not 1) through the uniform-field path against the FP64
+    oracle with the same constant fields."""
+    from nls.batch import FusedBatch
+    n, L, dt, nt, ns = 64, 10.0, 0.005, 201, 5
+    _, _, u0, v0 = make_case(n, 2, 1, ring=True)
+    m, c = np.full((n, n), 0.7), np.full((n, n), 1.3)
+    b = FusedBatch([n], L, dt, krylov_dim=8, mu2=1.0, snapshot_every=nt // ns, max_snapshots=ns,
+                   cells_per_thread=16, pass_kernel=2)
+    b.set_initial(0, m, c, u0, v0)
+    b.run(nt - 1)
+    u, _ = b.snapshots(0)
+    ur, _ = orc.kge_run(n, L, dt * nt, nt, ns, m, c, u0, v0, krylov=8, mu2=1.0)
+    for k in range(ns):
+        assert rel_l2(u[k], ur[k]) < TOL, k
