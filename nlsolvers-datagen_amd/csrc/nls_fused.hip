@@ -3644,6 +3644,8 @@ int nls_batch_create(const nls_batch_params *p, int ntraj, const nls_traj_desc *
     NLS_REQUIRE(p->dt > 0, "dt must be positive");
     NLS_REQUIRE(p->pass_kernel >= 0 && p->pass_kernel <= 5, "pass_kernel must be in [0, 5]");
     NLS_REQUIRE(p->reorth >= 0 && p->reorth <= 2, "reorth must be 0, 1 or 2");
+    NLS_REQUIRE(p->always_load_fields == 0 || p->always_load_fields == 1,
+                "always_load_fields must be 0 or 1");
     NLS_REQUIRE(p->reorth == 0 || p->pass_kernel != 4,
                 "reorthogonalisation needs a pass kernel that stores w' (not kernel 3)");
     NLS_REQUIRE(p->lanes >= 0 && p->lanes <= nls_batch::MAXL, "lanes must be in [0, 4]");

@@ -52,7 +52,8 @@ constexpr int SCPT = 1;
 constexpr int NXCD = 8;
 
 struct ReqDev {
-  int n, nchunks, mode, pad;
+  int n, nchunks, mode;
+  int uni; // m and c spatially uniform: mval, cval instead of the fields (the same numbers)
   int64_t N;
   double *u0, *u1, *u2; // state ring
   const float *mf, *cf;
@@ -60,23 +61,27 @@ struct ReqDev {
   uint64_t seed;
   uint32_t stream, pad2;
   double tau, s, gp_inv, gm, t2, namp, s_eff;
+  float mval, cval;
 };
 
+// UNI: the request's m and c are uniform (values from the descriptor, no field loads)
+template <bool UNI>
 __device__ inline double sp4_new(const ReqDev &R, const double *u, const double *up, int64_t g,
                                  int64_t i, uint32_t it) {
   const int n = R.n;
   const int64_t row = i / n, col = i - row * n;
   const double ui = u[g];
-  const double ci = R.cf[g];
+  const double ci = UNI ? (double)R.cval : (double)R.cf[g];
+  auto cn = [&](int64_t o) { return UNI ? ci : (double)R.cf[o]; };
   double lap = 0.0;
   // div(c grad u) with edge weights (c_i + c_j)/2 (laplacians.hpp:75-91), 5-point
-  if (row > 0) lap += 0.5 * (ci + (double)R.cf[g - n]) * (u[g - n] - ui);
-  if (col > 0) lap += 0.5 * (ci + (double)R.cf[g - 1]) * (u[g - 1] - ui);
-  if (col < n - 1) lap += 0.5 * (ci + (double)R.cf[g + 1]) * (u[g + 1] - ui);
-  if (row < n - 1) lap += 0.5 * (ci + (double)R.cf[g + n]) * (u[g + n] - ui);
+  if (row > 0) lap += 0.5 * (ci + cn(g - n)) * (u[g - n] - ui);
+  if (col > 0) lap += 0.5 * (ci + cn(g - 1)) * (u[g - 1] - ui);
+  if (col < n - 1) lap += 0.5 * (ci + cn(g + 1)) * (u[g + 1] - ui);
+  if (row < n - 1) lap += 0.5 * (ci + cn(g + n)) * (u[g + n] - ui);
   lap *= R.s;
   const double z = noise1(R.seed, R.stream, it, (uint64_t)i);
-  const double mv = R.mf[g];
+  const double mv = UNI ? (double)R.mval : (double)R.mf[g];
   if (R.mode == NLS_SP4_REFERENCE) {
     const double nl = -mv * (ui - ui * ui * ui + R.s_eff * z);
     double x = 2.0 * ui - up[g];
@@ -106,7 +111,7 @@ __global__ __launch_bounds__(SBS) void k_sp4_group_step(const ReqDev *__restrict
     if (i >= R.N) break;
     const int64_t r = i / n, c = i - r * n;
     const int64_t src = cl(r) * n + cl(c); // Neumann ghost copy (boundaries.cuh:9-19)
-    un[i] = sp4_new(R, u, up, src, src, it);
+    un[i] = R.uni ? sp4_new<true>(R, u, up, src, src, it) : sp4_new<false>(R, u, up, src, src, it);
   }
 }
 
@@ -114,6 +119,8 @@ struct Request {
   int64_t id;
   nls_request q;
   std::vector<float> mf, cf;
+  bool uni = false; // mf and cf spatially uniform, with the values mval, cval
+  float mval = 1.0f, cval = 1.0f;
   std::vector<double> u0, v0;
   std::vector<double> m64, c64;        // caller's fields, kept for the HDF5 header
   std::shared_ptr<h5::Stream> stream;  // touched by the writer thread only
@@ -335,6 +342,9 @@ struct nls_server {
     d.mf = (const float *)(base + 3 * rr(8 * N));
     d.cf = (const float *)(base + 3 * rr(8 * N) + rr(4 * N));
     d.base_step = r.steps_done;
+    d.uni = r.uni ? 1 : 0;
+    d.mval = r.mval;
+    d.cval = r.cval;
     d.seed = r.q.seed;
     d.stream = 0;
     const double dt = r.q.dt, dx = 2.0 * p.L / (r.q.n - 1);
@@ -533,9 +543,15 @@ int nls_server_submit(nls_server *sv, const nls_request *q, const double *m, con
                 "request needs more device memory than the whole serving budget");
     r->mf.resize(r->N);
     r->cf.resize(r->N);
+    r->uni = true;
     for (int64_t i = 0; i < r->N; ++i) {
       r->mf[i] = m ? (float)m[i] : 1.0f;
       r->cf[i] = c ? (float)c[i] : 1.0f;
+      r->uni = r->uni && r->mf[i] == r->mf[0] && r->cf[i] == r->cf[0];
+    }
+    if (r->N > 0) {
+      r->mval = r->mf[0];
+      r->cval = r->cf[0];
     }
     if (!sv->h5_dir.empty()) {
       r->m64.assign(r->N, 1.0);

@@ -561,31 +561,33 @@ def test_fused_nlse_3d_vs_oracle(orc, kernel, cpt, q6):
 @pytest.mark.parametrize("equation,dim,kernel,cpt", [("kge", 2, 2, 16), ("kge", 3, 4, 4),
                                                      ("nlse", 2, 2, 8), ("nlse", 3, 4, 4)])
 def test_fused_uniform_fields_equal_streamed_bitwise(equation, dim, kernel, cpt):
-    """A trajectory whose m and c are spatially uniform runs with the values from its
-    descriptor instead of streaming the fields: bitwise the result of streaming them
-    (always_load_fields), and its batch-mate with varying fields keeps its own result."""
+    """A batch whose trajectories all have spatially uniform m and c runs the kernel
+    variants that take the values from the descriptor instead of streaming the fields:
+    bitwise the result of streaming them (always_load_fields), and bitwise the result the
+    same trajectory gets in a batch with a varying-field mate (which streams the fields;
+    a fixed tile makes a trajectory's bits independent of its batch-mates)."""
     from nls.batch import FusedBatch
     from nls.samplers import nlse_packets_2d
     sizes = [40, 56] if dim == 2 else [20, 28]
     L = 10.0 if dim == 2 else 8.0
-    cases = []
-    for t, n in enumerate(sizes):
+
+    def case(t):
+        n = sizes[t]
         if equation == "kge":
             m, c, u0, v0 = make_case(n, 6, t) if dim == 2 else make_case_3d(n, 6, t)
+        elif dim == 3:
+            (m, c, u0), v0 = nlse_case_3d(n, L, t), None
         else:
-            if dim == 3:
-                m, c, u0 = nlse_case_3d(n, L, t)
-            else:
-                rng = np.random.default_rng(60 + t)
-                m, c = 0.8 + 0.4 * rng.random((n, n)), 0.8 + 0.4 * rng.random((n, n))
-                u0 = nlse_packets_2d(n, L, 1, t)[0]
-            v0 = None
-        if t == 0:  # uniform, values other than 1
-            m = np.full_like(np.real(m), 0.7)
-            c = np.full_like(np.real(c), 1.3)
-        cases.append((m, c, u0, v0))
-    out = []
-    for load in (False, True):
+            rng = np.random.default_rng(60 + t)
+            m, c = 0.8 + 0.4 * rng.random((n, n)), 0.8 + 0.4 * rng.random((n, n))
+            u0, v0 = nlse_packets_2d(n, L, 1, t)[0], None
+        return m, c, u0, v0
+
+    varying = [case(t) for t in range(2)]
+    uniform = [(np.full(np.shape(m), mv), np.full(np.shape(c), cv), u0, v0)
+               for (m, c, u0, v0), mv, cv in zip(varying, (0.7, 1.1), (1.3, 0.9))]
+
+    def run(cases, load):
         kw = dict(equation="nlse", method="strang", krylov_dim=12) if equation == "nlse" else \
             dict(krylov_dim=8, mu2=1.0)
         b = FusedBatch(sizes, L, 0.004, dim=dim, cells_per_thread=cpt, pass_kernel=kernel,
@@ -597,9 +599,15 @@ def test_fused_uniform_fields_equal_streamed_bitwise(equation, dim, kernel, cpt)
                 b.set_initial(t, m, c, u0)
         b.run(25)
         assert b.tile()["pass_kind"] == kernel
-        out.append([b.state(t)[0] for t in range(len(sizes))])
+        return [b.state(t)[0] for t in range(len(sizes))]
+
+    uni = run(uniform, False)       # uniform-field variants
+    streamed = run(uniform, True)   # the same fields, streamed
+    mixed = run([uniform[0], varying[1]], False)  # a varying mate: streamed variants
     for t in range(len(sizes)):
-        assert np.array_equal(out[0][t], out[1][t]), t
+        assert np.array_equal(uni[t], streamed[t]), t
+    assert np.array_equal(uni[0], mixed[0])
+    assert not np.array_equal(uni[1], mixed[1])  # the mate's own fields took effect
 
 
 def test_fused_uniform_fields_vs_oracle(orc):

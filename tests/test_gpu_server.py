@@ -48,6 +48,38 @@ def test_server_mix_equals_alone_and_oracle(orc):
     assert st["peak_active"] >= 2
 
 
+def test_server_fields_uniform_and_varying_vs_oracle(orc):
+    """Requests with uniform fields (m = c = 1 by default, or other constants) take the
+    values from the request descriptor; a request with varying fields streams them.  In
+    one mix, each agrees with the FP64 oracle on its own fields.  The serving state keeps m
+    and c in FP32, so the fields here are FP32 numbers (the oracle then sees the same
+    values)."""
+    from nls.samplers import gaussian_sum_2d
+    from nls.server import TrajectoryServer
+    dt, L, nt = 0.005, 10.0, 120
+    rng = np.random.default_rng(3)
+    f32 = lambda a: a.astype(np.float32).astype(np.float64)
+    cases = [(48, None, None), (40, np.full((40, 40), 0.75), np.full((40, 40), 1.25)),
+             (56, f32(0.8 + 0.4 * rng.random((56, 56))), f32(0.8 + 0.4 * rng.random((56, 56))))]
+    srv = TrajectoryServer(hbm_budget_bytes=64 << 20, max_active=8, L_=L, gamma=0.1,
+                           max_steps_per_group=16)
+    ids = []
+    for k, (n, m, c) in enumerate(cases):
+        u0, _, _ = gaussian_sum_2d(n, L, 20 + k, 0)
+        ids.append((srv.submit(n, nt, dt, 0.2, 30 + k, num_snapshots=3, m=m, c=c, u0=u0,
+                               v0=0.1 * u0), u0))
+    srv.drain(300)
+    for k, ((n, m, c), (rid, u0)) in enumerate(zip(cases, ids)):
+        got = srv.fetch(rid)
+        mm = np.ones((n, n)) if m is None else m
+        cc = np.ones((n, n)) if c is None else c
+        ou, _ = orc.sp4_run(n, L, dt * nt, nt, 3, mm, cc, u0, 0.1 * u0, mode=1, gamma=0.1,
+                            sigma=0.2, seed=30 + k, stream=0)
+        for j in range(3):
+            assert rel_l2(got["snapshots"][j], ou[j]) < 1e-10, (n, j)
+    srv.close()
+
+
 def test_server_budget_limits_concurrency():
     from nls.server import TrajectoryServer
     # each 64^2 request needs 3*32 KiB + 2*16 KiB = 128 KiB; a 300 KiB budget admits two
