@@ -61,6 +61,7 @@ struct TrajDev {
   int64_t off, N;
   float s, mu2;
   double tau;
+  double sd; // FP64 1 / dx^2 (Stormer-Verlet step)
   // spatially uniform coefficient fields (bit 0: c, bit 1: m) and their FP32 values; a
   // batch whose trajectories all have both runs the UNI kernel variants, which use
   // cval / mval instead of streaming the fields (the same numbers)
@@ -1692,6 +1693,120 @@ __global__ __launch_bounds__(FBS) void k_kge_combine_snap(KgeArgs a, SnapArgs sn
   }
 }
 
+// ---------------------------------------------------------------- Stormer-Verlet step
+// kg_single.cuh:129-179 on the batch, FP64 throughout:
+//   u+ = ((2u - u-) + tau^2 (L_c - mu2) u) + tau^2 (-m u^3),   v = (u+ - u) / tau,
+// with the operator row formed as the reference assembles it (laplacians.hpp:54-103 in 2D,
+// :158-218 in 3D incl. the Q6 couplings: weights (c_a + c_b) / 2, diagonal -(sum in the
+// loop order) times 1/dx^2, then the shift) and summed in ascending column order as the CSR
+// SpMV sums it, without contraction, so the result is the reference arithmetic's.  The
+// ghost ring takes its source cell's u+ (the combine's clamp); the velocity snapshot is
+// taken before the copy.  UNI: m and c from the descriptor.
+template <int DIM, bool UNI>
+__global__ __launch_bounds__(FBS) void k_kge_sv(KgeArgs a, SnapArgs snap) {
+#pragma clang fp contract(off)
+  const int2 cm = a.cmap[blockIdx.x];
+  if (cm.x < 0) return;
+  const TrajDev T = a.traj[cm.x];
+  // 32-bit cell indices and quotients from a float reciprocal corrected to the exact
+  // value (row_of; every trajectory of the batch holds fewer than 2^24 cells)
+  const int n = T.n, n2 = n * n, N = (int)T.N;
+  const float rn = 1.0f / (float)n, rn2 = 1.0f / (float)n2;
+  const double *u = a.u + T.off;
+  const float *cT = a.cf + T.off;
+  const double sc = T.sd, shift = -(double)T.mu2, t2 = T.tau * T.tau;
+  auto C = [&](int r) { return UNI ? (double)T.cval : (double)cT[r]; };
+  auto unew = [&](int r) {
+    const double cr = C(r);
+    double acc = 0.0;
+    if (DIM == 2) {
+      const int i = row_of(r, n, rn), j = r - i * n;
+      const bool hl = j > 0, hr = j < n - 1, hu = i > 0, hd = i < n - 1;
+      double wl = 0.0, wr = 0.0, wu = 0.0, wd = 0.0, dsum = 0.0;
+      if (hl) wl = (C(r - 1) + cr) / 2.0;
+      if (hr) wr = (cr + C(r + 1)) / 2.0;
+      if (hu) wu = (C(r - n) + cr) / 2.0;
+      if (hd) wd = (cr + C(r + n)) / 2.0;
+      if (hl) dsum += wl;
+      if (hr) dsum += wr;
+      if (hu) dsum += wu;
+      if (hd) dsum += wd;
+      if (hu) acc += (wu * sc) * u[r - n];
+      if (hl) acc += (wl * sc) * u[r - 1];
+      acc += ((-dsum * sc) + shift) * u[r];
+      if (hr) acc += (wr * sc) * u[r + 1];
+      if (hd) acc += (wd * sc) * u[r + n];
+    } else {
+      const int jp = r - row_of(r, n2, rn2) * n2, i = r - row_of(r, n, rn) * n;
+      const bool hxm = i > 0, hxp = i < n - 1;
+      const bool hym = a.q6 ? r >= n : jp >= n, hyp = a.q6 ? r + n < N : jp < n2 - n;
+      const bool hzm = r >= n2, hzp = r + n2 < N;
+      double wxm = 0.0, wxp = 0.0, wym = 0.0, wyp = 0.0, wzm = 0.0, wzp = 0.0, dsum = 0.0;
+      if (hxm) wxm = (C(r - 1) + cr) / 2.0;
+      if (hxp) wxp = (cr + C(r + 1)) / 2.0;
+      if (hym) wym = (C(r - n) + cr) / 2.0;
+      if (hyp) wyp = (cr + C(r + n)) / 2.0;
+      if (hzm) wzm = (C(r - n2) + cr) / 2.0;
+      if (hzp) wzp = (cr + C(r + n2)) / 2.0;
+      if (hxm) dsum += wxm;
+      if (hxp) dsum += wxp;
+      if (hym) dsum += wym;
+      if (hyp) dsum += wyp;
+      if (hzm) dsum += wzm;
+      if (hzp) dsum += wzp;
+      if (hzm) acc += (wzm * sc) * u[r - n2];
+      if (hym) acc += (wym * sc) * u[r - n];
+      if (hxm) acc += (wxm * sc) * u[r - 1];
+      acc += ((-dsum * sc) + shift) * u[r];
+      if (hxp) acc += (wxp * sc) * u[r + 1];
+      if (hyp) acc += (wyp * sc) * u[r + n];
+      if (hzp) acc += (wzp * sc) * u[r + n2];
+    }
+    const double ur = u[r];
+    const double mm = UNI ? (double)T.mval : (double)a.mf[T.off + r];
+    const double g = -mm * ur * ur * ur;
+    double x = 2.0 * ur - a.up[T.off + r];
+    x = x + t2 * acc;
+    return x + t2 * g;
+  };
+  const int cbeg = cm.y * T.range;
+  const int cend = cbeg + T.range < N ? cbeg + T.range : N;
+  auto cl = [n](int x) { return x < 1 ? 1 : (x > n - 2 ? n - 2 : x); };
+  auto source = [&](int i) { // Neumann ghost copy (boundaries.cuh:9-81), as clamp_src
+    if (DIM == 2) {
+      const int r = row_of(i, n, rn);
+      return cl(r) * n + cl(i - r * n);
+    }
+    const int z = row_of(i, n2, rn2), rem = i - z * n2, y = row_of(rem, n, rn);
+    return (cl(z) * n + cl(y)) * n + cl(rem - y * n);
+  };
+  // G cells per thread at a time, every load before the stores (independent loads)
+  constexpr int G = 4;
+  for (int i0 = cbeg + (int)threadIdx.x; i0 < cend; i0 += G * FBS) {
+    double us[G], ui[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+      const int i = i0 + k * FBS;
+      if (i < cend) {
+        const int src = source(i);
+        us[k] = unew(src);
+        if (snap.u) ui[k] = src == i ? us[k] : unew(i);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+      const int i = i0 + k * FBS;
+      if (i < cend) {
+        a.un[T.off + i] = us[k];
+        if (snap.u) {
+          snap.u[T.off + i] = us[k];
+          snap.v[T.off + i] = (ui[k] - u[i]) / T.tau;
+        }
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------- full reorthogonalisation
 // Classical Gram-Schmidt of the Lanczos residual against the earlier basis vectors, with
 // FP64 dot products and coefficients (the reference makes one such pass in every
@@ -1949,6 +2064,7 @@ static Layout make_layout(const nls_traj_desc *td, int ntraj, int dim, int cpt, 
     L.total_chunks += T.nchunks;
     const double dx = 2.0 * td[t].L / (T.n - 1);
     T.s = (float)(1.0 / (dx * dx));
+    T.sd = 1.0 / (dx * dx);
     T.mu2 = (float)mu2;
     T.tau = dt;
   }
@@ -3014,6 +3130,7 @@ struct nls_batch {
   }
 
   bool nlse() const { return p.equation == NLS_EQ_NLSE; }
+  bool sv() const { return p.equation == NLS_EQ_KGE && p.method == NLS_KGE_SV; }
   int nstates() const { return nlse() ? 2 : 3; }
 
   NlseArgs nargs(int cur) const {
@@ -3367,6 +3484,19 @@ struct nls_batch {
       }
       return;
     }
+    if (sv()) {
+      KgeArgs a = args(cur);
+      SnapArgs sn{nullptr, nullptr};
+      if (snapshot) sn = {d_snapU + (size_t)slot * lay.cells, d_snapV + (size_t)slot * lay.cells};
+      const dim3 grid((unsigned)lay.cmap.size()), blk(FBS);
+      with_uni([&](auto u) {
+        constexpr bool U = decltype(u)::value;
+        if (p.dim == 2) hipLaunchKernelGGL((k_kge_sv<2, U>), grid, blk, 0, s, a, sn);
+        else hipLaunchKernelGGL((k_kge_sv<3, U>), grid, blk, 0, s, a, sn);
+      });
+      check_launch("fused kge sv step");
+      return;
+    }
     if (p.dim == 2) launch_step_t<2>(cur, snapshot, slot, prof);
     else launch_step_t<3>(cur, snapshot, slot, prof);
   }
@@ -3501,6 +3631,16 @@ struct nls_batch {
   }
 
   void run(int nsteps, float *ms) {
+    if (!tuned && sv()) {
+      // one flat-range kernel per step (chunks of 256 x tile cells, one stream), launched
+      // directly: for a single kernel a graph replay costs more than the launch
+      // (71.8 against 65.1 us per step on 100 x 300^2)
+      pass_kind = 0;
+      lanes = 1;
+      set_layout(p.cells_per_thread > 0 ? p.cells_per_thread : 16);
+      tuned = true;
+      recapture = false;
+    }
     if (!tuned) {
       if (p.cells_per_thread > 0) {
         const int c = p.cells_per_thread;
@@ -3517,7 +3657,7 @@ struct nls_batch {
       recapture = false;
     }
     if (recapture) {
-      if (p.use_graph) capture_graphs();
+      if (p.use_graph && !sv()) capture_graphs();
       recapture = false;
     }
     hipEvent_t e0, e1;
@@ -3637,10 +3777,17 @@ int nls_batch_create(const nls_batch_params *p, int ntraj, const nls_traj_desc *
     NLS_REQUIRE(p->equation == NLS_EQ_KGE || p->equation == NLS_EQ_NLSE,
                 "nls_batch_create: the fused batch runs Klein-Gordon and NLSE; stochastic phi^4 "
                 "is served by nls_server");
-    if (p->equation == NLS_EQ_KGE)
-      NLS_REQUIRE(p->method == NLS_KGE_GAUTSCHI || p->method == NLS_KGE_SEWI,
-                  "fused Klein-Gordon batch supports the Gautschi and sEWI methods");
-    else
+    if (p->equation == NLS_EQ_KGE) {
+      NLS_REQUIRE(p->method == NLS_KGE_GAUTSCHI || p->method == NLS_KGE_SEWI ||
+                      p->method == NLS_KGE_SV,
+                  "fused Klein-Gordon batch supports the Gautschi, sEWI and Stormer-Verlet methods");
+      NLS_REQUIRE(p->method != NLS_KGE_SV || (p->reorth == 0 && p->pass_kernel == 0),
+                  "the Stormer-Verlet batch has no Krylov passes (reorth and pass_kernel 0)");
+      for (int t = 0; t < ntraj && p->method == NLS_KGE_SV; ++t)
+        NLS_REQUIRE((p->dim == 3 ? (int64_t)trajs[t].n * trajs[t].n * trajs[t].n
+                                 : (int64_t)trajs[t].n * trajs[t].n) < (1 << 24),
+                    "Stormer-Verlet batch: a trajectory needs fewer than 2^24 cells");
+    } else
       NLS_REQUIRE(p->method == NLS_NLSE_STRANG, "fused NLSE batch supports the Strang method");
     NLS_REQUIRE(p->dim == 2 || p->dim == 3, "dim must be 2 or 3");
     NLS_REQUIRE(p->dt > 0, "dt must be positive");
@@ -3659,7 +3806,7 @@ int nls_batch_create(const nls_batch_params *p, int ntraj, const nls_traj_desc *
     b->p = *p;
     b->ntraj = ntraj;
     b->desc.assign(trajs, trajs + ntraj);
-    b->m = pick_m(*p, ntraj, trajs);
+    b->m = b->sv() ? 3 : pick_m(*p, ntraj, trajs);
     NLS_REQUIRE(b->m >= 3 && b->m <= kMaxKrylov, "fused path needs a Krylov dimension in [3, 32]");
     NLS_HIP(hipStreamCreate(&b->s));
     b->sl[0] = b->s;
@@ -3697,8 +3844,10 @@ int nls_batch_create(const nls_batch_params *p, int ntraj, const nls_traj_desc *
     b->d_cf = b->alloc<float>(cells);
     const int nst = b->nstates();
     for (int k = 0; k < nst; ++k) b->d_state[k] = b->alloc<double>(cells);
-    b->d_V = b->alloc<float>((size_t)cells * NSEQ * (b->m - 2));
-    b->d_W = b->alloc<float>((size_t)cells * NSEQ * 2);
+    if (!b->sv()) { // Krylov basis and w' (none for Stormer-Verlet)
+      b->d_V = b->alloc<float>((size_t)cells * NSEQ * (b->m - 2));
+      b->d_W = b->alloc<float>((size_t)cells * NSEQ * 2);
+    }
     if (b->nlse()) b->d_X = b->alloc<float>((size_t)cells * 2);
     if (p->max_snapshots > 0) {
       // KGE: u and v (FP64); NLSE: psi (complex128) in d_snapU

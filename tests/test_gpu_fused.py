@@ -625,3 +625,38 @@ def test_fused_uniform_fields_vs_oracle(orc):
     ur, _ = orc.kge_run(n, L, dt * nt, nt, ns, m, c, u0, v0, krylov=8, mu2=1.0)
     for k in range(ns):
         assert rel_l2(u[k], ur[k]) < TOL, k
+
+
+def _f32(a):
+    return np.asarray(a, np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("dim,q6", [(2, True), (3, True), (3, False)])
+def test_fused_kge_sv_bitwise_vs_oracle(orc, dim, q6):
+    """The Stormer-Verlet batch (kg_single.cuh:129-179) runs the reference arithmetic in
+    FP64: each operator row formed as the reference assembles it and summed in CSR column
+    order without contraction.  With FP32-representable m and c (the batch keeps the
+    fields in FP32) every snapshot of u and v is bitwise the oracle's, for an uneven batch
+    with varying fields and for a trajectory with uniform ones (the uniform-field kernel)."""
+    from nls.batch import FusedBatch
+    if dim == 2:
+        sizes, L, dt, nt, ns = [48, 33, 64], 10.0, 0.002, 201, 5
+        cases = [make_case(n, 8, t) for t, n in enumerate(sizes)]
+    else:
+        sizes, L, dt, nt, ns = [20, 17], 8.0, 0.002, 61, 3
+        cases = [make_case_3d(n, 8, t) for t, n in enumerate(sizes)]
+    cases = [(_f32(m), _f32(c), u0, v0) for m, c, u0, v0 in cases]
+    uniform = [(np.full_like(m, 0.75), np.full_like(c, 1.25), u0, v0) for m, c, u0, v0 in cases]
+    for batch in (cases, uniform):
+        b = FusedBatch(sizes, L, dt, method="sv", dim=dim, mu2=1.0, q6=q6,
+                       snapshot_every=nt // ns, max_snapshots=ns)
+        for t, cs in enumerate(batch):
+            b.set_initial(t, *cs)
+        b.run(nt - 1)
+        for t, n in enumerate(sizes):
+            m, c, u0, v0 = batch[t]
+            ur, vr = orc.kge_run(n, L, dt * nt, nt, ns, m, c, u0, v0, method=1, mu2=1.0,
+                                 dim=dim, q6=q6)
+            u, v = b.snapshots(t)
+            assert np.array_equal(u, ur), (t, np.abs(u - ur).max())
+            assert np.array_equal(v, vr), (t, np.abs(v - vr).max())
