@@ -3781,8 +3781,8 @@ int nls_batch_create(const nls_batch_params *p, int ntraj, const nls_traj_desc *
       NLS_REQUIRE(p->method == NLS_KGE_GAUTSCHI || p->method == NLS_KGE_SEWI ||
                       p->method == NLS_KGE_SV,
                   "fused Klein-Gordon batch supports the Gautschi, sEWI and Stormer-Verlet methods");
-      NLS_REQUIRE(p->method != NLS_KGE_SV || (p->reorth == 0 && p->pass_kernel == 0),
-                  "the Stormer-Verlet batch has no Krylov passes (reorth and pass_kernel 0)");
+      NLS_REQUIRE(p->method != NLS_KGE_SV || (p->reorth == 0 && p->pass_kernel <= 1),
+                  "the Stormer-Verlet batch has no Krylov passes (reorth 0, pass_kernel 0 or 1)");
       for (int t = 0; t < ntraj && p->method == NLS_KGE_SV; ++t)
         NLS_REQUIRE((p->dim == 3 ? (int64_t)trajs[t].n * trajs[t].n * trajs[t].n
                                  : (int64_t)trajs[t].n * trajs[t].n) < (1 << 24),

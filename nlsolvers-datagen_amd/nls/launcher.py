@@ -46,7 +46,7 @@ class LaunchConfig:
     m_classes: List[str] = field(default_factory=lambda: ["constant"])
     c_classes: List[str] = field(default_factory=lambda: ["constant"])
     phenomenon: str = "gaussian"
-    integrator: str = ""          # gautschi | sewi (KGE), strang (NLSE), damped | reference (phi^4)
+    integrator: str = ""          # gautschi | sewi | sv (KGE), strang (NLSE), damped | reference (phi^4)
     krylov_dim: int = 0           # 0: autotuner
     mu2: float = 0.0
     gamma: float = 0.1
