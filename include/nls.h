@@ -227,7 +227,9 @@ NLS_API int nls_stream_copy_bench(size_t bytes, int reps, double *gbps, double *
 /* ---------------------------------------------------------------- fused batched path */
 /* Many trajectories (uneven grids allowed) advanced together by the fused
  * FP32 matrix-free stencil+Lanczos kernels with FP64 state and FP64 small
- * stage; steps captured into HIP graphs.  No reference counterpart as an
+ * stage; steps captured into HIP graphs.  Klein-Gordon with NLS_KGE_SV runs the
+ * Stormer-Verlet step instead (FP64, the reference's arithmetic, no Krylov
+ * passes: reorth 0, pass_kernel 0 or 1).  No reference counterpart as an
  * object: it replaces running one kg_gautschi_2d_dev / nlse_2d_dev / sp4 process
  * per trajectory (scripts/real_launcher_2d.py:260-306). */
 enum { NLS_EQ_KGE = 0, NLS_EQ_NLSE = 1, NLS_EQ_SP4 = 2 };

@@ -84,7 +84,8 @@ def test_batch_parameter_checks_fail_before_any_device_call():
     from nls._lib import NLSError
     from nls.batch import FusedBatch
     bad = [dict(reorth=3), dict(reorth="always", pass_kernel=3), dict(pass_kernel=5),
-           dict(lanes=5), dict(cells_per_thread=2), dict(always_load_fields=2)]
+           dict(lanes=5), dict(cells_per_thread=2), dict(always_load_fields=2),
+           dict(method="sv", reorth="always"), dict(method="sv", pass_kernel=2)]
     for kw in bad:
         with pytest.raises(NLSError):
             FusedBatch([32], 10.0, 0.005, krylov_dim=8, **kw)
