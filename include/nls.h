@@ -288,8 +288,8 @@ NLS_API int nls_batch_set_initial(nls_batch *b, int t, const double *m, const do
                                   const double *u0, const double *v0);
 /* Seeded initial-condition samplers of the benchmark configurations (the Python
  * samplers nls/samplers.py gaussian_sum_2d, kink_ring_2d, nlse_packets_2d, bubbles_3d;
- * field forms from util.hpp:94-125, real_sampler.py:464-467, nlse_sampler.py:43-100,
- * real_sampler.py:1775-1792).  Draw s of trajectory traj is Philox4x32-10 at counter
+ * field forms from util.hpp:94-125 (Gaussians, 2D and 3D), real_sampler.py:464-467
+ * (kink ring), nlse_sampler.py:43-100 (packets)).  Draw s of trajectory traj is Philox4x32-10 at counter
  * (s, traj, tag, 0) under the 64-bit seed. */
 enum { NLS_IC_GAUSSIANS_2D = 0, NLS_IC_KINK_RING_2D = 1, NLS_IC_PACKETS_2D = 2,
        NLS_IC_BUBBLES_3D = 3 };

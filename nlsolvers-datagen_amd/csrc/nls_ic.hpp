@@ -5,7 +5,7 @@
 //   kink ring 2D  u0 = tanh((R - r) / w)                               (real_sampler.py:464-467)
 //   packets 2D    psi0 = sum_k A_k exp(-|x - x_k|^2) exp(i(v_k.(x - x_k) + phi_k))
 //                                                                      (nlse_sampler.py:43-100)
-//   bubbles 3D    u0 = sum_k A_k exp(-|x - x_k|^2 / w_k^2)             (real_sampler.py:1775-1792)
+//   bubbles 3D    u0 = sum_k A_k exp(-|x - x_k|^2 / w_k^2)             (the Gaussian of util.hpp:94-125)
 // on the grid linspace(-L, L, n) with "ij" indexing (real_sampler.py:17-25); v0 = 0.
 //
 // Parameters are drawn on the host by ParamStream: draw s of trajectory t is the first

@@ -135,7 +135,7 @@ def m_field(kind, n, L, seed, traj, c=None, m0=1.0):
              "square": np.maximum(np.abs(X), np.abs(Y)) - bp * L,
              "horizontal": Y, "vertical": X, "diagonal": X + Y}[bt]
         return m0 + (m2 - m0) * 0.5 * (1 + np.tanh(b / (sw * L))), \
-            {"type": kind, "m2": m2, "boundary_type": bt, "boundary_param": bp}
+            {"type": kind, "m2": m2, "boundary_type": bt, "boundary_param": bp, "smooth_width": sw}
     if kind == "gradient":
         gam, eps = ps.uniform(0.5, 2.0), ps.uniform(1e-3, 1e-1)
         gx, gy = np.zeros_like(c), np.zeros_like(c)

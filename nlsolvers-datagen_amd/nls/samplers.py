@@ -6,7 +6,8 @@ configurations state:
   kink_ring_2d      tanh((R - r)/w)                        (real_sampler.py:464-467)
   nlse_packets_2d   Gaussian packets A exp(-|x-x0|^2/w^2) exp(i(v.(x-x0) + phi))
                                                            (nlse_sampler.py:43-100 form)
-  bubbles_3d        sum of 3D Gaussian bubbles             (real_sampler.py:1775-1792)
+  bubbles_3d        sum of 3D Gaussian bubbles A exp(-|x-x0|^2/w^2)  (util.hpp:94-125
+                    profile; the configurations' 3D Gaussian bubbles)
 Grids are linspace(-L, L, n) with indexing="ij" (real_sampler.py:17-25).
 
 Every random parameter is a Philox4x32-10 draw keyed by the seed, with counter

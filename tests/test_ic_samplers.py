@@ -68,3 +68,31 @@ def test_python_fields_closed_forms():
     assert abs(psi[i, j] - want) < 1e-14
     u3, _ = bubbles_3d(5, 8.0, 3, 1)
     assert u3.shape == (5, 5, 5) and u3.min() >= 0
+
+
+def test_field_forms_match_reference_golden():
+    """The kink ring and the coefficient-field classes whose form is a function of
+    explicit parameters, evaluated at the parameters this library draws, against the
+    reference's own functions at the same parameters (tests/golden/sampler_forms.npz,
+    written by tests/golden/make_sampler_golden.py from real_sampler.py:435-467,
+    c_fields_2d.py:12-52, m_fields_2d.py:14-82)."""
+    import os
+    from nls.fields import c_field, m_field
+    from nls.samplers import kink_ring_2d
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "sampler_forms.npz"))
+    n, L = int(g["n"]), float(g["L"])
+    for s, t in g["seeds"]:
+        s, t = int(s), int(t)
+        tag = f"{s}_{t}"
+        u, v, _ = kink_ring_2d(n, L, s, t)
+        np.testing.assert_allclose(u, g[f"ring_u_{tag}"], rtol=0, atol=1e-15)
+        np.testing.assert_allclose(v, g[f"ring_v_{tag}"], rtol=0, atol=0)
+        for kind in ("constant", "periodic_structure", "piecewise_constant",
+                     "sign_changing_mass"):
+            np.testing.assert_allclose(c_field(kind, n, L, s, t)[0], g[f"c_{kind}_{tag}"],
+                                       rtol=1e-15, atol=1e-15, err_msg=kind)
+        c = g[f"c_for_m_{tag}"]
+        np.testing.assert_array_equal(c, c_field("sign_changing_mass", n, L, s, t)[0])
+        for kind in ("constant", "piecewise", "gradient", "phase", "topological"):
+            np.testing.assert_allclose(m_field(kind, n, L, s, t, c=c)[0], g[f"m_{kind}_{tag}"],
+                                       rtol=1e-14, atol=1e-14, err_msg=kind)
