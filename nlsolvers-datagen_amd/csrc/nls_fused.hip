@@ -1215,7 +1215,7 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax,
       if (!UNI) sc[e] = sc[MSUB + e];
       sv0[e] = sv0[MSUB + e];
       sv1[e] = sv1[MSUB + e];
-      if (STAGE >= 1) {
+      if (STAGE >= 1 && e >= n) { // the previous vector is read at own cells only
         sp0[e] = sp0[MSUB + e];
         sp1[e] = sp1[MSUB + e];
       }
@@ -2575,7 +2575,7 @@ __global__ __launch_bounds__(FBS) void k_nlse_march2d(NlseArgs a, int j, int ema
       if (!UNI) sc[e] = sc[MSUB + e];
       sv0[e] = sv0[MSUB + e];
       sv1[e] = sv1[MSUB + e];
-      if (STAGE >= 1) {
+      if (STAGE >= 1 && e >= n) { // the previous vector is read at own cells only
         sp0[e] = sp0[MSUB + e];
         sp1[e] = sp1[MSUB + e];
       }
