@@ -458,3 +458,25 @@ def test_sp4_reference_form_deterministic_limit(orc):
         traj.append(x)
     for s in range(ns):
         np.testing.assert_allclose(u[s], traj[s * (nt // ns)], rtol=1e-12)
+
+
+def test_reference_semantics_conditioning_with_uniform_fields(orc):
+    """The reference Lanczos semantics (T[m-1, m-1] left at zero, f at |lambda|: variant 0)
+    are ill-conditioned at small m on a symmetric kink ring with uniform fields: rounding
+    only the initial data to FP32 (a 1.9e-8 change) moves the FP64 result after 150 steps
+    by 4.5e-5 at m = 8 (3.2e-6 at m = 10, 3e-8 at m = 16), while the intended Lanczos
+    (variant 1) moves by 2.7e-8 at every m.  This bounds what any FP32 Krylov
+    implementation of the reference semantics can match at m = 8."""
+    from nls.samplers import kink_ring_2d
+    n, L, dt, nt = 128, 10.0, 0.005, 300
+    u0, v0, _ = kink_ring_2d(n, L, 2, 0)
+    one = np.ones((n, n))
+    u0p = u0.astype(np.float32).astype(np.float64)
+
+    def sens(m, variant):
+        a = orc.kge_run(n, L, dt * nt, nt, 2, one, one, u0, v0, krylov=m, mu2=1.0, variant=variant)[0][1]
+        b = orc.kge_run(n, L, dt * nt, nt, 2, one, one, u0p, v0, krylov=m, mu2=1.0, variant=variant)[0][1]
+        return np.linalg.norm(a - b) / np.linalg.norm(a)
+    assert sens(8, 0) > 1e-5
+    assert sens(8, 1) < 1e-7
+    assert sens(16, 0) < 1e-7
