@@ -379,8 +379,8 @@ def run_nlse2d(args):
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
                          "traffic": None,
-                         "kernel": ("k_nlse_march2d<2> (row-march complex64 pass)"
-                                    if info["pass_kind"] == 2 else
+                         "kernel": ("k_nlse_march2d<2, true> (row-march complex64 pass, "
+                                    "uniform fields)" if info["pass_kind"] == 2 else
                                     "k_nlse_pass<2,2,CPT> (direct complex64 pass)"),
                          "bytes_per_launch": pt["bytes_per_launch"],
                          "avg_launch_ms": pt["avg_ms"]},
@@ -468,8 +468,9 @@ def run_kge3d(args):
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
                          "traffic": None,
-                         "kernel": ("k_kge_march3d<2,CPT> (plane-march fused 7-point pass)"
-                                    if info["pass_kind"] == 4 else
+                         "kernel": (f"k_kge_march3d<2, {info['cells_per_thread']}, "
+                                    f"{'false' if args.load_fields else 'true'}> (plane-march "
+                                    "fused 7-point pass)" if info["pass_kind"] == 4 else
                                     "k_kge_pass<3,2,CPT> (direct fused 7-point pass)"),
                          "bytes_per_launch": pt["bytes_per_launch"],
                          "avg_launch_ms": pt["avg_ms"],
