@@ -325,6 +325,21 @@ __global__ __launch_bounds__(kWave) void k_small_de(int m, const double *__restr
     }
     return;
   }
+  if (kind & F_WAVE_QL) { // the fused batch's register-resident QL (real symmetric T)
+    const int lane = threadIdx.x;
+    double dk = lane < m ? d[lane] : 0.0, ek = lane < m - 1 ? e[lane] : 0.0;
+    if (small_ql_wave(m, dk, ek, ws.Z)) {
+      if (lane == 0) *status = 1;
+      return;
+    }
+    double r, i;
+    small_apply_f_wave(m, dk, ws.Z, kind & ~F_WAVE_QL, t_re, t_im, beta0, 1.0, 0.0, r, i);
+    if (lane < m) {
+      yr[lane] = r;
+      yi[lane] = i;
+    }
+    return;
+  }
   if (threadIdx.x == 0) {
     for (int i = 0; i < m; ++i) { ws.d[i] = d[i]; ws.phr[i] = 1.0; ws.phi[i] = 0.0; }
     for (int i = 0; i < m - 1; ++i) ws.e[i] = e[i];

@@ -30,6 +30,9 @@ enum FuncKind : int {
   F_SINC = 9,       // sin(dt l)/(dt l), 1 if |.| < 1e-8  (matfunc_complex.hpp:293-300)
   F_EXP_SERIES = 10, // exp(dt T) e1 by a scaled Taylor series (no eigen-decomposition)
 };
+// Added to a real kind: the register-resident single-wavefront QL of the fused batch
+// (small_ql_wave + small_apply_f_wave) instead of the workgroup version.
+constexpr int F_WAVE_QL = 0x100;
 
 struct cplx {
   double re, im;
@@ -181,11 +184,13 @@ __device__ inline int small_ql_wave(int m, double &dk, double &ek, double (*Z)[k
   for (int l = 0; l < m; ++l) {
     int iter = 0;
     for (;;) {
-      int mm;
-      for (mm = l; mm < m - 1; ++mm) {
-        const double dd = fabs(lane_read(dk, mm)) + fabs(lane_read(dk, mm + 1));
-        if (fabs(lane_read(ek, mm)) <= 2.220446049250313e-16 * dd) break;
-      }
+      // first mm >= l with a negligible e_mm (or m - 1): every lane tests its own e_k
+      // against |d_k| + |d_{k+1}| at once, and the ballot gives the first one
+      const double dnext = __shfl_down(dk, 1);
+      const bool neg = lane < m - 1 && fabs(ek) <= 2.220446049250313e-16 * (fabs(dk) + fabs(dnext));
+      const unsigned long long stop =
+          (__ballot(neg) | (1ull << (m - 1))) & ~((1ull << l) - 1ull);
+      const int mm = __ffsll((long long)stop) - 1;
       if (mm == l) break;
       if (++iter > 60) return 1;
       const double dl = lane_read(dk, l), el = lane_read(ek, l);
@@ -197,7 +202,13 @@ __device__ inline int small_ql_wave(int m, double &dk, double &ek, double (*Z)[k
       for (int i = mm - 1; i >= l; --i) {
         const double ei = lane_read(ek, i);
         const double f = s * ei, b = c * ei;
-        r = sqrt(f * f + g * g);
+        // r = sqrt(q), 1 / r = rsq(q) with two Newton steps: one long-latency operation
+        // per rotation instead of a square root and a division
+        const double q = f * f + g * g;
+        double rs = __builtin_amdgcn_rsq(q);
+        rs = fma(0.5 * rs, fma(-q * rs, rs, 1.0), rs);
+        rs = fma(0.5 * rs, fma(-q * rs, rs, 1.0), rs);
+        r = q > 0.0 ? q * rs : 0.0;
         if (lane == i + 1) ek = r;
         if (r == 0.0) {
           const double di1 = lane_read(dk, i + 1);
@@ -206,9 +217,8 @@ __device__ inline int small_ql_wave(int m, double &dk, double &ek, double (*Z)[k
           deflated = true;
           break;
         }
-        const double ir = 1.0 / r;
-        s = f * ir;
-        c = g * ir;
+        s = f * rs;
+        c = g * rs;
         g = lane_read(dk, i + 1) - p;
         r = (lane_read(dk, i) - g) * s + 2.0 * c * b;
         p = s * r;

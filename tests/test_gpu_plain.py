@@ -67,9 +67,10 @@ def test_small_matfunc_gpu_vs_oracle(orc):
         Tcm = np.ascontiguousarray(T.T)
         for kind, t in ((orc.F_COS_SQRT, 0.05), (orc.F_SINC2_SQRT, 0.05), (orc.F_ID_SQRT, 0.05),
                         (orc.F_SINC2_HALF, 0.1), (orc.F_COSM1_SQRT, 0.01)):
-            y = small_matfunc(d, e, kind, t).real
             ref = orc.small_matfunc_real(Tcm, kind, t)
-            assert np.abs(y - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
+            for route in (0, 0x100):  # workgroup QL (plain path), wavefront QL (fused batch)
+                y = small_matfunc(d, e, kind + route, t).real
+                assert np.abs(y - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max()), (m, kind, route)
         y = small_matfunc(d, e, 8, 0.005j)
         Tc = Tcm.astype(np.complex128)
         ref = orc.small_matfunc_complex(Tc, 0, 0.005j)
