@@ -199,6 +199,9 @@ __device__ inline int small_ql_wave(int m, double &dk, double &ek, double (*Z)[k
       g = lane_read(dk, mm) - dl + el / (g + (g >= 0.0 ? r : -r));
       double s = 1.0, c = 1.0, p = 0.0;
       bool deflated = false;
+      // Z[lane][i + 1] of the rotation in flight stays in a register: each rotation reads
+      // one element Z[lane][i] no rotation has written yet and writes one finished element
+      double zc = lane < m ? Z[lane][mm] : 0.0;
       for (int i = mm - 1; i >= l; --i) {
         const double ei = lane_read(ek, i);
         const double f = s * ei, b = c * ei;
@@ -214,6 +217,7 @@ __device__ inline int small_ql_wave(int m, double &dk, double &ek, double (*Z)[k
           const double di1 = lane_read(dk, i + 1);
           if (lane == i + 1) dk = di1 - p;
           if (lane == mm) ek = 0.0;
+          if (lane < m) Z[lane][i + 1] = zc;
           deflated = true;
           break;
         }
@@ -225,13 +229,14 @@ __device__ inline int small_ql_wave(int m, double &dk, double &ek, double (*Z)[k
         const double dnew = g + p;
         g = c * r - b;
         if (lane < m) {
-          const double zf = Z[lane][i + 1], zi = Z[lane][i];
-          Z[lane][i + 1] = s * zi + c * zf;
-          Z[lane][i] = c * zi - s * zf;
+          const double zi = Z[lane][i];
+          Z[lane][i + 1] = s * zi + c * zc;
+          zc = c * zi - s * zc;
         }
         if (lane == i + 1) dk = dnew;
       }
       if (deflated) continue;
+      if (lane < m) Z[lane][l] = zc;
       if (lane == l) {
         dk -= p;
         ek = g;
