@@ -1533,7 +1533,7 @@ __global__ __launch_bounds__(FBS, (CPT == 4 && UNI) ? 4 : 1) void k_kge_march3d(
   }
 }
 
-// Combine over the workgroup's range in groups of 4 cells per thread (independent loads).
+// Combine over the workgroup's range in groups of G cells per thread (independent loads).
 // v_{m-1} = (w'_{m-2} - alpha_{m-2} v_{m-2}) / b_{m-1} is folded into the
 // coefficients: d = y0 x/b0 + sum_{k<m-2} y_k v_k + (y_{m-2} - y_{m-1} alpha/b) v_{m-2}
 // + (y_{m-1}/b) w'_{m-2}, so each stored vector is read exactly once.
@@ -1561,7 +1561,9 @@ __global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a) {
     cw[s] = S.y[s][m - 1] * ibl;
   }
   __syncthreads();
-  constexpr int G = 4;
+  // 2 cells per thread at a time: 118 us per launch at 100 x 300^2, m = 8, against 130 us
+  // at 1, 142 us at 4 and 251 us at 8 (more loads in flight per wave run slower)
+  constexpr int G = 2;
   const int n = T.n;
   const int64_t cells = a.cells;
   const int64_t cbeg = (int64_t)cm.y * T.range;
