@@ -2867,19 +2867,34 @@ __global__ __launch_bounds__(FBS) void k_nlse_combine_rt(NlseArgs a, SnapC snap)
   const int n = T.n;
   const int64_t cbeg = (int64_t)cm.y * T.range;
   const int64_t cend = cbeg + T.range < T.N ? cbeg + T.range : T.N;
-  for (int64_t i = cbeg + threadIdx.x; i < cend; i += FBS) {
-    const int64_t src = clamp_src<DIM>(i, n);
-    const int64_t gi = T.off + i, gs = T.off + src;
-    double sr, si;
-    nlse_psi3<UNI>(a, T, S, ys[0], ys[1], gs, sr, si);
-    a.psi_next[gi] = c32{(float)sr, (float)si};
-    if (snap.psi) { // snapshot before the ghost copy (nlse_dev.hpp:195-197)
-      double ir = sr, ii = si;
-      if (src != i) nlse_psi3<UNI>(a, T, S, ys[0], ys[1], gi, ir, ii);
-      snap.psi[2 * gi] = ir;
-      snap.psi[2 * gi + 1] = ii;
+  // 2 cells per thread at a time: both cells' loads are issued before either store
+  constexpr int G = 2;
+  for (int64_t i0 = cbeg + threadIdx.x; i0 < cend; i0 += G * FBS) {
+    double sr[G], si[G];
+    int64_t srcs[G];
+#pragma unroll
+    for (int kk = 0; kk < G; ++kk) {
+      const int64_t i = i0 + (int64_t)kk * FBS;
+      if (i < cend) {
+        srcs[kk] = clamp_src<DIM>(i, n);
+        nlse_psi3<UNI>(a, T, S, ys[0], ys[1], T.off + srcs[kk], sr[kk], si[kk]);
+      }
     }
-    if (orth) nlse_orth_partial(a, gi, o);
+#pragma unroll
+    for (int kk = 0; kk < G; ++kk) {
+      const int64_t i = i0 + (int64_t)kk * FBS;
+      if (i < cend) {
+        const int64_t gi = T.off + i;
+        a.psi_next[gi] = c32{(float)sr[kk], (float)si[kk]};
+        if (snap.psi) { // snapshot before the ghost copy (nlse_dev.hpp:195-197)
+          double ir = sr[kk], ii = si[kk];
+          if (srcs[kk] != i) nlse_psi3<UNI>(a, T, S, ys[0], ys[1], gi, ir, ii);
+          snap.psi[2 * gi] = ir;
+          snap.psi[2 * gi + 1] = ii;
+        }
+        if (orth) nlse_orth_partial(a, gi, o);
+      }
+    }
   }
   if (orth) {
     block_sum_n<2>(o, sho);
