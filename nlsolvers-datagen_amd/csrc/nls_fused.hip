@@ -1259,10 +1259,11 @@ inline size_t march3d_lds_bytes(int cpt, int nmax, bool uni = false) {
   return sizeof(float) * (uni ? 2 : 3) * (size_t)(FBS * cpt + 2 * nmax);
 }
 
-// The tile-4 uniform-field variant is held to 128 VGPRs (4 waves per SIMD instead of 3, at
-// the cost of a 12-byte spill in stage 2): 3.64 -> 3.09 ms per pass on 40 x 220^3.
+// The tile-4 uniform-field stage-2 pass is held to 128 VGPRs (4 waves per SIMD instead of 3,
+// at the cost of a 12-byte spill): 3.64 -> 3.09-3.31 ms per pass on 40 x 220^3.  Stage 1 under
+// the same cap spills 36 bytes and slows from 3.68 to 4.82 ms, so it is left uncapped.
 template <int STAGE, int CPT, bool UNI = false>
-__global__ __launch_bounds__(FBS, (CPT == 4 && UNI) ? 4 : 1) void k_kge_march3d(KgeArgs a, int j, int emax) {
+__global__ __launch_bounds__(FBS, (CPT == 4 && UNI && STAGE == 2) ? 4 : 1) void k_kge_march3d(KgeArgs a, int j, int emax) {
   extern __shared__ float lds[];
   const int2 cm = a.cmap[blockIdx.x];
   if (cm.x < 0) return;
