@@ -271,6 +271,14 @@ typedef struct {
                         upload) has the row- and plane-march passes and the combines use
                         the value instead of streaming the field (the same numbers, 4 B per
                         cell and pass less); 1: always stream the fields */
+  int variant;       /* Lanczos variant of the Krylov passes (KGE Gautschi / sEWI, NLSE):
+                        0 = the reference's, T[m-1][m-1] = 0 (device/include/lanczos.hpp:
+                        234-263); 1 = full tridiagonal T: one more pass per step forms
+                        v_{m-1} on the fly and computes alpha_{m-1} = <A v_{m-1}, v_{m-1}>
+                        (it reads 16 B per cell for KGE and stores nothing).  Variant 1 runs
+                        on pass kernels 0, 2 and 4.  With krylov_dim = 0 the autotuner
+                        picks m >= 10 for variant 0 (the reference drivers' m) and
+                        m >= 8 for variant 1 */
 } nls_batch_params;
 typedef struct {
   int n;             /* full side */

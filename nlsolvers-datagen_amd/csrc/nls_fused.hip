@@ -59,9 +59,13 @@ struct TrajDev {
   // the planes [zseg * (c / npen), zseg * (c / npen + 1))
   int npen, prange, zseg;
   int64_t off, N;
-  float s, mu2;
   double tau;
-  double sd; // FP64 1 / dx^2 (Stormer-Verlet step)
+  // The Krylov passes apply the unscaled operator B (weights (c_i + c_nb)/2; KGE: the
+  // negated div(c grad), NLSE: div(c grad)) and the small stages form T of the operator
+  // sd B + mu2d I in FP64 (the Krylov space of B is that of A = sd B + mu2d I, and
+  // T_A = sd T_B + mu2d I), so the FP32 rounding of 1/dx^2 does not shift the spectrum.
+  double sd;   // FP64 1 / dx^2
+  double mu2d; // KGE mass term mu2
   // spatially uniform coefficient fields (bit 0: c, bit 1: m) and their FP32 values; a
   // batch whose trajectories all have both runs the UNI kernel variants, which use
   // cval / mval instead of streaming the fields (the same numbers)
@@ -254,8 +258,10 @@ __device__ inline void lanczos_scalars(double q0, double q1, double q2, double &
 
 // ---------------------------------------------------------------- Lanczos pass
 // STAGE 0: pass j = 0 from the seeds; STAGE 1: j = 1; STAGE 2: j >= 2.
-template <int DIM, int STAGE, int CPT>
+// LAST: pass j = m - 1 of the full-T Lanczos variant (partials only, nothing stored).
+template <int DIM, int STAGE, int CPT, bool LAST = false>
 __global__ __launch_bounds__(FBS) void k_kge_pass(KgeArgs a, int j) {
+  static_assert(!(LAST && STAGE == 0), "the last pass follows pass 0");
   const int2 cm = a.cmap[blockIdx.x];
   if (cm.x < 0) return;
   const TrajDev T = a.traj[cm.x];
@@ -301,7 +307,6 @@ __global__ __launch_bounds__(FBS) void k_kge_pass(KgeArgs a, int j) {
     }
   }
   const int n = T.n;
-  const float sc = T.s, mu2 = T.mu2;
   const int64_t cells = a.cells;
   double acc[NSEQ][PN];
 #pragma unroll
@@ -319,12 +324,8 @@ __global__ __launch_bounds__(FBS) void k_kge_pass(KgeArgs a, int j) {
     neighbours<DIM>(i, n, T.N, a.q6, nb);
     const float ci = a.cf[gi];
     float w[6];
-    float wsum = 0.0f;
 #pragma unroll
-    for (int q = 0; q < 2 * DIM; ++q) {
-      w[q] = nb.has[q] ? 0.5f * (ci + a.cf[gi + nb.off[q]]) : 0.0f;
-      wsum += w[q];
-    }
+    for (int q = 0; q < 2 * DIM; ++q) w[q] = nb.has[q] ? 0.5f * (ci + a.cf[gi + nb.off[q]]) : 0.0f;
 #pragma unroll
     for (int s = 0; s < NSEQ; ++s) {
       auto val = [&](int64_t g) -> float {
@@ -338,11 +339,10 @@ __global__ __launch_bounds__(FBS) void k_kge_pass(KgeArgs a, int j) {
         return (wv - cf[s].A * vp) * cf[s].B;
       };
       const float vi = val(gi);
-      float nbsum = 0.0f;
+      float Av = 0.0f; // flux form sum_nb w (v_i - v_nb), as apply5
 #pragma unroll
       for (int q = 0; q < 2 * DIM; ++q)
-        if (nb.has[q]) nbsum += w[q] * val(gi + nb.off[q]);
-      const float Av = sc * (wsum * vi - nbsum) + mu2 * vi;
+        if (nb.has[q]) Av += w[q] * (vi - val(gi + nb.off[q]));
       if (STAGE == 0) {
         a.W[((size_t)s * 2 + 0) * cells + gi] = Av;
         acc[s][0] += (double)vi * vi;
@@ -353,8 +353,10 @@ __global__ __launch_bounds__(FBS) void k_kge_pass(KgeArgs a, int j) {
         if (STAGE == 1) vprev = seed_x(s, a, gi);
         else vprev = a.V[((size_t)s * (a.m - 2) + (j - 2)) * cells + gi];
         const float wp = Av - cf[s].C * vprev;
-        a.V[((size_t)s * (a.m - 2) + (j - 1)) * cells + gi] = vi;
-        a.W[((size_t)s * 2 + (j & 1)) * cells + gi] = wp;
+        if (!LAST) {
+          a.V[((size_t)s * (a.m - 2) + (j - 1)) * cells + gi] = vi;
+          a.W[((size_t)s * 2 + (j & 1)) * cells + gi] = wp;
+        }
         acc[s][0] += (double)wp * vi;
         acc[s][1] += (double)wp * wp;
         acc[s][2] += (double)vi * vi;
@@ -373,8 +375,11 @@ __global__ __launch_bounds__(FBS) void k_kge_pass(KgeArgs a, int j) {
 }
 
 // ---------------------------------------------------------------- small stage
-// One wavefront per (trajectory, sequence): alpha_{m-2}, b_{m-1} from the last
-// pass, T = tridiag(b, [alpha_0..alpha_{m-2}, 0], b), y = b0 Q f(L) Q^T e1.
+// One wavefront per (trajectory, sequence): the scalars of the last pass, T of
+// A = sd B + mu2 I (B the operator the passes apply) and y = b0 Q f(L) Q^T e1.
+// last_pass = m - 2: the reference's variant, alpha_{m-2} and b_{m-1} from the last
+// pass and T = tridiag(b, [alpha_0..alpha_{m-2}, 0], b) (lanczos.hpp:234-263, Q1).
+// last_pass = m - 1: full T, alpha_{m-1} from the extra pass (b_{m-1} was set by it).
 __global__ __launch_bounds__(kWave) void k_kge_small(KgeArgs a, int last_pass) {
   const int t = a.tlist ? a.tlist[blockIdx.x / NSEQ] : (int)(blockIdx.x / NSEQ);
   const int s = blockIdx.x % NSEQ;
@@ -414,13 +419,17 @@ __global__ __launch_bounds__(kWave) void k_kge_small(KgeArgs a, int last_pass) {
   }
   double al, b;
   lanczos_scalars(red[0], red[1], red[2], al, b);
+  const bool full = last_pass == m - 1;
   if (lane == 0) {
-    S.alpha[s][m - 2] = al;
-    S.beta[s][m - 1] = b;
+    S.alpha[s][last_pass] = al;
+    if (!full) S.beta[s][m - 1] = b;
   }
-  // lane k: d_k = alpha_k (d_{m-1} = 0, Q1), e_k = b_{k+1}
-  double dk = lane < m - 2 ? S.alpha[s][lane] : (lane == m - 2 ? al : 0.0);
-  double ek = lane < m - 2 ? S.beta[s][lane + 1] : (lane == m - 2 ? b : 0.0);
+  // lane k: d_k = sd alpha_k + mu2 (reference variant: d_{m-1} = 0, Q1), e_k = sd b_{k+1}
+  const double alk = lane < last_pass ? S.alpha[s][lane] : (lane == last_pass ? al : 0.0);
+  const double bk = lane < m - 2 ? S.beta[s][lane + 1]
+                                 : (lane == m - 2 ? (full ? S.beta[s][m - 1] : b) : 0.0);
+  double dk = lane <= last_pass ? T.sd * alk + T.mu2d : 0.0;
+  double ek = T.sd * bk;
   const double b0 = S.b0[s];
   if (small_ql_wave(m, dk, ek, ws.Z)) {
     if (lane == 0) S.status = 1;
@@ -541,7 +550,6 @@ __global__ __launch_bounds__(FBS) void k_kge_pass2d(KgeArgs a, int j, int emax) 
     }
   }
   __syncthreads();
-  const float scl = T.s, mu2 = T.mu2;
   double acc[NSEQ][PN];
 #pragma unroll
   for (int s = 0; s < NSEQ; ++s)
@@ -558,18 +566,16 @@ __global__ __launch_bounds__(FBS) void k_kge_pass2d(KgeArgs a, int j, int emax) 
       const float wl = col > 0 ? 0.5f * (ci + sc[e - 1]) : 0.0f;
       const float wr = col < n - 1 ? 0.5f * (ci + sc[e + 1]) : 0.0f;
       const float wd = r < n - 1 ? 0.5f * (ci + sc[e + n]) : 0.0f;
-      const float wsum = ((wu + wl) + wr) + wd;
       const int64_t gi = T.off + i;
 #pragma unroll
       for (int s = 0; s < NSEQ; ++s) {
         const float *sv = s == 0 ? sv0 : sv1;
         const float vi = sv[e];
-        float nbsum = 0.0f;
-        if (r > 0) nbsum += wu * sv[e - n];
-        if (col > 0) nbsum += wl * sv[e - 1];
-        if (col < n - 1) nbsum += wr * sv[e + 1];
-        if (r < n - 1) nbsum += wd * sv[e + n];
-        const float Av = scl * (wsum * vi - nbsum) + mu2 * vi;
+        float Av = 0.0f; // flux form, as apply5
+        if (r > 0) Av += wu * (vi - sv[e - n]);
+        if (col > 0) Av += wl * (vi - sv[e - 1]);
+        if (col < n - 1) Av += wr * (vi - sv[e + 1]);
+        if (r < n - 1) Av += wd * (vi - sv[e + n]);
         if (STAGE == 0) {
           a.W[(size_t)(s * 2) * cells + gi] = Av;
           acc[s][0] += (double)vi * vi;
@@ -656,7 +662,7 @@ inline size_t march2d_lds_bytes(int nmax, bool uni = false) {
 // STAGE 1: j = 1 (P = x, no Q); STAGE 2: j = 2 (P = v_1, Q = x);
 // STAGE 3: j >= 3 (P = v_{j-1}, Q = v_{j-2}).
 struct Sten5 {
-  float wu, wl, wr, wd, wsum;
+  float wu, wl, wr, wd;
   bool hu, hl, hr, hd;
 };
 
@@ -669,51 +675,68 @@ __device__ inline int row_of(int i, int n, float rn) {
   return r;
 }
 
-// Weights (c_i + c_nb)/2 of cell i (grid row/column from i) read from an LDS
-// window at e, neighbour offsets (up, left, right, down) in window coordinates.
+// Weights (c_i + c_nb)/2 of cell i (grid column from i; N = n * n cells) read from an LDS
+// window at e, neighbour offsets (up, left, right, down) in window coordinates; a missing
+// neighbour has weight 0.
 __device__ inline Sten5 sten5(const float *sc, int e, int oU, int oL, int oR, int oD, int i,
-                              int n, float rn) {
-  const int r = row_of(i, n, rn), col = i - r * n;
+                              int n, int N, float rn) {
+  const int col = i - row_of(i, n, rn) * n;
   Sten5 w;
-  w.hu = r > 0;
+  w.hu = i >= n;
   w.hl = col > 0;
   w.hr = col < n - 1;
-  w.hd = r < n - 1;
+  w.hd = i < N - n;
   const float ci = sc[e];
   w.wu = w.hu ? 0.5f * (ci + sc[e + oU]) : 0.0f;
   w.wl = w.hl ? 0.5f * (ci + sc[e + oL]) : 0.0f;
   w.wr = w.hr ? 0.5f * (ci + sc[e + oR]) : 0.0f;
   w.wd = w.hd ? 0.5f * (ci + sc[e + oD]) : 0.0f;
-  w.wsum = ((w.wu + w.wl) + w.wr) + w.wd;
   return w;
 }
 
 // sten5 for a uniform c: every weight (c + c) / 2 is c, formed as the field path forms it
-__device__ inline Sten5 sten5_uniform(float cu, int i, int n, float rn) {
-  const int r = row_of(i, n, rn), col = i - r * n;
+__device__ inline Sten5 sten5_uniform(float cu, int i, int n, int N, float rn) {
+  const int col = i - row_of(i, n, rn) * n;
   Sten5 w;
-  w.hu = r > 0;
+  w.hu = i >= n;
   w.hl = col > 0;
   w.hr = col < n - 1;
-  w.hd = r < n - 1;
+  w.hd = i < N - n;
   const float wc = 0.5f * (cu + cu);
   w.wu = w.hu ? wc : 0.0f;
   w.wl = w.hl ? wc : 0.0f;
   w.wr = w.hr ? wc : 0.0f;
   w.wd = w.hd ? wc : 0.0f;
-  w.wsum = ((w.wu + w.wl) + w.wr) + w.wd;
   return w;
 }
 
+// B v = sum_nb w (v_i - v_nb) (the negated div(c grad) without 1/dx^2) in flux form: each
+// difference is exact for neighbouring values within a factor 2, and the implicit diagonal
+// is exactly the sum of the rounded off-diagonal weights, so B 1 = 0 holds in FP32 as in
+// the exact operator (a diagonal formed as a rounded sum of the weights would add a
+// random potential of ~1/dx^2 * 6e-8 per cell, which shifts the low frequencies and
+// accumulates over long runs).  apply5: neighbours read only where present.
 __device__ inline float apply5(const Sten5 &w, const float *v, int e, int oU, int oL, int oR,
-                               int oD, float scl, float mu2) {
+                               int oD) {
   const float vi = v[e];
-  float nb = 0.0f;
-  if (w.hu) nb += w.wu * v[e + oU];
-  if (w.hl) nb += w.wl * v[e + oL];
-  if (w.hr) nb += w.wr * v[e + oR];
-  if (w.hd) nb += w.wd * v[e + oD];
-  return scl * (w.wsum * vi - nb) + mu2 * vi;
+  float r = 0.0f;
+  if (w.hu) r += w.wu * (vi - v[e + oU]);
+  if (w.hl) r += w.wl * (vi - v[e + oL]);
+  if (w.hr) r += w.wr * (vi - v[e + oR]);
+  if (w.hd) r += w.wd * (vi - v[e + oD]);
+  return r;
+}
+
+// apply5 on a window whose cells outside the grid hold finite values (zeros): every
+// neighbour is read and a missing one contributes w = 0 (the same sum, no branches).
+__device__ inline float apply5z(const Sten5 &w, const float *v, int e, int oU, int oL, int oR,
+                                int oD) {
+  const float vi = v[e];
+  float r = w.wu * (vi - v[e + oU]);
+  r += w.wl * (vi - v[e + oL]);
+  r += w.wr * (vi - v[e + oR]);
+  r += w.wd * (vi - v[e + oD]);
+  return r;
 }
 
 constexpr int RKS = 6; // window shift registers: supports n <= RKS * FBS / 3 = 512
@@ -784,7 +807,6 @@ __global__ __launch_bounds__(FBS) void k_kge_rmarch2d(KgeArgs a, int j, int emax
   float *Wout0 = a.W + ((size_t)0 * 2 + (j & 1)) * cells + T.off;
   float *Wout1 = a.W + ((size_t)1 * 2 + (j & 1)) * cells + T.off;
   const bool last = j == a.m - 2;
-  const float scl = T.s, mu2 = T.mu2;
   double acc[NSEQ * PN];
 #pragma unroll
   for (int k = 0; k < NSEQ * PN; ++k) acc[k] = 0.0;
@@ -826,9 +848,9 @@ __global__ __launch_bounds__(FBS) void k_kge_rmarch2d(KgeArgs a, int j, int emax
   };
   // v_j at cell i: P window index eP, v window index eV
   auto phaseA = [&](int i, int eP, int eV, float q0, float q1) {
-    const Sten5 w = sten5(sc, eP, oU, oL, oR, oD, i, n, rn);
-    const float a0 = apply5(w, sp0, eP, oU, oL, oR, oD, scl, mu2);
-    const float a1 = apply5(w, sp1, eP, oU, oL, oR, oD, scl, mu2);
+    const Sten5 w = sten5(sc, eP, oU, oL, oR, oD, i, n, N, rn);
+    const float a0 = apply5(w, sp0, eP, oU, oL, oR, oD);
+    const float a1 = apply5(w, sp1, eP, oU, oL, oR, oD);
     const float w0 = a0 - qc[0] * q0;
     const float w1 = a1 - qc[1] * q1;
     sv0[eV] = (w0 - pa[0] * sp0[eP]) * pb[0];
@@ -900,13 +922,13 @@ __global__ __launch_bounds__(FBS) void k_kge_rmarch2d(KgeArgs a, int j, int emax
       if (l < ce) {
         const int i = cell(l);
         const int eP = l - cb + 2 * n, eV = l - cb + n;
-        const Sten5 w = sten5(sc, eP, oU, oL, oR, oD, i, n, rn);
+        const Sten5 w = sten5(sc, eP, oU, oL, oR, oD, i, n, N, rn);
         if (STAGE == 0) {
 #pragma unroll
           for (int s = 0; s < NSEQ; ++s) {
             const float *sp = s == 0 ? sp0 : sp1;
             const float x = sp[eP];
-            const float Av = apply5(w, sp, eP, oU, oL, oR, oD, scl, mu2);
+            const float Av = apply5(w, sp, eP, oU, oL, oR, oD);
             acc[s * PN + 0] += (double)x * x;
             acc[s * PN + 1] += (double)x * Av;
             acc[s * PN + 2] += (double)Av * Av;
@@ -916,7 +938,7 @@ __global__ __launch_bounds__(FBS) void k_kge_rmarch2d(KgeArgs a, int j, int emax
           for (int s = 0; s < NSEQ; ++s) {
             const float *sv = s == 0 ? sv0 : sv1;
             const float vi = sv[eV];
-            const float Av = apply5(w, sv, eV, oU, oL, oR, oD, scl, mu2);
+            const float Av = apply5(w, sv, eV, oU, oL, oR, oD);
             const float wp = Av - pc[s] * (s == 0 ? sp0 : sp1)[eP];
             (s == 0 ? Vout0 : Vout1)[i] = vi;
             if (last) (s == 0 ? Wout0 : Wout1)[i] = wp;
@@ -1031,8 +1053,11 @@ inline size_t rmarch2d_lds_bytes(int nmax) {
 // odd range index march downward (alt != 0), so shared halo rows come from L2.
 // STAGE 0: j = 0 on the seeds x (stores z = A x); 1: j = 1 (from z and x); 2: j >= 2.
 // UNI: every trajectory of the batch has uniform m and c (values from the descriptor).
-template <int STAGE, bool UNI = false>
+// LAST: pass j = m - 1 of the full-T Lanczos variant: v_{m-1} is formed and A applied for
+// the partials that give alpha_{m-1}; nothing is stored (the combine folds v_{m-1}).
+template <int STAGE, bool UNI = false, bool LAST = false>
 __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax, int alt) {
+  static_assert(!(LAST && STAGE == 0), "the last pass follows pass 0");
   extern __shared__ float lds[];
   const int2 cm = a.cmap[blockIdx.x];
   if (cm.x < 0) return;
@@ -1059,7 +1084,6 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax,
   float *Vout1 = a.V + (STAGE >= 1 ? (1 * m2 + (size_t)(j - 1)) * cells : 0) + T.off;
   float *Wout0 = a.W + ((size_t)0 * 2 + (j & 1)) * cells + T.off;
   float *Wout1 = a.W + ((size_t)1 * 2 + (j & 1)) * cells + T.off;
-  const float scl = T.s, mu2 = T.mu2;
   double acc[NSEQ * PN];
 #pragma unroll
   for (int k = 0; k < NSEQ * PN; ++k) acc[k] = 0.0;
@@ -1149,49 +1173,48 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax,
       }
     }
   };
+  // Window cells outside the trajectory (i < 0 or i >= N) are deposited as zeros, so the
+  // stencil reads all four window neighbours unconditionally (a missing one has weight 0
+  // and value 0).  Loads use clamped indices and are all issued: their number, and the
+  // number of stores of a full sub-chunk, are fixed, so waiting for the prefetched window
+  // does not have to wait for the stores issued after it.
+  auto clampi = [N](int i) { return i < 0 ? 0 : (i >= N ? N - 1 : i); };
   // first window [-n, SUB + n), MSPT loads per thread in flight before their deposits
   for (int e0 = 0; e0 < MSUB + 2 * n; e0 += MSPT * FBS) {
     float rc[MSPT], x0[MSPT], x1[MSPT], p0[MSPT], p1[MSPT], rm[MSPT];
     double ru[MSPT];
+    bool ok[MSPT];
 #pragma unroll
     for (int q = 0; q < MSPT; ++q) {
       const int e = e0 + threadIdx.x + q * FBS, i = cell(e - n);
-      if (e < MSUB + 2 * n && i >= 0 && i < N)
-        fetch(i, rc[q], x0[q], x1[q], p0[q], p1[q], ru[q], rm[q]);
+      ok[q] = i >= 0 && i < N;
+      if (e < MSUB + 2 * n) fetch(clampi(i), rc[q], x0[q], x1[q], p0[q], p1[q], ru[q], rm[q]);
     }
     if (e0 == 0) coefs();
 #pragma unroll
     for (int q = 0; q < MSPT; ++q) {
-      const int e = e0 + threadIdx.x + q * FBS, i = cell(e - n);
-      if (e < MSUB + 2 * n && i >= 0 && i < N)
+      const int e = e0 + threadIdx.x + q * FBS;
+      if (e < MSUB + 2 * n) {
+        if (!ok[q]) rc[q] = x0[q] = x1[q] = p0[q] = p1[q] = rm[q] = 0.0f, ru[q] = 0.0;
         deposit(e, rc[q], x0[q], x1[q], p0[q], p1[q], ru[q], rm[q]);
-    }
-  }
-  float nc[MSPT], nx0[MSPT], nx1[MSPT], np0[MSPT], np1[MSPT], nm[MSPT];
-  double nu[MSPT];
-  for (int cb = 0; cb < len; cb += MSUB) {
-    const int ce = cb + MSUB < len ? cb + MSUB : len;
-    const bool more = cb + MSUB < len;
-    if (more) {
-      // the next window's new cells [cb' + n, cb' + SUB + n)
-#pragma unroll
-      for (int q = 0; q < MSPT; ++q) {
-        const int i = cell(cb + MSUB + n + threadIdx.x + q * FBS);
-        if (i >= 0 && i < N) fetch(i, nc[q], nx0[q], nx1[q], np0[q], np1[q], nu[q], nm[q]);
       }
     }
-    __syncthreads();
+  }
+  // stencil, stores and partials of the cells [cb, ce); FULL: ce = cb + SUB
+  auto compute = [&](auto full_t, int cb, int ce) {
+    constexpr bool FULL = decltype(full_t)::value;
 #pragma unroll
     for (int kk = 0; kk < MSPT; ++kk) {
       const int l = cb + threadIdx.x + kk * FBS;
-      if (l < ce) {
+      if (FULL || l < ce) {
         const int i = cell(l), e = l - cb + n;
-        const Sten5 w = UNI ? sten5_uniform(T.cval, i, n, rn) : sten5(sc, e, oU, oL, oR, oD, i, n, rn);
+        const Sten5 w = UNI ? sten5_uniform(T.cval, i, n, N, rn)
+                            : sten5(sc, e, oU, oL, oR, oD, i, n, N, rn);
 #pragma unroll
         for (int s = 0; s < NSEQ; ++s) {
           const float *sv = s == 0 ? sv0 : sv1;
           const float vi = sv[e];
-          const float Av = apply5(w, sv, e, oU, oL, oR, oD, scl, mu2);
+          const float Av = apply5z(w, sv, e, oU, oL, oR, oD);
           if (STAGE == 0) {
             __builtin_nontemporal_store(Av, &(s == 0 ? Wout0 : Wout1)[i]);
             acc[s * PN + 0] += (double)vi * vi;
@@ -1199,8 +1222,10 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax,
             acc[s * PN + 2] += (double)Av * Av;
           } else {
             const float wp = Av - cC[s] * (s == 0 ? sp0 : sp1)[e];
-            __builtin_nontemporal_store(vi, &(s == 0 ? Vout0 : Vout1)[i]);
-            __builtin_nontemporal_store(wp, &(s == 0 ? Wout0 : Wout1)[i]);
+            if (!LAST) {
+              __builtin_nontemporal_store(vi, &(s == 0 ? Vout0 : Vout1)[i]);
+              __builtin_nontemporal_store(wp, &(s == 0 ? Wout0 : Wout1)[i]);
+            }
             acc[s * PN + 0] += (double)wp * vi;
             acc[s * PN + 1] += (double)wp * wp;
             acc[s * PN + 2] += (double)vi * vi;
@@ -1208,7 +1233,27 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax,
         }
       }
     }
-    if (!more) break;
+  };
+  float nc[MSPT], nx0[MSPT], nx1[MSPT], np0[MSPT], np1[MSPT], nm[MSPT];
+  double nu[MSPT];
+  bool nok[MSPT];
+  for (int cb = 0; cb < len; cb += MSUB) {
+    const bool more = cb + MSUB < len;
+    if (more) {
+      // the next window's new cells [cb' + n, cb' + SUB + n)
+#pragma unroll
+      for (int q = 0; q < MSPT; ++q) {
+        const int i = cell(cb + MSUB + n + threadIdx.x + q * FBS);
+        nok[q] = i >= 0 && i < N;
+        fetch(clampi(i), nc[q], nx0[q], nx1[q], np0[q], np1[q], nu[q], nm[q]);
+      }
+    }
+    __syncthreads();
+    if (!more) {
+      compute(std::false_type{}, cb, len);
+      break;
+    }
+    compute(std::true_type{}, cb, cb + MSUB);
     __syncthreads(); // every read of this window is done
     // keep [cb' - n, cb' + n) (disjoint source and destination: 2n <= SUB)
     for (int e = threadIdx.x; e < 2 * n; e += FBS) {
@@ -1223,9 +1268,8 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax,
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < MSPT; ++q) {
-      const int i = cell(cb + MSUB + n + threadIdx.x + q * FBS);
-      if (i >= 0 && i < N)
-        deposit(2 * n + threadIdx.x + q * FBS, nc[q], nx0[q], nx1[q], np0[q], np1[q], nu[q], nm[q]);
+      if (!nok[q]) nc[q] = nx0[q] = nx1[q] = np0[q] = np1[q] = nm[q] = 0.0f, nu[q] = 0.0;
+      deposit(2 * n + threadIdx.x + q * FBS, nc[q], nx0[q], nx1[q], np0[q], np1[q], nu[q], nm[q]);
     }
   }
   block_sum_n<NSEQ * PN>(acc, shn);
@@ -1262,8 +1306,10 @@ inline size_t march3d_lds_bytes(int cpt, int nmax, bool uni = false) {
 // The tile-4 uniform-field stage-2 pass is held to 128 VGPRs (4 waves per SIMD instead of 3,
 // at the cost of a 12-byte spill): 3.64 -> 3.09-3.31 ms per pass on 40 x 220^3.  Stage 1 under
 // the same cap spills 36 bytes and slows from 3.68 to 4.82 ms, so it is left uncapped.
-template <int STAGE, int CPT, bool UNI = false>
+// LAST: pass j = m - 1 of the full-T Lanczos variant (partials only, nothing stored).
+template <int STAGE, int CPT, bool UNI = false, bool LAST = false>
 __global__ __launch_bounds__(FBS, (CPT == 4 && UNI && STAGE == 2) ? 4 : 1) void k_kge_march3d(KgeArgs a, int j, int emax) {
+  static_assert(!(LAST && STAGE == 0), "the last pass follows pass 0");
   extern __shared__ float lds[];
   const int2 cm = a.cmap[blockIdx.x];
   if (cm.x < 0) return;
@@ -1317,7 +1363,6 @@ __global__ __launch_bounds__(FBS, (CPT == 4 && UNI && STAGE == 2) ? 4 : 1) void 
   float *Vout1 = a.V + (STAGE >= 1 ? (1 * m2 + (size_t)(j - 1)) * cells : 0) + T.off;
   float *Wout0 = a.W + ((size_t)0 * 2 + (j & 1)) * cells + T.off;
   float *Wout1 = a.W + ((size_t)1 * 2 + (j & 1)) * cells + T.off;
-  const float scl = T.s, mu2 = T.mu2;
   const int q6 = a.q6;
   double acc[NSEQ * PN];
 #pragma unroll
@@ -1485,19 +1530,17 @@ __global__ __launch_bounds__(FBS, (CPT == 4 && UNI && STAGE == 2) ? 4 : 1) void 
       const float wxp = hxp ? 0.5f * (ci + (UNI ? ci : sc[e + 1])) : 0.0f;
       const float wyp = hyp ? 0.5f * (ci + (UNI ? ci : sc[e + n])) : 0.0f;
       const float wzp = hzp ? 0.5f * (ci + cnz[q]) : 0.0f;
-      const float wsum = ((((wzm + wym) + wxm) + wxp) + wyp) + wzp;
 #pragma unroll
       for (int s = 0; s < NSEQ; ++s) {
         const float *sv = s == 0 ? sv0 : sv1;
         const float vi = s == 0 ? vc0[q] : vc1[q];
-        float nb = 0.0f;
-        if (hzm) nb += wzm * (s == 0 ? vm0[q] : vm1[q]);
-        if (hym) nb += wym * sv[e - n];
-        if (hxm) nb += wxm * sv[e - 1];
-        if (hxp) nb += wxp * sv[e + 1];
-        if (hyp) nb += wyp * sv[e + n];
-        if (hzp) nb += wzp * (s == 0 ? vn0[q] : vn1[q]);
-        const float Av = scl * (wsum * vi - nb) + mu2 * vi;
+        float Av = 0.0f; // flux form sum_nb w (v_i - v_nb), as apply5
+        if (hzm) Av += wzm * (vi - (s == 0 ? vm0[q] : vm1[q]));
+        if (hym) Av += wym * (vi - sv[e - n]);
+        if (hxm) Av += wxm * (vi - sv[e - 1]);
+        if (hxp) Av += wxp * (vi - sv[e + 1]);
+        if (hyp) Av += wyp * (vi - sv[e + n]);
+        if (hzp) Av += wzp * (vi - (s == 0 ? vn0[q] : vn1[q]));
         if (STAGE == 0) {
           __builtin_nontemporal_store(Av, &(s == 0 ? Wout0 : Wout1)[g]);
           acc[s * PN + 0] += (double)vi * vi;
@@ -1505,8 +1548,10 @@ __global__ __launch_bounds__(FBS, (CPT == 4 && UNI && STAGE == 2) ? 4 : 1) void 
           acc[s * PN + 2] += (double)Av * Av;
         } else {
           const float wp = Av - cC[s] * (s == 0 ? pc0[q] : pc1[q]);
-          __builtin_nontemporal_store(vi, &(s == 0 ? Vout0 : Vout1)[g]);
-          __builtin_nontemporal_store(wp, &(s == 0 ? Wout0 : Wout1)[g]);
+          if (!LAST) {
+            __builtin_nontemporal_store(vi, &(s == 0 ? Vout0 : Vout1)[g]);
+            __builtin_nontemporal_store(wp, &(s == 0 ? Wout0 : Wout1)[g]);
+          }
           acc[s * PN + 0] += (double)wp * vi;
           acc[s * PN + 1] += (double)wp * wp;
           acc[s * PN + 2] += (double)vi * vi;
@@ -1718,7 +1763,7 @@ __global__ __launch_bounds__(FBS) void k_kge_sv(KgeArgs a, SnapArgs snap) {
   const float rn = 1.0f / (float)n, rn2 = 1.0f / (float)n2;
   const double *u = a.u + T.off;
   const float *cT = a.cf + T.off;
-  const double sc = T.sd, shift = -(double)T.mu2, t2 = T.tau * T.tau;
+  const double sc = T.sd, shift = -T.mu2d, t2 = T.tau * T.tau;
   auto C = [&](int r) { return UNI ? (double)T.cval : (double)cT[r]; };
   auto unew = [&](int r) {
     const double cr = C(r);
@@ -2067,9 +2112,8 @@ static Layout make_layout(const nls_traj_desc *td, int ntraj, int dim, int cpt, 
     T.chunk0 = L.total_chunks;
     L.total_chunks += T.nchunks;
     const double dx = 2.0 * td[t].L / (T.n - 1);
-    T.s = (float)(1.0 / (dx * dx));
     T.sd = 1.0 / (dx * dx);
-    T.mu2 = (float)mu2;
+    T.mu2d = mu2;
     T.tau = dt;
   }
   std::vector<int> all(ntraj);
@@ -2157,8 +2201,10 @@ __device__ inline void reduce_partials3(const NlseArgs &a, const TrajDev &T, int
   __syncthreads();
 }
 
-template <int DIM, int STAGE, int CPT>
+// LAST: pass j = m - 1 of the full-T Lanczos variant (partials only, nothing stored).
+template <int DIM, int STAGE, int CPT, bool LAST = false>
 __global__ __launch_bounds__(FBS) void k_nlse_pass(NlseArgs a, int j) {
+  static_assert(!(LAST && STAGE == 0), "the last pass follows pass 0");
   const int2 cm = a.cmap[blockIdx.x];
   if (cm.x < 0) return;
   const TrajDev T = a.traj[cm.x];
@@ -2194,7 +2240,6 @@ __global__ __launch_bounds__(FBS) void k_nlse_pass(NlseArgs a, int j) {
     }
   }
   const int n = T.n;
-  const float sc = T.s;
   const int64_t cells = a.cells;
   double acc[PN] = {0.0, 0.0, 0.0};
   const int64_t base = (int64_t)cm.y * (FBS * CPT);
@@ -2207,24 +2252,20 @@ __global__ __launch_bounds__(FBS) void k_nlse_pass(NlseArgs a, int j) {
     neighbours<DIM>(i, n, T.N, a.q6, nb);
     const float ci = a.cf[gi];
     float w[6];
-    float wsum = 0.0f;
 #pragma unroll
-    for (int q = 0; q < 2 * DIM; ++q) {
-      w[q] = nb.has[q] ? 0.5f * (ci + a.cf[gi + nb.off[q]]) : 0.0f;
-      wsum += w[q];
-    }
+    for (int q = 0; q < 2 * DIM; ++q) w[q] = nb.has[q] ? 0.5f * (ci + a.cf[gi + nb.off[q]]) : 0.0f;
     auto val = [&](int64_t g) -> c32 {
       if (STAGE == 0) return nl_half_phase(a.psi[g], a.mf[g], a.dt);
       if (STAGE == 1) return cB * (a.W[g] - cA * a.X[g]);
       return cB * (a.W[(size_t)((j - 1) & 1) * cells + g] - cA * a.V[(size_t)(j - 2) * cells + g]);
     };
     const c32 vi = val(gi);
-    c32 nbsum{0.f, 0.f};
+    // B v = sum_nb w (v_nb - v_i): div(c grad) without 1/dx^2, not negated
+    // (nlse_cubic_driver_2d.cpp:94-97), in flux form (as apply5)
+    c32 Lv{0.f, 0.f};
 #pragma unroll
     for (int q = 0; q < 2 * DIM; ++q)
-      if (nb.has[q]) nbsum = nbsum + w[q] * val(gi + nb.off[q]);
-    // L v = s * (sum_nb w v_nb - wsum v_i): div(c grad), not negated (nlse_cubic_driver_2d.cpp:94-97)
-    const c32 Lv = sc * (nbsum - wsum * vi);
+      if (nb.has[q]) Lv = Lv + w[q] * (val(gi + nb.off[q]) - vi);
     if (STAGE == 0) {
       a.X[gi] = vi;
       a.W[gi] = Lv;
@@ -2234,8 +2275,10 @@ __global__ __launch_bounds__(FBS) void k_nlse_pass(NlseArgs a, int j) {
     } else {
       const c32 vprev = STAGE == 1 ? a.X[gi] : a.V[(size_t)(j - 2) * cells + gi];
       const c32 wp = Lv - cC * vprev;
-      a.V[(size_t)(j - 1) * cells + gi] = vi;
-      a.W[(size_t)(j & 1) * cells + gi] = wp;
+      if (!LAST) {
+        a.V[(size_t)(j - 1) * cells + gi] = vi;
+        a.W[(size_t)(j & 1) * cells + gi] = wp;
+      }
       acc[0] += (double)wp.re * vi.re + (double)wp.im * vi.im;
       acc[1] += (double)wp.re * wp.re + (double)wp.im * wp.im;
       acc[2] += (double)vi.re * vi.re + (double)vi.im * vi.im;
@@ -2289,12 +2332,17 @@ __global__ __launch_bounds__(kWave) void k_nlse_small(NlseArgs a, int last_pass)
   }
   double al, b;
   lanczos_scalars(red[0], red[1], red[2], al, b);
+  const bool full = last_pass == m - 1; // as k_kge_small: full T or d_{m-1} = 0 (Q1)
   if (lane == 0) {
-    S.alpha[0][m - 2] = al;
-    S.beta[0][m - 1] = b;
+    S.alpha[0][last_pass] = al;
+    if (!full) S.beta[0][m - 1] = b;
   }
-  double dk = lane < m - 2 ? S.alpha[0][lane] : (lane == m - 2 ? al : 0.0); // d_{m-1} = 0 (Q1)
-  double ek = lane < m - 2 ? S.beta[0][lane + 1] : (lane == m - 2 ? b : 0.0);
+  // T of L = sd B (B the unscaled operator the passes apply)
+  const double alk = lane < last_pass ? S.alpha[0][lane] : (lane == last_pass ? al : 0.0);
+  const double bk = lane < m - 2 ? S.beta[0][lane + 1]
+                                 : (lane == m - 2 ? (full ? S.beta[0][m - 1] : b) : 0.0);
+  double dk = lane <= last_pass ? T.sd * alk : 0.0;
+  double ek = T.sd * bk;
   // exp(i dt T) e1 is analytic in T: a scaled Taylor series gives it without the QL
   // iteration (the eigen-decomposition route, small_ql_wave + small_apply_f_wave, takes
   // ~84 us for m = 12 against a few us here)
@@ -2413,14 +2461,16 @@ __global__ __launch_bounds__(FBS) void k_nlse_combine(NlseArgs a, SnapC snap) {
 // ---------------------------------------------------------------- NLSE 2D row march
 // The stored-w' row march of k_kge_march2d for the complex64 NLSE sequence: the real
 // and imaginary parts of v_j are two real sequences sharing the (real) Lanczos
-// scalars, L v = s (sum_nb w v_nb - wsum v) (not negated, as in k_nlse_pass), and the
+// scalars, B v = sum_nb w (v_nb - v) (not negated, as in k_nlse_pass), and the
 // partials add both parts.  STAGE 0 forms the seed psi1 = psi exp(i dt/2 m|psi|^2) when
 // a cell is deposited and stores it (X) with W[0] = L psi1; STAGE 1 forms v_1 from W[0]
 // and X; STAGE 2 is the general pass.
 typedef float f2v __attribute__((ext_vector_type(2)));
 
-template <int STAGE, bool UNI = false>
+// LAST: pass j = m - 1 of the full-T Lanczos variant (partials only, nothing stored).
+template <int STAGE, bool UNI = false, bool LAST = false>
 __global__ __launch_bounds__(FBS) void k_nlse_march2d(NlseArgs a, int j, int emax, int alt) {
+  static_assert(!(LAST && STAGE == 0), "the last pass follows pass 0");
   extern __shared__ float lds[];
   const int2 cm = a.cmap[blockIdx.x];
   if (cm.x < 0) return;
@@ -2441,7 +2491,6 @@ __global__ __launch_bounds__(FBS) void k_nlse_march2d(NlseArgs a, int j, int ema
   c32 *Vout = a.V + (STAGE >= 1 ? (size_t)(j - 1) * cells : 0) + T.off;
   c32 *Wout = a.W + (size_t)(j & 1) * cells + T.off;
   c32 *Xout = a.X + T.off;
-  const float scl = T.s;
   double acc[PN] = {0.0, 0.0, 0.0};
   const int rbeg = cm.y * T.range;
   const int rend = rbeg + T.range < N ? rbeg + T.range : N;
@@ -2479,15 +2528,10 @@ __global__ __launch_bounds__(FBS) void k_nlse_march2d(NlseArgs a, int j, int ema
       sp1[e] = pv.im;
     }
   };
-  // L v at LDS index e: s (sum_nb w v_nb - wsum v)
+  // B v at LDS index e: sum_nb w (v_nb - v) in flux form (the small stage scales T by
+  // 1/dx^2): minus apply5
   auto lap = [&](const Sten5 &w, const float *v, int e) {
-    const float vi = v[e];
-    float nb = 0.0f;
-    if (w.hu) nb += w.wu * v[e + oU];
-    if (w.hl) nb += w.wl * v[e + oL];
-    if (w.hr) nb += w.wr * v[e + oR];
-    if (w.hd) nb += w.wd * v[e + oD];
-    return scl * (nb - w.wsum * vi);
+    return -apply5(w, v, e, oU, oL, oR, oD);
   };
   // Lanczos scalars of the previous pass; formed after the first window loads are
   // issued, so the reduction's latency overlaps theirs
@@ -2554,7 +2598,7 @@ __global__ __launch_bounds__(FBS) void k_nlse_march2d(NlseArgs a, int j, int ema
       const int l = cb + threadIdx.x + kk * FBS;
       if (l < ce) {
         const int i = cell(l), e = l - cb + n;
-        const Sten5 w = UNI ? sten5_uniform(T.cval, i, n, rn) : sten5(sc, e, oU, oL, oR, oD, i, n, rn);
+        const Sten5 w = UNI ? sten5_uniform(T.cval, i, n, N, rn) : sten5(sc, e, oU, oL, oR, oD, i, n, N, rn);
         const float vr = sv0[e], vi = sv1[e];
         const float lr = lap(w, sv0, e), li = lap(w, sv1, e);
         if (STAGE == 0) {
@@ -2565,8 +2609,10 @@ __global__ __launch_bounds__(FBS) void k_nlse_march2d(NlseArgs a, int j, int ema
           acc[2] += (double)lr * lr + (double)li * li;
         } else {
           const float wr = lr - cC * sp0[e], wi = li - cC * sp1[e];
-          store2(Vout + i, vr, vi);
-          store2(Wout + i, wr, wi);
+          if (!LAST) {
+            store2(Vout + i, vr, vi);
+            store2(Wout + i, wr, wi);
+          }
           acc[0] += (double)wr * vr + (double)wi * vi;
           acc[1] += (double)wr * wr + (double)wi * wi;
           acc[2] += (double)vr * vr + (double)vi * vi;
@@ -2600,13 +2646,15 @@ __global__ __launch_bounds__(FBS) void k_nlse_march2d(NlseArgs a, int j, int ema
 }
 
 // The plane march of k_kge_march3d for the complex64 NLSE sequence: real and imaginary
-// parts of v_j share the (real) Lanczos scalars, L v = s (sum_nb w v_nb - wsum v), and the
+// parts of v_j share the (real) Lanczos scalars, B v = sum_nb w (v_nb - v), and the
 // partials add both parts; STAGE 0 forms the seed psi1 = psi exp(i dt/2 m|psi|^2) at each
 // cell it loads and stores it (X) with W[0] = L psi1, STAGE 1 forms v_1 from W[0] and X,
 // STAGE 2 is the general pass.  Same window, registers and cross-plane couplings as the
 // Klein-Gordon kernel.
-template <int STAGE, int CPT, bool UNI = false>
+// LAST: pass j = m - 1 of the full-T Lanczos variant (partials only, nothing stored).
+template <int STAGE, int CPT, bool UNI = false, bool LAST = false>
 __global__ __launch_bounds__(FBS) void k_nlse_march3d(NlseArgs a, int j, int emax) {
+  static_assert(!(LAST && STAGE == 0), "the last pass follows pass 0");
   extern __shared__ float lds[];
   const int2 cm = a.cmap[blockIdx.x];
   if (cm.x < 0) return;
@@ -2652,7 +2700,6 @@ __global__ __launch_bounds__(FBS) void k_nlse_march3d(NlseArgs a, int j, int ema
   c32 *Vout = a.V + (STAGE >= 1 ? (size_t)(j - 1) * cells : 0) + T.off;
   c32 *Wout = a.W + (size_t)(j & 1) * cells + T.off;
   c32 *Xout = a.X + T.off;
-  const float scl = T.s;
   const int q6 = a.q6;
   double acc[PN] = {0.0, 0.0, 0.0};
   auto store2 = [](c32 *p, float re, float im) {
@@ -2798,20 +2845,19 @@ __global__ __launch_bounds__(FBS) void k_nlse_march3d(NlseArgs a, int j, int ema
       const float wxp = hxp ? 0.5f * (ci + (UNI ? ci : sc[e + 1])) : 0.0f;
       const float wyp = hyp ? 0.5f * (ci + (UNI ? ci : sc[e + n])) : 0.0f;
       const float wzp = hzp ? 0.5f * (ci + cnz[q]) : 0.0f;
-      const float wsum = ((((wzm + wym) + wxm) + wxp) + wyp) + wzp;
       float lv[2];
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
         const float *sv = s == 0 ? sv0 : sv1;
         const float vi = s == 0 ? vc0[q] : vc1[q];
-        float nb = 0.0f;
-        if (hzm) nb += wzm * (s == 0 ? vm0[q] : vm1[q]);
-        if (hym) nb += wym * sv[e - n];
-        if (hxm) nb += wxm * sv[e - 1];
-        if (hxp) nb += wxp * sv[e + 1];
-        if (hyp) nb += wyp * sv[e + n];
-        if (hzp) nb += wzp * (s == 0 ? vn0[q] : vn1[q]);
-        lv[s] = scl * (nb - wsum * vi);
+        float r = 0.0f; // flux form, minus the apply5 sum
+        if (hzm) r += wzm * (vi - (s == 0 ? vm0[q] : vm1[q]));
+        if (hym) r += wym * (vi - sv[e - n]);
+        if (hxm) r += wxm * (vi - sv[e - 1]);
+        if (hxp) r += wxp * (vi - sv[e + 1]);
+        if (hyp) r += wyp * (vi - sv[e + n]);
+        if (hzp) r += wzp * (vi - (s == 0 ? vn0[q] : vn1[q]));
+        lv[s] = -r;
       }
       const float vr = vc0[q], vi = vc1[q];
       if (STAGE == 0) {
@@ -2822,8 +2868,10 @@ __global__ __launch_bounds__(FBS) void k_nlse_march3d(NlseArgs a, int j, int ema
         acc[2] += (double)lv[0] * lv[0] + (double)lv[1] * lv[1];
       } else {
         const float wr = lv[0] - cC * pc0[q], wi = lv[1] - cC * pc1[q];
-        store2(Vout + g, vr, vi);
-        store2(Wout + g, wr, wi);
+        if (!LAST) {
+          store2(Vout + g, vr, vi);
+          store2(Wout + g, wr, wi);
+        }
         acc[0] += (double)wr * vr + (double)wi * vi;
         acc[1] += (double)wr * wr + (double)wi * wi;
         acc[2] += (double)vr * vr + (double)vi * vi;
@@ -3198,7 +3246,8 @@ struct nls_batch {
           constexpr bool U = decltype(u)::value;
           if (stage == 0) hipLaunchKernelGGL((k_nlse_march3d<0, CPT, U>), grid, blk, lds3, s, a, j, emax3);
           else if (stage == 1) hipLaunchKernelGGL((k_nlse_march3d<1, CPT, U>), grid, blk, lds3, s, a, j, emax3);
-          else hipLaunchKernelGGL((k_nlse_march3d<2, CPT, U>), grid, blk, lds3, s, a, j, emax3);
+          else if (stage == 2) hipLaunchKernelGGL((k_nlse_march3d<2, CPT, U>), grid, blk, lds3, s, a, j, emax3);
+          else hipLaunchKernelGGL((k_nlse_march3d<2, CPT, U, true>), grid, blk, lds3, s, a, j, emax3);
         });
       }
     };
@@ -3212,7 +3261,8 @@ struct nls_batch {
         constexpr bool U = decltype(u)::value;
         if (stage == 0) hipLaunchKernelGGL((k_nlse_march2d<0, U>), grid, blk, lds, s, a, j, emax, 1);
         else if (stage == 1) hipLaunchKernelGGL((k_nlse_march2d<1, U>), grid, blk, lds, s, a, j, emax, 1);
-        else hipLaunchKernelGGL((k_nlse_march2d<2, U>), grid, blk, lds, s, a, j, emax, 1);
+        else if (stage == 2) hipLaunchKernelGGL((k_nlse_march2d<2, U>), grid, blk, lds, s, a, j, emax, 1);
+        else hipLaunchKernelGGL((k_nlse_march2d<2, U, true>), grid, blk, lds, s, a, j, emax, 1);
       });
     };
     if (march) {
@@ -3234,7 +3284,12 @@ struct nls_batch {
       if (prof) NLS_HIP(hipEventRecord(next_event(), s));
       reorth(j);
     }
-    hipLaunchKernelGGL(k_nlse_small, dim3(nt), dim3(kWave), 0, s, a, m - 2);
+    if (p.variant == 1) { // full T: the partials of alpha_{m-1} (stage 3 = last pass)
+      if (march) march_pass(3, m - 1);
+      else if (plane) plane_pass(3, m - 1);
+      else hipLaunchKernelGGL((k_nlse_pass<DIM, 2, CPT, true>), grid, blk, 0, s, a, m - 1);
+    }
+    hipLaunchKernelGGL(k_nlse_small, dim3(nt), dim3(kWave), 0, s, a, p.variant == 1 ? m - 1 : m - 2);
     SnapC sn{nullptr};
     if (snapshot) sn.psi = d_snapU + (size_t)slot * lay.cells * 2;
     if (march || plane) {
@@ -3377,33 +3432,36 @@ struct nls_batch {
   // chunk + halo rows (2D), 2 = row march with an LDS window (2D), 3 = row march
   // that stores only the basis and recomputes w' (2D, launch_rpass), 4 = plane march
   // (3D, in-plane ranges of 256 * cpt cells).
-  template <int DIM, int STAGE> void launch_pass(const KgeArgs &a, int j, hipStream_t s,
-                                                 unsigned ngrid) {
+  // LAST: the extra pass j = m - 1 of the full-T variant (kinds 0, 2 and 4 only).
+  template <int DIM, int STAGE, bool LAST = false>
+  void launch_pass(const KgeArgs &a, int j, hipStream_t s, unsigned ngrid) {
     const dim3 grid(ngrid), blk(FBS);
     if (pass_kind == 4) {
       const size_t lds = march3d_lds_bytes(cpt, nmax, uni_all);
       const int emax = FBS * cpt + 2 * nmax;
       with_uni([&](auto u) {
         constexpr bool U = decltype(u)::value;
-        if (cpt == 4) hipLaunchKernelGGL((k_kge_march3d<STAGE, 4, U>), grid, blk, lds, s, a, j, emax);
-        else hipLaunchKernelGGL((k_kge_march3d<STAGE, 8, U>), grid, blk, lds, s, a, j, emax);
+        if (cpt == 4) hipLaunchKernelGGL((k_kge_march3d<STAGE, 4, U, LAST>), grid, blk, lds, s, a, j, emax);
+        else hipLaunchKernelGGL((k_kge_march3d<STAGE, 8, U, LAST>), grid, blk, lds, s, a, j, emax);
       });
     } else if (pass_kind == 2) {
       const size_t lds = march2d_lds_bytes(nmax, uni_all);
       const int emax = MSUB + 2 * nmax;
       with_uni([&](auto u) {
         constexpr bool U = decltype(u)::value;
-        hipLaunchKernelGGL((k_kge_march2d<STAGE, U>), grid, blk, lds, s, a, j, emax, 1);
+        hipLaunchKernelGGL((k_kge_march2d<STAGE, U, LAST>), grid, blk, lds, s, a, j, emax, 1);
       });
     } else if (pass_kind == 1) {
-      const size_t lds = pass2d_lds_bytes(cpt, nmax);
-      const int emax = FBS * cpt + 2 * nmax;
-      if (cpt == 4) hipLaunchKernelGGL((k_kge_pass2d<STAGE, 4>), grid, blk, lds, s, a, j, emax);
-      else hipLaunchKernelGGL((k_kge_pass2d<STAGE, 8>), grid, blk, lds, s, a, j, emax);
+      if constexpr (!LAST) {
+        const size_t lds = pass2d_lds_bytes(cpt, nmax);
+        const int emax = FBS * cpt + 2 * nmax;
+        if (cpt == 4) hipLaunchKernelGGL((k_kge_pass2d<STAGE, 4>), grid, blk, lds, s, a, j, emax);
+        else hipLaunchKernelGGL((k_kge_pass2d<STAGE, 8>), grid, blk, lds, s, a, j, emax);
+      }
     } else {
-      if (cpt == 4) hipLaunchKernelGGL((k_kge_pass<DIM, STAGE, 4>), grid, blk, 0, s, a, j);
-      else if (cpt == 8) hipLaunchKernelGGL((k_kge_pass<DIM, STAGE, 8>), grid, blk, 0, s, a, j);
-      else hipLaunchKernelGGL((k_kge_pass<DIM, STAGE, 16>), grid, blk, 0, s, a, j);
+      if (cpt == 4) hipLaunchKernelGGL((k_kge_pass<DIM, STAGE, 4, LAST>), grid, blk, 0, s, a, j);
+      else if (cpt == 8) hipLaunchKernelGGL((k_kge_pass<DIM, STAGE, 8, LAST>), grid, blk, 0, s, a, j);
+      else hipLaunchKernelGGL((k_kge_pass<DIM, STAGE, 16, LAST>), grid, blk, 0, s, a, j);
     }
   }
 
@@ -3453,8 +3511,13 @@ struct nls_batch {
         if (prof) NLS_HIP(hipEventRecord(next_event(), st));
         launch_reorth(a, j, st, ngrid);
       }
+      if (p.variant == 1) { // full T: the partials of alpha_{m-1}
+        if (m - 1 == 1) launch_pass<DIM, 1, true>(a, 1, st, ngrid);
+        else launch_pass<DIM, 2, true>(a, m - 1, st, ngrid);
+      }
     }
-    hipLaunchKernelGGL(k_kge_small, dim3(nt * NSEQ), dim3(kWave), 0, st, a, m - 2);
+    hipLaunchKernelGGL(k_kge_small, dim3(nt * NSEQ), dim3(kWave), 0, st, a,
+                       p.variant == 1 ? m - 1 : m - 2);
     if (snapshot) {
       SnapArgs sn{d_snapU + (size_t)slot * lay.cells, d_snapV + (size_t)slot * lay.cells};
       with_uni([&](auto u) {
@@ -3530,7 +3593,9 @@ struct nls_batch {
              (kind == 2 && p.dim == 2 && 2 * nm <= MSUB && march2d_lds_bytes(nm) <= 65536 &&
               c >= 4 && c <= 64) ||
              (kind == 4 && p.dim == 3 && (c == 4 || c == 8) && 2 * nm <= FBS * M3_HPT);
-    // row-march kernels take any range length (tile = target cells per workgroup / 256)
+    // row-march kernels take any range length (tile = target cells per workgroup / 256);
+    // kinds 1 and 3 have no last pass of the full-T variant
+    if ((kind == 1 || kind == 3) && p.variant != 0) return false;
     if (kind == 3) return p.reorth == 0 && p.dim == 2 && 3 * nm <= RKS * FBS && 2 * nm <= MSUB &&
                           rmarch2d_lds_bytes(nm) <= 65536 && c >= 4 && c <= 64;
     if (kind == 2) return p.dim == 2 && 2 * nm <= MSUB && march2d_lds_bytes(nm) <= 65536 &&
@@ -3768,15 +3833,23 @@ static size_t estimate(const nls_batch_params &p, int ntraj, const nls_traj_desc
   return b;
 }
 
+// Krylov dimension: the caller's, or the a-priori bound below.  The reference's variant
+// (T[m-1][m-1] = 0) is ill-conditioned at small m on smooth, symmetric data: rounding the
+// input to FP32 moves the FP64 result by up to ~1e-4 relative over a few hundred steps at
+// m = 8 and ~3e-6 at m = 10, while the full-T variant moves ~3e-8 at every m.  So the
+// reference variant is never tuned below the reference drivers' m = 10
+// (kg_driver_dev_2d.cpp:138).
 static int pick_m(const nls_batch_params &p, int ntraj, const nls_traj_desc *td) {
   if (p.krylov_dim > 0) return p.krylov_dim;
   std::vector<double> r(ntraj);
   for (int t = 0; t < ntraj; ++t) {
     const double dx = 2.0 * td[t].L / (td[t].n - 1);
-    // Gershgorin bound with max(c) <= 4 assumed for an unset field; refined in set_initial
+    // Gershgorin bound of the operator with max(c) <= 4 assumed (the fields are not
+    // known when the batch, and with it the Krylov basis, is sized)
     r[t] = (4.0 * p.dim * 4.0 / (dx * dx) + p.mu2) * p.dt * p.dt;
   }
-  return autotune_krylov(r);
+  const int m = autotune_krylov(r);
+  return p.variant == 0 ? std::max(m, 10) : m;
 }
 
 extern "C" {
@@ -3816,6 +3889,9 @@ int nls_batch_create(const nls_batch_params *p, int ntraj, const nls_traj_desc *
                 "always_load_fields must be 0 or 1");
     NLS_REQUIRE(p->reorth == 0 || p->pass_kernel != 4,
                 "reorthogonalisation needs a pass kernel that stores w' (not kernel 3)");
+    NLS_REQUIRE(p->variant == 0 || p->variant == 1, "variant must be 0 or 1");
+    NLS_REQUIRE(p->variant == 0 || (p->pass_kernel != 2 && p->pass_kernel != 4),
+                "the full-T Lanczos variant runs on pass kernels 0, 2 and 4");
     NLS_REQUIRE(p->lanes >= 0 && p->lanes <= nls_batch::MAXL, "lanes must be in [0, 4]");
     NLS_REQUIRE(p->cells_per_thread == 0 ||
                     (p->cells_per_thread >= 4 && p->cells_per_thread <= 64),

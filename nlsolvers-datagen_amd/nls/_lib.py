@@ -42,7 +42,8 @@ class BatchParams(C.Structure):
                 ("gamma", C.c_double), ("use_graph", C.c_int), ("snapshot_every", C.c_int),
                 ("max_snapshots", C.c_int), ("cells_per_thread", C.c_int),
                 ("pass_kernel", C.c_int), ("lanes", C.c_int), ("reorth", C.c_int),
-                ("reorth_tol", C.c_double), ("always_load_fields", C.c_int)]
+                ("reorth_tol", C.c_double), ("always_load_fields", C.c_int),
+                ("variant", C.c_int)]
 
 
 class ServerParams(C.Structure):

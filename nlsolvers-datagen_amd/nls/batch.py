@@ -14,6 +14,13 @@ from ._lib import as_f64, call, dptr
 IC_KINDS = {"gaussians_2d": 0, "kink_ring_2d": 1, "packets_2d": 2, "bubbles_3d": 3}
 
 
+def lanczos_variant(v):
+    """Lanczos variant code: "reference" (0) keeps the reference's T[m-1][m-1] = 0
+    (device/include/lanczos.hpp:234-263); "full" (1) computes alpha_{m-1} with one more
+    pass per step, the full tridiagonal T."""
+    return {"reference": 0, "ref": 0, "full": 1, "textbook": 1}.get(v, v)
+
+
 def ic_params(kind, seed, traj):
     """The sampler parameters drawn for one trajectory, as the library draws them
     (host only; see nls_ic_draw for the layout)."""
@@ -28,7 +35,7 @@ class FusedBatch:
                  mu2=0.0, q6=True, gamma=0.0, use_graph=True, snapshot_every=0,
                  max_snapshots=0, sigmas=None, seeds=None, cells_per_thread=0,
                  pass_kernel=None, lanes=0, reorth=0, reorth_tol=0.0,
-                 always_load_fields=False):
+                 always_load_fields=False, variant="reference"):
         eq = {"kge": L.EQ_KGE, "nlse": L.EQ_NLSE, "sp4": L.EQ_SP4}[equation]
         meth = {"gautschi": L.KGE_GAUTSCHI, "sewi": L.KGE_SEWI, "sv": L.KGE_SV,
                 "strang": L.NLSE_STRANG, "reference": L.SP4_REFERENCE,
@@ -44,7 +51,7 @@ class FusedBatch:
                                snapshot_every, max_snapshots, cells_per_thread,
                                0 if pass_kernel is None else pass_kernel + 1, lanes,
                                {"never": 0, "always": 1, "auto": 2}.get(reorth, reorth),
-                               reorth_tol, int(always_load_fields))
+                               reorth_tol, int(always_load_fields), lanczos_variant(variant))
         self.td = (L.TrajDesc * k)(*[L.TrajDesc(n, l, s, sd)
                                      for n, l, s, sd in zip(self.ns, Ls, sigmas, seeds)])
         self.h = C.c_void_p()
@@ -52,12 +59,12 @@ class FusedBatch:
 
     @staticmethod
     def estimate_bytes(ns_list, Ls, dt, dim=2, krylov_dim=0, mu2=0.0, max_snapshots=0,
-                       equation="kge"):
+                       equation="kge", variant="reference"):
         k = len(ns_list)
         Ls = list(Ls) if np.ndim(Ls) else [Ls] * k
         eq = {"kge": L.EQ_KGE, "nlse": L.EQ_NLSE, "sp4": L.EQ_SP4}[equation]
         p = L.BatchParams(eq, 0, dim, dt, krylov_dim, mu2, 1, 0.0, 1, 0, max_snapshots, 0, 0, 0,
-                          0, 0.0)
+                          0, 0.0, 0, lanczos_variant(variant))
         td = (L.TrajDesc * k)(*[L.TrajDesc(n, l, 0.0, 0) for n, l in zip(ns_list, Ls)])
         out = C.c_size_t()
         call("nls_batch_estimate_bytes", C.byref(p), k, td, C.byref(out))
