@@ -108,6 +108,20 @@ NLS_API int nls_matfunc_destroy(nls_matfunc *h);
  * Replaces MatrixFunctionApplicatorReal::apply (matfunc_real.hpp:190-313). */
 NLS_API int nls_matfunc_apply_real(nls_matfunc *h, double *out, const double *in, double t,
                                    int kind);
+/* Same as nls_matfunc_apply_real with no handle and no allocation: the caller passes the
+ * device CSR (row_ptr, col, val of an n x n operator), in and out (device, may alias),
+ * the Krylov dimension m (2..32) and a stream (NULL = default); the Krylov workspace is a
+ * fixed device array of 32 MiB held by each GPU the library runs on (the sizes it takes:
+ * 8 n (m + 1) bytes plus ~280 KB, e.g. n = 300^2 up to m = 32), and larger requests fail
+ * with a message.  It calls no hipMalloc, hipFree or allocating library routine.  It
+ * synchronises the stream to report an eigensolver failure, except while the stream is
+ * being captured into a HIP graph.  All calls on one GPU share its array: calls that
+ * overlap on one GPU (other streams or threads) race.  No counterpart in the reference:
+ * for callers that cannot allocate (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). */
+NLS_API int nls_matfunc_apply_real_noalloc(int64_t n, const int32_t *row_ptr, const int32_t *col,
+                                           const double *val, double *out, const double *in,
+                                           double t, int kind, int m, void *stream);
 /* Replaces MatrixFunctionApplicatorComplex::apply (matfunc_complex.hpp:254-375). */
 NLS_API int nls_matfunc_apply_complex(nls_matfunc *h, double *out, const double *in, double dt_re,
                                       double dt_im, int kind);

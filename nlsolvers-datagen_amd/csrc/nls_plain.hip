@@ -17,6 +17,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 namespace nls {
@@ -507,33 +509,83 @@ static int reduce_grid(int64_t n) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(512, (n + 4 * BS - 1) / (4 * BS)));
 }
 
-template <class S> PlainKrylov<S>::PlainKrylov(int64_t n_, int m_, hipStream_t s_)
+// Zeroes `words` 4-byte words (a kernel rather than hipMemsetAsync, so that a caller may
+// capture the calls into a HIP graph).
+__global__ __launch_bounds__(BS) void k_zero_words(uint32_t *__restrict__ p, int64_t words) {
+  for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < words; i += (int64_t)gridDim.x * BS)
+    p[i] = 0u;
+}
+
+static void zero_words(void *p, size_t bytes, hipStream_t s) {
+  const int64_t words = (int64_t)(bytes / 4);
+  const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(64, (words + BS - 1) / BS));
+  hipLaunchKernelGGL(k_zero_words, dim3(g), dim3(BS), 0, s, static_cast<uint32_t *>(p), words);
+}
+
+// Byte offsets of the buffers in the workspace block (each 256-byte aligned).
+namespace {
+struct KrylovCarve {
+  size_t V, w, T, part, h, y, scal, dscal, status, total;
+  template <class S> static KrylovCarve of(int64_t n, int m, int G) {
+    KrylovCarve c{};
+    size_t o = 0;
+    auto take = [&o](size_t bytes) {
+      const size_t at = o;
+      o += (bytes + 255) / 256 * 256;
+      return at;
+    };
+    c.V = take(sizeof(S) * (size_t)n * m);
+    c.w = take(sizeof(S) * (size_t)n);
+    c.T = take(sizeof(S) * (size_t)m * m);
+    c.part = take(sizeof(double) * 2 * (size_t)G * (m + 1));
+    c.h = take(sizeof(S) * kMaxKrylov);
+    c.y = take(sizeof(S) * kMaxKrylov);
+    c.scal = take(sizeof(S) * 8);
+    c.dscal = take(sizeof(double) * 8);
+    c.status = take(sizeof(int));
+    c.total = o;
+    return c;
+  }
+};
+} // namespace
+
+template <class S> size_t PlainKrylov<S>::workspace_bytes(int64_t n, int m) {
+  return KrylovCarve::of<S>(n, m, reduce_grid(n)).total;
+}
+
+template <class S>
+PlainKrylov<S>::PlainKrylov(int64_t n_, int m_, hipStream_t s_, char *mem)
     : n(n_), m(m_), stream(s_) {
   NLS_REQUIRE(m >= 2 && m <= kMaxKrylov, "Krylov dimension must be in [2, 32]");
   G = reduce_grid(n);
-  NLS_HIP(hipMalloc(&V, sizeof(S) * n * m));
-  NLS_HIP(hipMalloc(&w, sizeof(S) * n));
-  NLS_HIP(hipMalloc(&T, sizeof(S) * m * m));
-  NLS_HIP(hipMalloc(&part, sizeof(double) * 2 * (size_t)G * (m + 1)));
-  NLS_HIP(hipMalloc(&h, sizeof(S) * kMaxKrylov));
-  NLS_HIP(hipMalloc(&y, sizeof(S) * kMaxKrylov));
-  NLS_HIP(hipMalloc(&scal, sizeof(S) * 8));
-  NLS_HIP(hipMalloc(&dscal, sizeof(double) * 8));
-  NLS_HIP(hipMalloc(&status, sizeof(int)));
-  NLS_HIP(hipMemsetAsync(status, 0, sizeof(int), stream));
+  const KrylovCarve c = KrylovCarve::of<S>(n, m, G);
+  if (mem) {
+    block = mem;
+  } else {
+    NLS_HIP(hipMalloc(&block, c.total));
+    owns = true;
+  }
+  V = reinterpret_cast<S *>(block + c.V);
+  w = reinterpret_cast<S *>(block + c.w);
+  T = reinterpret_cast<S *>(block + c.T);
+  part = reinterpret_cast<double *>(block + c.part);
+  h = reinterpret_cast<S *>(block + c.h);
+  y = reinterpret_cast<S *>(block + c.y);
+  scal = reinterpret_cast<S *>(block + c.scal);
+  dscal = reinterpret_cast<double *>(block + c.dscal);
+  status = reinterpret_cast<int *>(block + c.status);
+  zero_words(status, sizeof(int), stream);
 }
 
 template <class S> PlainKrylov<S>::~PlainKrylov() {
-  (void)hipFree(V); (void)hipFree(w); (void)hipFree(T); (void)hipFree(part);
-  (void)hipFree(h); (void)hipFree(y); (void)hipFree(scal); (void)hipFree(dscal);
-  (void)hipFree(status);
+  if (owns) (void)hipFree(block);
 }
 
 // Reference Lanczos (lanczos.hpp:208-267 / lanczos_complex.hpp:320-519).
-template <class S> void PlainKrylov<S>::lanczos(const CsrDev &A, const S *in) {
+template <class S> void PlainKrylov<S>::lanczos(const CsrView &A, const S *in) {
   const dim3 g(G), b(BS);
   hipStream_t s = stream;
-  NLS_HIP(hipMemsetAsync(T, 0, sizeof(S) * m * m, s));
+  zero_words(T, sizeof(S) * m * m, s);
   double *beta0 = dscal + 0, *inv0 = dscal + 1, *invb = dscal + 2;
   S *negh = scal + 0, *nega = scal + 1;
   hipLaunchKernelGGL((k_dot_s1<S>), g, b, 0, s, n, in, in, part);
@@ -566,7 +618,7 @@ template <class S> void PlainKrylov<S>::lanczos(const CsrDev &A, const S *in) {
 }
 
 template <class S>
-void PlainKrylov<S>::apply(const CsrDev &A, S *out, const S *in, double t_re, double t_im,
+void PlainKrylov<S>::apply(const CsrView &A, S *out, const S *in, double t_re, double t_im,
                            int kind) {
   lanczos(A, in);
   hipLaunchKernelGGL((k_small_plain<S>), dim3(1), dim3(kWave), 0, stream, m, T, kind, t_re, t_im,
@@ -680,10 +732,10 @@ struct nls_kge {
   }
   void step() { // kg_single.cuh:94-126
     const unsigned g = blocks_for(N, BS);
-    kry->apply(*A, b2->as<double>(), u->as<double>(), dt, 0.0, F_COS_SQRT);
+    kry->apply(A->view(), b2->as<double>(), u->as<double>(), dt, 0.0, F_COS_SQRT);
     hipLaunchKernelGGL(k_neg_mu3, dim3(g), dim3(BS), 0, s, N, m->as<double>(), u->as<double>(),
                        b3->as<double>());
-    kry->apply(*A, b3->as<double>(), b3->as<double>(), dt, 0.0,
+    kry->apply(A->view(), b3->as<double>(), b3->as<double>(), dt, 0.0,
                p.method == NLS_KGE_SEWI ? F_SINC2_HALF : F_SINC2_SQRT);
     hipLaunchKernelGGL(k_gautschi_update, dim3(g), dim3(BS), 0, s, N, u->as<double>(),
                        up->as<double>(), v->as<double>(), b2->as<double>(), b3->as<double>(), dt);
@@ -758,7 +810,7 @@ struct nls_nlse {
     const unsigned g = blocks_for(N, BS);
     hipLaunchKernelGGL(k_nl_phase, dim3(g), dim3(BS), 0, s, N, buf->as<c128>(), u->as<c128>(),
                        m->as<double>(), 0.5 * dt);
-    kry->apply(*A, u->as<c128>(), buf->as<c128>(), 0.0, dt, F_EXP);
+    kry->apply(A->view(), u->as<c128>(), buf->as<c128>(), 0.0, dt, F_EXP);
     hipLaunchKernelGGL(k_nl_phase, dim3(g), dim3(BS), 0, s, N, u->as<c128>(), u->as<c128>(),
                        m->as<double>(), 0.5 * dt);
     check_launch("nlse step");
@@ -774,9 +826,9 @@ struct nls_nlse {
     NLS_HIP(hipMemcpyAsync(buf->p, u->p, sizeof(c128) * N, hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_nlse_B, dim3(g), dim3(BS), 0, s, N, b2->as<c128>(), u->as<c128>(),
                        m->as<double>());
-    kry->apply(*A, b3->as<c128>(), b2->as<c128>(), dt, 0.0, F_SINC);
-    kry->apply(*A, b2->as<c128>(), b3->as<c128>(), 0.0, dt, F_EXP);
-    kry->apply(*A, b3->as<c128>(), uprev->as<c128>(), 0.0, 2.0 * dt, F_EXP);
+    kry->apply(A->view(), b3->as<c128>(), b2->as<c128>(), dt, 0.0, F_SINC);
+    kry->apply(A->view(), b2->as<c128>(), b3->as<c128>(), 0.0, dt, F_EXP);
+    kry->apply(A->view(), b3->as<c128>(), uprev->as<c128>(), 0.0, 2.0 * dt, F_EXP);
     hipLaunchKernelGGL(k_sewi_combine, dim3(g), dim3(BS), 0, s, N, u->as<c128>(), b2->as<c128>(),
                        b3->as<c128>(), dt);
     NLS_HIP(hipMemcpyAsync(uprev->p, buf->p, sizeof(c128) * N, hipMemcpyDeviceToDevice, s));
@@ -868,6 +920,31 @@ struct nls_matfunc {
   }
 };
 
+// ------------------------------------------------------------------ fixed scratch
+// Krylov workspace of nls_matfunc_apply_real_noalloc: a file-scope device array, so that
+// entry point allocates nothing.  Every GPU the library runs on holds its own copy (at
+// its own address) until the process ends.  On the host `scratch` names a stand-in, not
+// device memory: only the per-device address from hipGetSymbolAddress is used.
+constexpr size_t kScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+static char *scratch_for_current_device() {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static char *addr[kMaxDevices] = {};
+  int dev = 0;
+  NLS_HIP(hipGetDevice(&dev));
+  NLS_REQUIRE(dev >= 0 && dev < kMaxDevices, "device index out of range");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    NLS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    NLS_REQUIRE(p != nullptr, "hipGetSymbolAddress returned no address for the scratch array");
+    addr[dev] = static_cast<char *>(p);
+  }
+  return addr[dev];
+}
+
 // ------------------------------------------------------------------ C ABI
 static void check_params_common(int dim, int n, double L, double T, int nt, int ns) {
   NLS_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
@@ -922,8 +999,31 @@ int nls_matfunc_apply_real(nls_matfunc *h, double *out, const double *in, double
   return guard([&] {
     NLS_REQUIRE(h && h->kr, "nls_matfunc_apply_real: handle has no real workspace");
     NLS_REQUIRE(kind >= 0 && kind <= 4, "unknown real function kind");
-    h->kr->apply(*h->A, out, in, t, 0.0, kind);
+    h->kr->apply(h->A->view(), out, in, t, 0.0, kind);
     h->kr->check_status();
+  });
+}
+int nls_matfunc_apply_real_noalloc(int64_t n, const int32_t *row_ptr, const int32_t *col,
+                                   const double *val, double *out, const double *in, double t,
+                                   int kind, int m, void *stream) {
+  return guard([&] {
+    NLS_REQUIRE(row_ptr && col && val && out && in, "nls_matfunc_apply_real_noalloc: null argument");
+    NLS_REQUIRE(n >= 1, "nls_matfunc_apply_real_noalloc: n must be >= 1");
+    NLS_REQUIRE(m >= 2 && m <= kMaxKrylov, "nls_matfunc_apply_real_noalloc: m must be in [2, 32]");
+    NLS_REQUIRE(kind >= 0 && kind <= 4, "unknown real function kind");
+    const size_t need = PlainKrylov<double>::workspace_bytes(n, m);
+    NLS_REQUIRE(need <= kScratchBytes,
+                "nls_matfunc_apply_real_noalloc: n = " + std::to_string(n) + ", m = " +
+                    std::to_string(m) + " needs " + std::to_string(need) +
+                    " bytes of Krylov workspace, more than the fixed scratch of " +
+                    std::to_string(kScratchBytes) + " bytes (use nls_matfunc_create)");
+    char *ws = scratch_for_current_device();
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    PlainKrylov<double> kr(n, m, s, ws); // a view of the scratch: allocates nothing
+    kr.apply(CsrView{n, row_ptr, col, val}, out, in, t, 0.0, kind);
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    NLS_HIP(hipStreamIsCapturing(s, &cap));
+    if (cap == hipStreamCaptureStatusNone) kr.check_status(); // synchronises the stream
   });
 }
 int nls_matfunc_apply_complex(nls_matfunc *h, double *out, const double *in, double dt_re,
@@ -931,7 +1031,7 @@ int nls_matfunc_apply_complex(nls_matfunc *h, double *out, const double *in, dou
   return guard([&] {
     NLS_REQUIRE(h && h->kc, "nls_matfunc_apply_complex: handle has no complex workspace");
     NLS_REQUIRE(kind == F_EXP || kind == F_SINC, "unknown complex function kind");
-    h->kc->apply(*h->A, reinterpret_cast<c128 *>(out), reinterpret_cast<const c128 *>(in), dt_re,
+    h->kc->apply(h->A->view(), reinterpret_cast<c128 *>(out), reinterpret_cast<const c128 *>(in), dt_re,
                  dt_im, kind);
     h->kc->check_status();
   });

@@ -14,6 +14,13 @@ struct c64 {
   float re, im;
 };
 
+// An int32 CSR operator with FP64 values in device memory, owned elsewhere.
+struct CsrView {
+  int64_t n;
+  const int32_t *rp, *ci;
+  const double *val;
+};
+
 // Device copy of an int32 CSR operator with FP64 values.
 struct CsrDev {
   int64_t n = 0, nnz = 0;
@@ -23,6 +30,7 @@ struct CsrDev {
   ~CsrDev();
   CsrDev(const CsrDev &) = delete;
   CsrDev &operator=(const CsrDev &) = delete;
+  CsrView view() const { return CsrView{n, rp, ci, val}; }
 };
 
 // Assembles the full-array operator on the host and uploads it.
@@ -31,18 +39,25 @@ std::unique_ptr<CsrDev> build_operator(int dim, int n, double L, const double *c
 
 // Lanczos workspace of the plain path (KrylovInfo, device/include/lanczos.hpp:23-31,
 // plus the matrix-function buffers of matfunc_real.hpp:135-159) for S = double or c128.
+// The buffers are carved from one block of workspace_bytes(n, m): allocated by the
+// object, or (mem != nullptr) device memory the caller provides and keeps.
 template <class S> struct PlainKrylov {
   int64_t n;
   int m, G;
   hipStream_t stream;
+  char *block = nullptr;
+  bool owns = false;
   S *V = nullptr, *w = nullptr, *T = nullptr, *h = nullptr, *y = nullptr, *scal = nullptr;
   double *part = nullptr, *dscal = nullptr;
   int *status = nullptr;
-  PlainKrylov(int64_t n, int m, hipStream_t s);
+  PlainKrylov(int64_t n, int m, hipStream_t s, char *mem = nullptr);
   ~PlainKrylov();
-  void lanczos(const CsrDev &A, const S *in);
+  PlainKrylov(const PlainKrylov &) = delete;
+  PlainKrylov &operator=(const PlainKrylov &) = delete;
+  static size_t workspace_bytes(int64_t n, int m);
+  void lanczos(const CsrView &A, const S *in);
   // out = beta0 V f(T) e1 for f(kind; t_re + i t_im)
-  void apply(const CsrDev &A, S *out, const S *in, double t_re, double t_im, int kind);
+  void apply(const CsrView &A, S *out, const S *in, double t_re, double t_im, int kind);
   void check_status();
 };
 

@@ -121,6 +121,8 @@ _SIGS = {
     "nls_matfunc_destroy": [_P],
     "nls_matfunc_apply_real": [_P, _P, _P, C.c_double, C.c_int],
     "nls_matfunc_apply_complex": [_P, _P, _P, C.c_double, C.c_double, C.c_int],
+    "nls_matfunc_apply_real_noalloc": [C.c_int64, _P, _P, _P, _P, _P, C.c_double, C.c_int,
+                                       C.c_int, _P],
     "nls_matfunc_last_krylov": [_P, _D, _D],
     "nls_small_matfunc": [C.c_int, _D, _D, C.c_int, C.c_double, C.c_double, C.c_double, _D, _D],
     "nls_kge_create": [C.POINTER(KGEParams), _D, _D, _D, _D, C.POINTER(C.c_void_p)],
