@@ -149,6 +149,9 @@ typedef struct {
   int method;        /* NLS_KGE_* */
   double mu2;        /* mass term: operator -L + mu2 I (0 = reference equation) */
   int q6;            /* 3D: keep the reference's cross-plane couplings */
+  int variant;       /* Lanczos: 0 = the reference's (lanczos.hpp:208-267: recomputed lower
+                        subdiagonal, one Gram-Schmidt pass, T[m-1][m-1] = 0); 1 = textbook
+                        (symmetric T, no Gram-Schmidt, alpha_{m-1} computed) */
 } nls_kge_params;
 typedef struct nls_kge nls_kge;
 /* Assembles the operator from c (negated for Gautschi/sEWI, as the drivers do),
@@ -186,6 +189,8 @@ typedef struct {
   int krylov_dim; /* reference: 20 (2D Strang), 10, 25, 15 (nlse_cubic_*_driver_*.cpp) */
   int method;     /* NLS_NLSE_* */
   int q6;
+  int variant;    /* Lanczos variant as nls_kge_params; 1 also forms the intended SEWI
+                     update, where the reference swaps its two terms (nlse_dev.hpp:219-225) */
 } nls_nlse_params;
 typedef struct nls_nlse nls_nlse;
 /* Replaces NLSESolverDevice::NLSESolverDevice (device/include/nlse_dev.hpp:78-163) and
@@ -316,10 +321,13 @@ NLS_API int nls_batch_set_initial(nls_batch *b, int t, const double *m, const do
  * (s, traj, tag, 0) under the 64-bit seed. */
 enum { NLS_IC_GAUSSIANS_2D = 0, NLS_IC_KINK_RING_2D = 1, NLS_IC_PACKETS_2D = 2,
        NLS_IC_BUBBLES_3D = 3 };
-/* The parameters drawn for one trajectory, in draw order (host only, no device use):
- * gaussians [A, w, cx, cy] x 3; kink ring [R, w]; packets [K, (A, phi, vx, vy, cx, cy) x K];
- * bubbles [K, (A, w, cx, cy, cz) x K].  *count receives their number (at most 32). */
-NLS_API int nls_ic_draw(int kind, uint64_t seed, int traj, double *params, int cap, int *count);
+/* The parameters drawn for one trajectory on [-L, L]^dim, in draw order (host only, no
+ * device use): gaussians [A, w, cx, cy] x 3; kink ring [R, w]; packets
+ * [K, (A, phi, vx, vy, cx, cy) x K]; bubbles [K, (A, w, cx, cy, cz) x K].  Centres are
+ * U(-0.6 L, 0.6 L) (2D) or U(-0.5 L, 0.5 L) (3D) and ring radii U(0.2 L, 0.6 L).  *count
+ * receives their number (at most 32). */
+NLS_API int nls_ic_draw(int kind, uint64_t seed, int traj, double L, double *params, int cap,
+                        int *count);
 /* Initial data of trajectory t drawn and evaluated on the GPU, in place of
  * nls_batch_set_initial with host arrays: u0 (psi0 for NLS_IC_PACKETS_2D and an NLSE
  * batch) of sampler `kind` with Philox trajectory counter traj_index, v0 = 0, m and c

@@ -1045,6 +1045,17 @@ inline size_t rmarch2d_lds_bytes(int nmax) {
   return sizeof(float) * (3 * (size_t)(MSUB + 4 * nmax) + 2 * (size_t)(MSUB + 2 * nmax));
 }
 
+// Element idx of a per-trajectory array through a 32-bit byte offset from its (uniform)
+// base, so the access uses the scalar-base addressing form instead of a 64-bit vector
+// address per lane.  Needs idx * sizeof(T) < 2^32 (2D trajectories: fewer than 2^29 cells).
+template <class T> __device__ inline T &at32(T *base, uint32_t idx) {
+  return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + idx * (uint32_t)sizeof(T));
+}
+template <class T> __device__ inline const T &at32(const T *base, uint32_t idx) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) +
+                                      idx * (uint32_t)sizeof(T));
+}
+
 // ---------------------------------------------------------------- 2D row march, stored w'
 // Pass j reads w'_{j-1} and v_{j-1} (one grid row of halo), forms
 // v_j = (w'_{j-1} - alpha_{j-1} v_{j-1}) / b_j as it deposits each cell into the LDS
@@ -1101,18 +1112,19 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax,
   // deposit() uses them
   auto fetch = [&](int i, float &c, float &x0, float &x1, float &p0, float &p1, double &uu,
                    float &mm) {
-    c = UNI ? T.cval : cT[i];
+    const uint32_t u = (uint32_t)i;
+    c = UNI ? T.cval : at32(cT, u);
     if (STAGE <= 1) {
-      uu = uT[i];
-      mm = UNI ? T.mval : mT[i];
+      uu = at32(uT, u);
+      mm = UNI ? T.mval : at32(mT, u);
     }
     if (STAGE >= 1) {
-      x0 = X0[i];
-      x1 = X1[i];
+      x0 = at32(X0, u);
+      x1 = at32(X1, u);
     }
     if (STAGE >= 2) {
-      p0 = P0[i];
-      p1 = P1[i];
+      p0 = at32(P0, u);
+      p1 = at32(P1, u);
     }
   };
   auto deposit = [&](int e, float c, float x0, float x1, float p0, float p1, double uu,
@@ -1200,31 +1212,60 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax,
       }
     }
   }
+  // Grid column of this thread's cell at local coordinate l = cb + tid + kk * FBS, from
+  // the column of l = tid and the offset (cb + kk * FBS) mod n, which is the same for the
+  // whole workgroup (no per-cell division).
+  const int col_t = cell(threadIdx.x) - row_of(cell(threadIdx.x), n, rn) * n;
+  auto column = [&](int off_mod_n) {
+    int c = col_t + dir * off_mod_n;
+    c = c >= n ? c - n : c;
+    return c < 0 ? c + n : c;
+  };
   // stencil, stores and partials of the cells [cb, ce); FULL: ce = cb + SUB
   auto compute = [&](auto full_t, int cb, int ce) {
     constexpr bool FULL = decltype(full_t)::value;
+    const int cbm = cb % n;
 #pragma unroll
     for (int kk = 0; kk < MSPT; ++kk) {
       const int l = cb + threadIdx.x + kk * FBS;
       if (FULL || l < ce) {
         const int i = cell(l), e = l - cb + n;
-        const Sten5 w = UNI ? sten5_uniform(T.cval, i, n, N, rn)
-                            : sten5(sc, e, oU, oL, oR, oD, i, n, N, rn);
+        int om = cbm + (kk * FBS) % n;
+        om = om >= n ? om - n : om;
+        const int col = column(om);
+        Sten5 w;
+        w.hu = i >= n;
+        w.hl = col > 0;
+        w.hr = col < n - 1;
+        w.hd = i < N - n;
+        if (UNI) {
+          const float wc = 0.5f * (T.cval + T.cval); // (c + c) / 2, as the field path
+          w.wu = w.hu ? wc : 0.0f;
+          w.wl = w.hl ? wc : 0.0f;
+          w.wr = w.hr ? wc : 0.0f;
+          w.wd = w.hd ? wc : 0.0f;
+        } else {
+          const float ci = sc[e];
+          w.wu = w.hu ? 0.5f * (ci + sc[e + oU]) : 0.0f;
+          w.wl = w.hl ? 0.5f * (ci + sc[e + oL]) : 0.0f;
+          w.wr = w.hr ? 0.5f * (ci + sc[e + oR]) : 0.0f;
+          w.wd = w.hd ? 0.5f * (ci + sc[e + oD]) : 0.0f;
+        }
 #pragma unroll
         for (int s = 0; s < NSEQ; ++s) {
           const float *sv = s == 0 ? sv0 : sv1;
           const float vi = sv[e];
           const float Av = apply5z(w, sv, e, oU, oL, oR, oD);
           if (STAGE == 0) {
-            __builtin_nontemporal_store(Av, &(s == 0 ? Wout0 : Wout1)[i]);
+            __builtin_nontemporal_store(Av, &at32(s == 0 ? Wout0 : Wout1, (uint32_t)i));
             acc[s * PN + 0] += (double)vi * vi;
             acc[s * PN + 1] += (double)vi * Av;
             acc[s * PN + 2] += (double)Av * Av;
           } else {
             const float wp = Av - cC[s] * (s == 0 ? sp0 : sp1)[e];
             if (!LAST) {
-              __builtin_nontemporal_store(vi, &(s == 0 ? Vout0 : Vout1)[i]);
-              __builtin_nontemporal_store(wp, &(s == 0 ? Wout0 : Wout1)[i]);
+              __builtin_nontemporal_store(vi, &at32(s == 0 ? Vout0 : Vout1, (uint32_t)i));
+              __builtin_nontemporal_store(wp, &at32(s == 0 ? Wout0 : Wout1, (uint32_t)i));
             }
             acc[s * PN + 0] += (double)wp * vi;
             acc[s * PN + 1] += (double)wp * wp;
@@ -4011,11 +4052,11 @@ int nls_batch_set_initial(nls_batch *b, int t, const double *m, const double *c,
   });
 }
 
-int nls_ic_draw(int kind, uint64_t seed, int traj, double *params, int cap, int *count) {
+int nls_ic_draw(int kind, uint64_t seed, int traj, double L, double *params, int cap, int *count) {
   return guard([&] {
-    NLS_REQUIRE(params && count && traj >= 0, "nls_ic_draw: bad arguments");
+    NLS_REQUIRE(params && count && traj >= 0 && L > 0, "nls_ic_draw: bad arguments");
     double p[ic::kMaxParams];
-    const int c = ic::draw(kind, seed, (uint32_t)traj, p);
+    const int c = ic::draw(kind, seed, (uint32_t)traj, L, p);
     NLS_REQUIRE(c >= 0, "nls_ic_draw: unknown sampler kind");
     NLS_REQUIRE(c <= cap, "nls_ic_draw: parameter buffer too small");
     std::copy(p, p + c, params);
@@ -4038,7 +4079,7 @@ int nls_batch_set_initial_sampled(nls_batch *b, int t, int kind, uint64_t seed, 
     P.kind = kind;
     P.n = T.n;
     P.L = b->desc[t].L;
-    NLS_REQUIRE(ic::draw(kind, seed, (uint32_t)traj_index, P.p) >= 0, "sampler draw failed");
+    NLS_REQUIRE(ic::draw(kind, seed, (uint32_t)traj_index, P.L, P.p) >= 0, "sampler draw failed");
     const bool keep = b->p.max_snapshots > 0;
     const int64_t off = T.off;
     float *psi = b->nlse() ? (float *)b->d_state[0] + 2 * off : nullptr;

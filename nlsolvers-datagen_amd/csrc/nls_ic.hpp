@@ -7,6 +7,9 @@
 //                                                                      (nlse_sampler.py:43-100)
 //   bubbles 3D    u0 = sum_k A_k exp(-|x - x_k|^2 / w_k^2)             (the Gaussian of util.hpp:94-125)
 // on the grid linspace(-L, L, n) with "ij" indexing (real_sampler.py:17-25); v0 = 0.
+// Centres and ring radii scale with the half-width L (centres U(-0.6 L, 0.6 L) in 2D and
+// U(-0.5 L, 0.5 L) in 3D, R ~ U(0.2 L, 0.6 L)), so L = 10 (2D) and L = 8 (3D) give the
+// configurations' stated laws.
 //
 // Parameters are drawn on the host by ParamStream: draw s of trajectory t is the first
 // word (or the first words) of Philox4x32-10 at counter (s, t, tag, 0) under the 64-bit
@@ -48,25 +51,26 @@ struct ParamStream {
 
 inline int dim_of(int kind) { return kind == BUBBLES_3D ? 3 : 2; }
 
-// Draws of one trajectory in draw order; returns the count (<= kMaxParams), -1 for an
-// unknown kind.  Layouts: gaussians [A, w, cx, cy] x 3; kink [R, w]; packets
-// [K, (A, phi, vx, vy, cx, cy) x K]; bubbles [K, (A, w, cx, cy, cz) x K].
-inline int draw(int kind, uint64_t seed, uint32_t traj, double *p) {
+// Draws of one trajectory (domain [-L, L]^dim) in draw order; returns the count
+// (<= kMaxParams), -1 for an unknown kind.  Layouts: gaussians [A, w, cx, cy] x 3; kink
+// [R, w]; packets [K, (A, phi, vx, vy, cx, cy) x K]; bubbles [K, (A, w, cx, cy, cz) x K].
+inline int draw(int kind, uint64_t seed, uint32_t traj, double L, double *p) {
   int c = 0;
+  const double box2 = 6.0 * L / 10.0; // 2D centres U(-box2, box2)
   switch (kind) {
   case GAUSSIANS_2D: {
     ParamStream ps{seed, traj, 0x6A55u};
     for (int k = 0; k < 3; ++k) {
       p[c++] = ps.uniform(0.5, 2.0);
       p[c++] = ps.uniform(0.5, 2.0);
-      p[c++] = ps.uniform(-6.0, 6.0);
-      p[c++] = ps.uniform(-6.0, 6.0);
+      p[c++] = ps.uniform(-box2, box2);
+      p[c++] = ps.uniform(-box2, box2);
     }
     return c;
   }
   case KINK_RING_2D: {
     ParamStream ps{seed, traj, 0x4B12u};
-    p[c++] = ps.uniform(2.0, 6.0);
+    p[c++] = ps.uniform(2.0 * L / 10.0, 6.0 * L / 10.0);
     p[c++] = ps.uniform(0.5, 1.5);
     return c;
   }
@@ -79,22 +83,23 @@ inline int draw(int kind, uint64_t seed, uint32_t traj, double *p) {
       p[c++] = ps.uniform(0.0, 2.0 * 3.141592653589793);
       p[c++] = ps.normal();
       p[c++] = ps.normal();
-      p[c++] = ps.uniform(-6.0, 6.0);
-      p[c++] = ps.uniform(-6.0, 6.0);
+      p[c++] = ps.uniform(-box2, box2);
+      p[c++] = ps.uniform(-box2, box2);
     }
     return c;
   }
   case BUBBLES_3D: {
     ParamStream ps{seed, traj, 0x3DB0u};
+    const double b = 4.0 * L / 8.0;
     const int K = ps.integer(2, 5);
     p[c++] = K;
     for (int k = 0; k < K; ++k) {
       p[c++] = ps.uniform(0.5, 2.0);
       p[c++] = ps.uniform(0.8, 2.0);
-      const u32x4 r = ps.next(); // uniform(-4, 4, size=3): one counter, three words
-      p[c++] = -4.0 + 8.0 * ParamStream::unit(r.x);
-      p[c++] = -4.0 + 8.0 * ParamStream::unit(r.y);
-      p[c++] = -4.0 + 8.0 * ParamStream::unit(r.z);
+      const u32x4 r = ps.next(); // uniform(-b, b, size=3): one counter, three words
+      p[c++] = -b + (b - (-b)) * ParamStream::unit(r.x);
+      p[c++] = -b + (b - (-b)) * ParamStream::unit(r.y);
+      p[c++] = -b + (b - (-b)) * ParamStream::unit(r.z);
     }
     return c;
   }

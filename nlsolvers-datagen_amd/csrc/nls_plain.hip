@@ -155,16 +155,19 @@ __global__ __launch_bounds__(BS) void k_dot_s1(int64_t n, const S *__restrict__ 
   if (threadIdx.x == 0) a.store(part + (size_t)blockIdx.x * Acc<S>::W);
 }
 
-// Reduces G partials in fixed order; writes `value` to t1 (if non-null), -value to neg.
+// Reduces G partials in fixed order; writes `value` to t1 (if non-null), -value to neg;
+// real_only keeps the real part (the textbook complex Lanczos' alpha).
+__device__ inline double real_part(double v) { return v; }
+__device__ inline c128 real_part(c128 v) { return c128{v.re, 0.0}; }
 template <class S>
 __global__ __launch_bounds__(BS) void k_dot_s2(int G, const double *__restrict__ part, S *t1,
-                                               S *neg) {
+                                               S *neg, int real_only = 0) {
   __shared__ double sh[BS / kWave];
   Acc<S> a;
   for (int g = threadIdx.x; g < G; g += BS) a.load_add(part + (size_t)g * Acc<S>::W);
   a.sum(sh);
   if (threadIdx.x == 0) {
-    const S v = a.get();
+    const S v = real_only ? real_part(a.get()) : a.get();
     if (t1) *t1 = v;
     if (neg) *neg = -v;
   }
@@ -187,6 +190,11 @@ __global__ __launch_bounds__(BS) void k_norm_s2(int G, const double *__restrict_
     if (inv) *inv = v > 0.0 ? 1.0 / v : inv_if_zero;
     if (vout) *vout = v;
   }
+}
+
+// dst = -src (one scalar on the device)
+template <class S> __global__ void k_negate(const S *__restrict__ src, S *__restrict__ dst) {
+  if (threadIdx.x == 0) *dst = -*src;
 }
 
 template <class S>
@@ -598,21 +606,36 @@ template <class S> void PlainKrylov<S>::lanczos(const CsrView &A, const S *in) {
     launch_spmv(n, A.rp, A.ci, A.val, vj, w, s);
     if (j > 0) {
       S *vp = V + (size_t)(j - 1) * n;
-      hipLaunchKernelGGL((k_dot_s1<S>), g, b, 0, s, n, w, vp, part);
-      hipLaunchKernelGGL((k_dot_s2<S>), dim3(1), b, 0, s, G, part, T + (size_t)m * (j - 1) + j,
-                         negh);
+      if (variant == 0) { // Q2: the lower subdiagonal recomputed as <A v_j, v_{j-1}>
+        hipLaunchKernelGGL((k_dot_s1<S>), g, b, 0, s, n, w, vp, part);
+        hipLaunchKernelGGL((k_dot_s2<S>), dim3(1), b, 0, s, G, part,
+                           T + (size_t)m * (j - 1) + j, negh, 0);
+      } else { // textbook: w -= b_j v_{j-1} with b_j from the previous norm
+        hipLaunchKernelGGL((k_negate<S>), dim3(1), dim3(kWave), 0, s,
+                           T + (size_t)m * (j - 1) + j, negh);
+      }
       hipLaunchKernelGGL((k_axpy_dev<S>), g, b, 0, s, n, negh, vp, w);
     }
     hipLaunchKernelGGL((k_dot_s1<S>), g, b, 0, s, n, w, vj, part);
-    hipLaunchKernelGGL((k_dot_s2<S>), dim3(1), b, 0, s, G, part, T + (size_t)m * j + j, nega);
+    hipLaunchKernelGGL((k_dot_s2<S>), dim3(1), b, 0, s, G, part, T + (size_t)m * j + j, nega,
+                       variant);
     hipLaunchKernelGGL((k_axpy_dev<S>), g, b, 0, s, n, nega, vj, w);
-    hipLaunchKernelGGL((k_gemvT_s1<S>), g, b, 0, s, n, j + 1, V, w, part);
-    hipLaunchKernelGGL((k_gemvT_s2<S>), dim3(1), b, 0, s, G, j + 1, part, h);
-    hipLaunchKernelGGL((k_gemvN<S>), g, b, 0, s, n, j + 1, V, h, w);
+    if (variant == 0) { // Q3: one classical Gram-Schmidt pass over v_0 .. v_j
+      hipLaunchKernelGGL((k_gemvT_s1<S>), g, b, 0, s, n, j + 1, V, w, part);
+      hipLaunchKernelGGL((k_gemvT_s2<S>), dim3(1), b, 0, s, G, j + 1, part, h);
+      hipLaunchKernelGGL((k_gemvN<S>), g, b, 0, s, n, j + 1, V, h, w);
+    }
     hipLaunchKernelGGL((k_dot_s1<S>), g, b, 0, s, n, w, w, part);
     hipLaunchKernelGGL((k_norm_s2<S>), dim3(1), b, 0, s, G, part, T + (size_t)m * j + j + 1,
                        T + (size_t)m * (j + 1) + j, invb, (double *)nullptr, 1.0);
     hipLaunchKernelGGL((k_scale_copy<S>), g, b, 0, s, n, invb, w, V + (size_t)(j + 1) * n);
+  }
+  if (variant == 1) { // alpha_{m-1} = <A v_{m-1}, v_{m-1}> (real part for complex vectors)
+    S *vl = V + (size_t)(m - 1) * n;
+    launch_spmv(n, A.rp, A.ci, A.val, vl, w, s);
+    hipLaunchKernelGGL((k_dot_s1<S>), g, b, 0, s, n, w, vl, part);
+    hipLaunchKernelGGL((k_dot_s2<S>), dim3(1), b, 0, s, G, part,
+                       T + (size_t)m * (m - 1) + m - 1, (S *)nullptr, 1);
   }
   check_launch("lanczos");
 }
@@ -724,7 +747,9 @@ struct nls_kge {
     NLS_HIP(hipMemcpy(v->p, v0, B, hipMemcpyHostToDevice));
     NLS_HIP(hipMemcpy(up->p, upast.data(), B, hipMemcpyHostToDevice));
     NLS_HIP(hipMemcpy(m->p, m_h, B, hipMemcpyHostToDevice));
+    NLS_REQUIRE(p.variant == 0 || p.variant == 1, "variant must be 0 or 1");
     kry = std::make_unique<PlainKrylov<double>>(N, p.krylov_dim, s);
+    kry->variant = p.variant;
     store_snapshot(0);
   }
   ~nls_kge() {
@@ -794,7 +819,9 @@ struct nls_nlse {
     snap = std::make_unique<DevBuf>(B * p.num_snapshots);
     NLS_HIP(hipMemcpy(u->p, u0, B, hipMemcpyHostToDevice));
     NLS_HIP(hipMemcpy(m->p, m_h, sizeof(double) * N, hipMemcpyHostToDevice));
+    NLS_REQUIRE(p.variant == 0 || p.variant == 1, "variant must be 0 or 1");
     kry = std::make_unique<PlainKrylov<c128>>(N, p.krylov_dim, s);
+    kry->variant = p.variant;
     store_snapshot_online(); // nlse_cubic_driver_2d.cpp:106
   }
   ~nls_nlse() {
@@ -829,8 +856,15 @@ struct nls_nlse {
     kry->apply(A->view(), b3->as<c128>(), b2->as<c128>(), dt, 0.0, F_SINC);
     kry->apply(A->view(), b2->as<c128>(), b3->as<c128>(), 0.0, dt, F_EXP);
     kry->apply(A->view(), b3->as<c128>(), uprev->as<c128>(), 0.0, 2.0 * dt, F_EXP);
-    hipLaunchKernelGGL(k_sewi_combine, dim3(g), dim3(BS), 0, s, N, u->as<c128>(), b2->as<c128>(),
-                       b3->as<c128>(), dt);
+    // Q20: the reference passes (e^{tau L} sinc B, e^{2 tau L} psi-) to apply_sewi in swapped
+    // order (nlse_dev.hpp:54-64, 219-225); variant 1 forms the intended
+    // psi+ = e^{2 tau L} psi- - 2 tau e^{tau L} sinc(tau L) B(psi)
+    if (p.variant == 0)
+      hipLaunchKernelGGL(k_sewi_combine, dim3(g), dim3(BS), 0, s, N, u->as<c128>(),
+                         b2->as<c128>(), b3->as<c128>(), dt);
+    else
+      hipLaunchKernelGGL(k_sewi_combine, dim3(g), dim3(BS), 0, s, N, u->as<c128>(),
+                         b3->as<c128>(), b2->as<c128>(), dt);
     NLS_HIP(hipMemcpyAsync(uprev->p, buf->p, sizeof(c128) * N, hipMemcpyDeviceToDevice, s));
     check_launch("nlse sewi");
     if (step_number > 1 && step_number % freq == 0) store_snapshot_online();

@@ -50,6 +50,10 @@ template <class S> struct PlainKrylov {
   S *V = nullptr, *w = nullptr, *T = nullptr, *h = nullptr, *y = nullptr, *scal = nullptr;
   double *part = nullptr, *dscal = nullptr;
   int *status = nullptr;
+  // 0: the reference's Lanczos (lanczos.hpp:208-267: recomputed lower subdiagonal, one
+  // classical Gram-Schmidt pass, T[m-1][m-1] = 0); 1: textbook (symmetric T from b, no
+  // Gram-Schmidt, alpha_{m-1} computed; real alpha for complex vectors)
+  int variant = 0;
   PlainKrylov(int64_t n, int m, hipStream_t s, char *mem = nullptr);
   ~PlainKrylov();
   PlainKrylov(const PlainKrylov &) = delete;

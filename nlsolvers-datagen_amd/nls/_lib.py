@@ -20,13 +20,14 @@ class NLSError(RuntimeError):
 class KGEParams(C.Structure):
     _fields_ = [("dim", C.c_int), ("n", C.c_int), ("L", C.c_double), ("T", C.c_double),
                 ("nt", C.c_int), ("num_snapshots", C.c_int), ("krylov_dim", C.c_int),
-                ("method", C.c_int), ("mu2", C.c_double), ("q6", C.c_int)]
+                ("method", C.c_int), ("mu2", C.c_double), ("q6", C.c_int),
+                ("variant", C.c_int)]
 
 
 class NLSEParams(C.Structure):
     _fields_ = [("dim", C.c_int), ("n", C.c_int), ("L", C.c_double), ("T", C.c_double),
                 ("nt", C.c_int), ("num_snapshots", C.c_int), ("krylov_dim", C.c_int),
-                ("method", C.c_int), ("q6", C.c_int)]
+                ("method", C.c_int), ("q6", C.c_int), ("variant", C.c_int)]
 
 
 class SP4Params(C.Structure):
@@ -163,7 +164,7 @@ _SIGS = {
     "nls_batch_info": [_P, C.POINTER(C.c_int), _I64, C.POINTER(C.c_size_t)],
     "nls_batch_set_profiling": [_P, C.c_int],
     "nls_batch_monitor": [_P, C.c_int, C.POINTER(C.c_int), _D, _D, C.POINTER(C.c_int)],
-    "nls_ic_draw": [C.c_int, C.c_uint64, C.c_int, _D, C.c_int, C.POINTER(C.c_int)],
+    "nls_ic_draw": [C.c_int, C.c_uint64, C.c_int, C.c_double, _D, C.c_int, C.POINTER(C.c_int)],
     "nls_batch_set_initial_sampled": [_P, C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_double,
                                       C.c_double],
     "nls_batch_tile": [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],

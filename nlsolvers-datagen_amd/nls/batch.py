@@ -21,12 +21,14 @@ def lanczos_variant(v):
     return {"reference": 0, "ref": 0, "full": 1, "textbook": 1}.get(v, v)
 
 
-def ic_params(kind, seed, traj):
-    """The sampler parameters drawn for one trajectory, as the library draws them
-    (host only; see nls_ic_draw for the layout)."""
+def ic_params(kind, seed, traj, L=None):
+    """The sampler parameters drawn for one trajectory on [-L, L]^dim (default L: 10 in
+    2D, 8 in 3D, the configurations' domains), as the library draws them (host only; see
+    nls_ic_draw for the layout)."""
+    L = (8.0 if kind == "bubbles_3d" else 10.0) if L is None else float(L)
     buf = np.empty(32)
     cnt = C.c_int()
-    call("nls_ic_draw", IC_KINDS[kind], int(seed), int(traj), dptr(buf), 32, C.byref(cnt))
+    call("nls_ic_draw", IC_KINDS[kind], int(seed), int(traj), L, dptr(buf), 32, C.byref(cnt))
     return buf[:cnt.value].copy()
 
 

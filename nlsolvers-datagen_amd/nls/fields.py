@@ -41,10 +41,12 @@ def _quasi_waves(X, Y, ps, num_waves, amp_lo, amp_hi, base, scale_amp):
 
 
 # ---------------------------------------------------------------- c(x)
-def c_field(kind, n, L, seed, traj, base=1.0):
-    """One conductivity field of class `kind` (c_fields_2d.py:169-248 parameter ranges)."""
+def c_field(kind, n, L, seed, traj, base=1.0, rng=None):
+    """One conductivity field of class `kind` (c_fields_2d.py:169-248 parameter ranges).
+    rng (default: the trajectory's Philox stream) supplies the draws, in the order the
+    reference's generator makes its np.random calls for the classes drawn from loops."""
     X, Y = grid_2d(n, L)
-    ps = ParamStream(seed, traj, tag=0xC0C0)
+    ps = ParamStream(seed, traj, tag=0xC0C0) if rng is None else rng
     if kind == "constant":
         return np.ones((n, n)), {"type": "constant"}
     if kind == "periodic_structure":
@@ -85,8 +87,9 @@ def c_field(kind, n, L, seed, traj, base=1.0):
         p = np.full((n, n), float(base))
         pts = np.column_stack([X.ravel(), Y.ravel()])
         for _ in range(guides):
+            line = ps.uniform() < 0.5
             width = ps.uniform(w_lo, w_hi)
-            if ps.uniform() < 0.5:  # straight guide through a random point
+            if line:  # straight guide through a random point
                 x0, y0, th = ps.uniform(-L, L), ps.uniform(-L, L), ps.uniform(0, 2 * np.pi)
                 t = np.linspace(-1.5 * L, 1.5 * L, 100)
                 cx, cy = x0 + t * np.cos(th), y0 + t * np.sin(th)
@@ -118,11 +121,12 @@ def c_field(kind, n, L, seed, traj, base=1.0):
 
 
 # ---------------------------------------------------------------- m(x)
-def m_field(kind, n, L, seed, traj, c=None, m0=1.0):
+def m_field(kind, n, L, seed, traj, c=None, m0=1.0, rng=None):
     """One cubic-coefficient field of class `kind` (m_fields_2d.py:147-238 parameter
-    ranges); gradient / phase / topological derive from the conductivity c."""
+    ranges); gradient / phase / topological derive from the conductivity c.  rng as in
+    c_field."""
     X, Y = grid_2d(n, L)
-    ps = ParamStream(seed, traj, tag=0x3A55)
+    ps = ParamStream(seed, traj, tag=0x3A55) if rng is None else rng
     if kind in ("gradient", "phase", "topological") and c is None:
         kind = "constant"  # as the reference does without a c field
     if kind == "constant":

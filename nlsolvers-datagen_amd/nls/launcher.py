@@ -45,7 +45,7 @@ class LaunchConfig:
     Lz: float = 10.0
     m_classes: List[str] = field(default_factory=lambda: ["constant"])
     c_classes: List[str] = field(default_factory=lambda: ["constant"])
-    phenomenon: str = "gaussian"
+    phenomenon: str = ""          # "" = the equation's default (PHENOMENA)
     integrator: str = ""          # gautschi | sewi | sv (KGE), strang (NLSE), damped | reference (phi^4)
     krylov_dim: int = 0           # 0: autotuner
     mu2: float = 0.0
@@ -142,6 +142,34 @@ def downsample(u, cfg):
     raise ValueError(f"unknown downsampling strategy {cfg.dr_strategy!r}")
 
 
+# ---------------------------------------------------------------- initial data
+# The initial data each equation can be given: the reference launchers' phenomena that
+# are built (kink_field: real_launcher_2d.py:318 default, RealWaveSampler.kink_field;
+# multi_soliton: complex_launcher_2d.py:280 default, multi_soliton_state) and the
+# benchmark configurations' samplers.  The first name is the default; any other name is
+# refused (nothing is substituted under a name that was asked for).
+PHENOMENA = {
+    "real2d": ("kink_field", "kink_ring", "gaussians"),
+    "nlse2d": ("multi_soliton", "packets"),
+    "real3d": ("bubbles",),
+}
+_ALIASES = {"gaussian": "gaussians", "gaussian_sum": "gaussians", "kink_ring_2d": "kink_ring",
+            "multi_soliton_state": "multi_soliton", "gaussian_packets": "packets",
+            "bubbles_3d": "bubbles", "gaussian_bubbles": "bubbles"}
+
+
+def resolve_phenomenon(name, family):
+    """Canonical phenomenon name for an equation family ("real2d", "nlse2d", "real3d")."""
+    built = PHENOMENA[family]
+    if not name:
+        return built[0]
+    key = _ALIASES.get(name.lower(), name.lower())
+    if key not in built:
+        raise ValueError(f"phenomenon {name!r} is not built for this equation "
+                         f"(built: {', '.join(built)})")
+    return key
+
+
 # ---------------------------------------------------------------- the launcher
 class BatchLauncher:
     def __init__(self, cfg: LaunchConfig):
@@ -161,6 +189,14 @@ class BatchLauncher:
                              "(Klein-Gordon, Nonlinear Schrodinger, phi4)")
         if cfg.nx != cfg.ny or (cfg.dim == 3 and cfg.nz != cfg.nx):
             raise ValueError("grids must be square / cubic (as the reference solvers assume)")
+        if cfg.dim == 3 and self.kind == "nlse":
+            raise ValueError("3D NLSE datasets are not built here (the fused NLSE batch runs "
+                             "them; no 3D NLSE sampler is)")
+        family = "real3d" if cfg.dim == 3 else ("nlse2d" if self.kind == "nlse" else "real2d")
+        self.phenomenon = resolve_phenomenon(cfg.phenomenon, family)
+        if cfg.dim == 3 and (set(cfg.m_classes) | set(cfg.c_classes)) - {"constant"}:
+            raise ValueError("3D coefficient-field classes other than 'constant' are not "
+                             "built (the reference's field generators are 2D)")
         self.files = []
 
     def path(self, idx):
@@ -168,7 +204,7 @@ class BatchLauncher:
 
     def sample(self, idx):
         """(u0, v0, m, c, phenomenon params, m class) of run idx."""
-        cfg, n = self.cfg, self.cfg.nx
+        cfg, n, ph = self.cfg, self.cfg.nx, self.phenomenon
         if cfg.dim == 3:
             u0, v0 = S.bubbles_3d(n, cfg.Lx, cfg.seed, idx)
             one = np.ones_like(u0)
@@ -178,16 +214,20 @@ class BatchLauncher:
         c, _ = F.c_field(c_kind, n, cfg.Lx, cfg.seed, idx)
         m, _ = F.m_field(m_kind, n, cfg.Lx, cfg.seed, idx, c=c)
         if self.kind == "nlse":
-            psi, prm = S.nlse_packets_2d(n, cfg.Lx, cfg.seed, idx)
-            return psi, None, m, c, {"packets": prm}, m_kind
-        ph = cfg.phenomenon
-        if ph in ("kink_field", "kink_solution", "ring_soliton", "kink_ring"):
+            if ph == "multi_soliton":
+                psi, prm = S.multi_soliton_2d(n, cfg.Lx, cfg.seed, idx)
+            else:
+                psi, pk = S.nlse_packets_2d(n, cfg.Lx, cfg.seed, idx)
+                prm = {"packets": pk}
+            return psi, None, m, c, prm, m_kind
+        if ph == "kink_field":
+            u0, v0, prm = S.kink_field_2d(n, cfg.Lx, cfg.seed, idx)
+        elif ph == "kink_ring":
             u0, v0, prm = S.kink_ring_2d(n, cfg.Lx, cfg.seed, idx)
-        elif ph in ("gaussian", "gaussians"):
-            u0, v0, prm = S.gaussian_sum_2d(n, cfg.Lx, cfg.seed, idx)
         else:
-            raise ValueError(f"phenomenon {ph!r} not built (gaussian, kink_field/ring_soliton)")
-        return u0, v0, m, c, prm if isinstance(prm, dict) else {"gaussians": prm}, m_kind
+            u0, v0, g = S.gaussian_sum_2d(n, cfg.Lx, cfg.seed, idx)
+            prm = {"gaussians": g}
+        return u0, v0, m, c, prm, m_kind
 
     def _write(self, writer, idx, s, u, v, elapsed):
         cfg = self.cfg
@@ -195,7 +235,7 @@ class BatchLauncher:
         u = downsample(u, cfg)
         kw = dict(u0=u0, m=m, c=c, u=u, Lx=cfg.Lx, Ly=cfg.Ly, Lz=cfg.Lz, T=cfg.T, nt=cfg.nt,
                   num_snapshots=cfg.num_snapshots, run_id=self.run_id, run_index=idx,
-                  phenomenon=cfg.phenomenon if self.kind != "nlse" else "multi_soliton",
+                  phenomenon=self.phenomenon,
                   phenomenon_params=prm, m_type=m_kind, elapsed_time=elapsed,
                   problem_type={"kge": "klein_gordon", "nlse": "cubic", "sp4": "phi4"}[self.kind])
         if self.kind != "nlse":

@@ -124,13 +124,14 @@ class KGESolverDevice(_Handle):
     METHODS = {"gautschi": L.KGE_GAUTSCHI, "sv": L.KGE_SV, "sewi": L.KGE_SEWI}
 
     def __init__(self, n, L_, T, nt, num_snapshots, m, c, u0, v0, krylov_dim=10,
-                 method="gautschi", mu2=0.0, dim=2, q6=True, csr=None):
+                 method="gautschi", mu2=0.0, dim=2, q6=True, csr=None, variant="reference"):
+        from .batch import lanczos_variant
         super().__init__()
         self.N = n ** dim
         self.shape = (n,) * dim
         self.ns = num_snapshots
         self.p = L.KGEParams(dim, n, L_, T, nt, num_snapshots, krylov_dim,
-                             self.METHODS[method], mu2, int(q6))
+                             self.METHODS[method], mu2, int(q6), lanczos_variant(variant))
         mf, cf, u0, v0 = (as_f64(a).ravel() for a in (m, c, u0, v0))
         for a in (mf, cf, u0, v0):
             assert a.size == self.N, "field size does not match the grid"
@@ -176,13 +177,15 @@ class NLSESolverDevice(_Handle):
     _destroy = "nls_nlse_destroy"
 
     def __init__(self, n, L_, T, nt, num_snapshots, m, c, u0, krylov_dim=20, method="strang",
-                 dim=2, q6=True):
+                 dim=2, q6=True, variant="reference"):
+        from .batch import lanczos_variant
         super().__init__()
         self.N = n ** dim
         self.shape = (n,) * dim
         self.ns = num_snapshots
         meth = {"strang": L.NLSE_STRANG, "sewi": L.NLSE_SEWI}[method]
-        self.p = L.NLSEParams(dim, n, L_, T, nt, num_snapshots, krylov_dim, meth, int(q6))
+        self.p = L.NLSEParams(dim, n, L_, T, nt, num_snapshots, krylov_dim, meth, int(q6),
+                              lanczos_variant(variant))
         mf, cf = as_f64(m).ravel(), as_f64(c).ravel()
         u = as_c128_view(np.asarray(u0).ravel())
         call("nls_nlse_create", C.byref(self.p), dptr(mf), dptr(cf), dptr(u), C.byref(self.h))

@@ -91,3 +91,33 @@ def test_launcher_rejects_unbuilt_equations(tmp_path):
     from nls.launcher import BatchLauncher, LaunchConfig
     with pytest.raises(ValueError, match="integrator"):
         BatchLauncher(LaunchConfig(pde_type="sine_gordon", output_dir=str(tmp_path)))
+
+
+def test_launcher_refuses_unbuilt_phenomena_and_labels_what_it_generates(tmp_path):
+    """A phenomenon that is not built is refused (nothing else is generated under its
+    name); the default is the reference launchers' (kink_field for real fields,
+    multi_soliton for NLSE), and 3D refuses coefficient-field classes it does not build."""
+    from nls.launcher import BatchLauncher, LaunchConfig, resolve_phenomenon
+    assert resolve_phenomenon("", "real2d") == "kink_field"
+    assert resolve_phenomenon("", "nlse2d") == "multi_soliton"
+    assert resolve_phenomenon("gaussian", "real2d") == "gaussians"
+    for bad, fam in (("ring_soliton", "real2d"), ("kink_solution", "real2d"),
+                     ("kink_field", "nlse2d"), ("vortex_lattice", "nlse2d"),
+                     ("kink_field", "real3d")):
+        with pytest.raises(ValueError, match="not built"):
+            resolve_phenomenon(bad, fam)
+    with pytest.raises(ValueError, match="not built"):
+        BatchLauncher(LaunchConfig(pde_type="klein_gordon", phenomenon="breather_field",
+                                   output_dir=str(tmp_path)))
+    with pytest.raises(ValueError, match="3D coefficient"):
+        BatchLauncher(LaunchConfig(pde_type="klein_gordon", dim=3, nx=16, ny=16, nz=16,
+                                   m_classes=["defects"], output_dir=str(tmp_path)))
+    lk = BatchLauncher(LaunchConfig(pde_type="klein_gordon", nx=24, ny=24, Lx=3.0, Ly=3.0,
+                                    output_dir=str(tmp_path)))
+    u0, v0, m, c, prm, _ = lk.sample(0)
+    assert lk.phenomenon == "kink_field" and "winding_x" in prm and np.any(v0 != 0)
+    ln = BatchLauncher(LaunchConfig(pde_type="Nonlinear Schrodinger", nx=24, ny=24,
+                                    output_dir=str(tmp_path)))
+    psi, _, _, _, prm, _ = ln.sample(1)
+    assert ln.phenomenon == "multi_soliton" and "n_solitons" in prm
+    assert np.iscomplexobj(psi) and abs(np.abs(psi).max() - 1.0) < 1e-12

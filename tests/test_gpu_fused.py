@@ -603,6 +603,62 @@ def test_fused_kge_3d_long_horizon_vs_oracle(orc):
     assert max(errs) < TOL, errs
 
 
+@pytest.mark.timeout(400)
+def test_fused_kge_config4_size_vs_oracle(orc):
+    """Config 4 at its own size: one 220^3 trajectory on [-8, 8]^3 (7-point operator with
+    the reference's cross-plane couplings), sEWI, m = 12, dt = 0.004, mu2 = 1, Gaussian
+    bubbles of seed 3, on the plane march with 1024-cell ranges: 10 steps against the FP64
+    oracle (full-T Lanczos in both) at the 1e-5 bar."""
+    from nls.batch import FusedBatch
+    from nls.samplers import bubbles_3d
+    n, L, dt, steps, krylov = 220, 8.0, 0.004, 10, 12
+    u0, v0 = bubbles_3d(n, L, 3, 0)
+    one = np.ones((n, n, n))
+    b = FusedBatch([n], L, dt, method="sewi", dim=3, krylov_dim=krylov, mu2=1.0,
+                   snapshot_every=steps, max_snapshots=2, cells_per_thread=4, pass_kernel=4,
+                   variant="full")
+    b.set_initial(0, one, one, u0, v0)
+    b.run(steps)
+    assert b.tile()["pass_kind"] == 4
+    u, _ = b.snapshots(0)
+    b.close()
+    # the oracle's snapshot every step (freq = nt / ns = 1); snapshot `steps` is the state
+    ur, _ = orc.kge_run(n, L, dt * (steps + 1), steps + 1, steps + 1, one, one, u0, v0,
+                        krylov=krylov, mu2=1.0, dim=3, method=2, variant=1)
+    err = rel_l2(u[1], ur[steps])
+    print(f"kge 220^3 x {steps} (full T, m = {krylov}): rel-L2 {err:.2e}")
+    assert err < TOL, err
+
+
+@pytest.mark.timeout(300)
+def test_fused_kge_config4_batch_of_40_properties():
+    """The 40-trajectory 220^3 batch resident in HBM (about 55 GB with m = 12): after 5
+    sEWI steps every trajectory is finite and nonzero, and trajectory 7 is bitwise the same
+    as when it runs alone with the same tile."""
+    from nls.batch import FusedBatch
+    n, L, dt, steps, krylov = 220, 8.0, 0.004, 5, 12
+    kw = dict(method="sewi", dim=3, krylov_dim=krylov, mu2=1.0, cells_per_thread=4,
+              pass_kernel=4, variant="full")
+    b = FusedBatch([n] * 40, L, dt, **kw)
+    for t in range(40):
+        b.sample_initial(t, "bubbles_3d", 3, m=1.0, c=1.0)
+    b.run(steps)
+    assert b.info()["device_bytes"] > 40 * n ** 3 * 100
+    states = {}
+    for t in range(40):
+        u, _ = b.state(t)
+        assert np.all(np.isfinite(u)) and np.any(u != 0.0), t
+        if t == 7:
+            states[t] = u
+    b.close()
+    a = FusedBatch([n], L, dt, **kw)
+    a.sample_initial(0, "bubbles_3d", 3, traj_index=7, m=1.0, c=1.0)
+    a.run(steps)
+    u, _ = a.state(0)
+    a.close()
+    assert np.array_equal(u, states[7])
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 def test_fused_kge_energy_tracks_oracle(orc, dim):
     """Discrete energy E = 1/2 |v|^2 + 1/2 u.A u + 1/4 sum m u^4 (A = -L_c + mu2,

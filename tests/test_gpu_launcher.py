@@ -47,14 +47,16 @@ def test_launcher_kge_dataset_matches_oracle(orc, h5, tmp_path, integrator):
 
 def test_launcher_nlse_phi4_and_downsampling(h5, tmp_path):
     from nls.launcher import BatchLauncher, LaunchConfig
-    for pde, name in (("Nonlinear Schrodinger", "n"), ("phi4", "p")):
+    for pde, name, ph in (("Nonlinear Schrodinger", "n", "multi_soliton"),
+                          ("phi4", "p", "kink_field")):
         cfg = LaunchConfig(pde_type=pde, num_runs=2, num_snapshots=4, nt=40, T=0.2, nx=32, ny=32,
                            Lx=8.0, Ly=8.0, output_dir=str(tmp_path / name), run_id=name,
                            krylov_dim=12, dr_strategy="interpolation", dr_x=16, dr_y=16,
-                           phenomenon="kink_field")
+                           phenomenon=ph)
         res = BatchLauncher(cfg).run(log=lambda *a: None)
         assert res["files"] == 2
         p = str(tmp_path / name / "hdf5" / f"run_{name}_0001.h5")
+        assert h5.read_attr(p, "metadata", "phenomenon") == ph
         u = h5.read(p, "u")
         assert u.shape == (4, 16, 16) and np.all(np.isfinite(u))
         if name == "n":

@@ -140,6 +140,69 @@ def test_kge_gautschi_config1_vs_oracle(orc):
         assert rel_l2(v[k], vr[k]) < 1e-6, k
 
 
+def test_kge_config1_batch_of_four_mu2_vs_oracle(orc):
+    """Config 1 as stated: u_tt = Lap u - u - u^3 (mu2 = 1), 128^2 on [-10,10]^2, Gautschi
+    m = 10, dt = 0.01, nt = 500, 10 snapshots, a batch of 4 trajectories whose Gaussian
+    sums are drawn from seed 0 (trajectories 0..3): the plain FP64 drop-in with the
+    textbook Lanczos against the FP64 oracle's, every snapshot of every trajectory.  (With
+    mu2 = 1 the reference's Lanczos variant is ill-conditioned on these data: a 1e-15
+    relative change of u0 moves the FP64 oracle itself by up to 1e-4 within 50 steps,
+    test_oracle.py::test_config1_reference_variant_conditioning, so it is checked with
+    mu2 = 0 in test_kge_gautschi_config1_vs_oracle.)"""
+    from nls.samplers import gaussian_sum_2d
+    from nls.solvers import KGESolverDevice
+    n, L, T, nt, ns = 128, 10.0, 5.0, 500, 10
+    one = np.ones((n, n))
+    ics = [gaussian_sum_2d(n, L, 0, t)[:2] for t in range(4)]
+    ur, _ = orc.kge_run_batch(n, L, T, nt, ns, [one] * 4, [one] * 4, [ic[0] for ic in ics],
+                              [ic[1] for ic in ics], krylov=10, mu2=1.0, variant=1)
+    for t, (u0, v0) in enumerate(ics):
+        s = KGESolverDevice(n, L, T, nt, ns, one, one, u0, v0, krylov_dim=10, mu2=1.0,
+                            variant="full")
+        s.run()
+        u, _ = s.transfer_snapshots()
+        for k in range(ns):
+            assert rel_l2(u[k], ur[t][k]) < 1e-8, (t, k, rel_l2(u[k], ur[t][k]))
+
+
+def test_plain_full_variant_vs_oracle(orc):
+    """The plain path's textbook Lanczos (no Gram-Schmidt, symmetric T, alpha_{m-1}) against
+    the oracle's: the matrix function on a small Krylov space (m = 4, where the two
+    variants differ by ~1e-3) for real and complex vectors, and the NLSE SEWI step in its
+    intended form."""
+    from nls.solvers import NLSESolverDevice
+    n, L = 40, 10.0
+    rng = np.random.default_rng(5)
+    c = 0.8 + 0.4 * rng.random((n, n))
+    m = 0.8 + 0.4 * rng.random((n, n))
+    dx = 2 * L / (n - 1)
+    op = orc.laplacian_2d(n, dx, dx, c.ravel(), negate=True, shift=1.0)
+    u0, v0 = field(n, 1)
+    from nls.solvers import KGESolverDevice
+    for var in ("full", "reference"):
+        s = KGESolverDevice(n, L, 0.2, 20, 2, m, c, u0, v0, krylov_dim=4, mu2=1.0, variant=var)
+        s.run()
+        u, _ = s.transfer_snapshots()
+        vi = 1 if var == "full" else 0
+        ur, _ = orc.kge_run(n, L, 0.2, 20, 2, m, c, u0, v0, krylov=4, mu2=1.0, variant=vi)
+        ro, _ = orc.kge_run(n, L, 0.2, 20, 2, m, c, u0, v0, krylov=4, mu2=1.0, variant=1 - vi)
+        assert rel_l2(ro[1], ur[1]) > 1e-5  # the case tells the variants apart
+        assert rel_l2(u[1], ur[1]) < 1e-12, (var, rel_l2(u[1], ur[1]))
+    # complex vectors: Strang in both variants; SEWI in its intended form (the reference's
+    # swapped SEWI update, Q20, does not conserve mass and overflows on this case)
+    psi0 = u0 * np.exp(1j * 0.3 * np.arange(n)[:, None])
+    for method, var in (("strang", "full"), ("strang", "reference"), ("sewi", "full")):
+        vi = 1 if var == "full" else 0
+        s = NLSESolverDevice(n, L, 0.2, 40, 4, m, c, psi0, krylov_dim=6, method=method,
+                             variant=var)
+        s.run()
+        out = s.transfer_snapshots()
+        ref = orc.nlse_run(n, L, 0.2, 40, 4, m, c, psi0, krylov=6,
+                           method={"strang": 0, "sewi": 1}[method], variant=vi)
+        for k in range(4):
+            assert rel_l2(out[k], ref[k]) < 1e-10, (method, var, k)
+
+
 @pytest.mark.parametrize("method", ["sv", "sewi"])
 def test_kge_other_methods_vs_oracle(orc, method):
     from nls.solvers import KGESolverDevice

@@ -48,6 +48,44 @@ def test_server_mix_equals_alone_and_oracle(orc):
     assert st["peak_active"] >= 2
 
 
+@pytest.mark.timeout(300)
+def test_server_config5_sizes_vs_oracle(orc):
+    """The serving workload's own sizes and horizons: requests of n = 128, 256, 512 with
+    nt = 1000..1500 (gamma = 0.1, sigma in [0.05, 0.5], own Philox seeds) in one mix with
+    smaller requests; each result is bitwise the same as the request alone and agrees with
+    the FP64 oracle (same Philox noise) at every snapshot."""
+    from nls.samplers import gaussian_sum_2d
+    from nls.server import TrajectoryServer
+    dt, L = 0.005, 10.0
+    reqs = [(512, 1000, 31, 0.05), (256, 1500, 32, 0.5), (128, 1200, 33, 0.3),
+            (64, 1000, 34, 0.2)]
+    ics = {}
+    for n, nt, seed, sig in reqs:
+        u0, _, _ = gaussian_sum_2d(n, L, seed, 0)
+        ics[seed] = (u0, np.zeros((n, n)))
+    mix = TrajectoryServer(hbm_budget_bytes=256 << 20, max_active=8, L_=L, gamma=0.1)
+    ids = {seed: mix.submit(n, nt, dt, sig, seed, num_snapshots=5, u0=ics[seed][0],
+                            v0=ics[seed][1]) for n, nt, seed, sig in reqs}
+    mix.drain(300)
+    for n, nt, seed, sig in reqs:
+        got = mix.fetch(ids[seed])
+        u0, v0 = ics[seed]
+        alone = TrajectoryServer(hbm_budget_bytes=64 << 20, max_active=1, L_=L, gamma=0.1,
+                                 max_steps_per_group=13)
+        rid = alone.submit(n, nt, dt, sig, seed, num_snapshots=5, u0=u0, v0=v0)
+        alone.wait(rid, 300)
+        ref = alone.fetch(rid)
+        alone.close()
+        assert np.array_equal(got["u_final"], ref["u_final"]), n
+        assert np.array_equal(got["snapshots"], ref["snapshots"]), n
+        one = np.ones((n, n))
+        ou, _ = orc.sp4_run(n, L, dt * nt, nt, 5, one, one, u0, v0, mode=1, gamma=0.1,
+                            sigma=sig, seed=seed, stream=0)
+        for k in range(5):
+            assert rel_l2(got["snapshots"][k], ou[k]) < 1e-10, (n, k)
+    mix.close()
+
+
 def test_server_fields_uniform_and_varying_vs_oracle(orc):
     """Requests with uniform fields (m = c = 1 by default, or other constants) take the
     values from the request descriptor; a request with varying fields streams them.  In

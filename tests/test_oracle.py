@@ -480,3 +480,22 @@ def test_reference_semantics_conditioning_with_uniform_fields(orc):
     assert sens(8, 0) > 1e-5
     assert sens(8, 1) < 1e-7
     assert sens(16, 0) < 1e-7
+
+
+def test_config1_reference_variant_conditioning(orc):
+    """Why config 1 with mu2 = 1 is checked with the textbook Lanczos: on its own data
+    (Gaussian sums of seed 0, trajectory 1, 128^2, m = 10, dt = 0.01) the reference's
+    variant amplifies a 1e-15 relative change of u0 to more than 1e-6 within 50 steps
+    (measured 6.8e-6), while the textbook variant keeps it near rounding."""
+    from nls.samplers import gaussian_sum_2d
+    n, L = 128, 10.0
+    u0, v0, _ = gaussian_sum_2d(n, L, 0, 1)
+    up = u0 * (1 + 1e-15 * np.random.default_rng(1).standard_normal(u0.shape))
+    one = np.ones((n, n))
+    gap = {}
+    for var in (0, 1):
+        ur, _ = orc.kge_run_batch(n, L, 1.0, 100, 2, [one, one], [one, one], [u0, up],
+                                  [v0, v0], krylov=10, mu2=1.0, variant=var)
+        gap[var] = np.linalg.norm(ur[1][1] - ur[0][1]) / np.linalg.norm(ur[0][1])
+    assert gap[0] > 1e-6, gap
+    assert gap[1] < 1e-12, gap
