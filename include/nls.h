@@ -393,6 +393,8 @@ typedef struct {
   const double *u0, *v0, *m, *c; /* solver-grid arrays; v0 ignored for complex runs */
   int ns_out, ox, oy, oz;         /* stored trajectory shape (after any downsampling) */
   const double *u, *v;            /* ns_out * ox * oy * oz values (u complex128 if complex) */
+  const char *c_type;             /* conductivity class, metadata attribute c_type (NULL or
+                                     "" = none; the reference keeps it in the directory name) */
 } nls_h5_run;
 /* Writes the file now. */
 NLS_API int nls_h5_write_run(const nls_h5_run *r);
@@ -413,6 +415,26 @@ NLS_API int nls_h5_read_dataset(const char *path, const char *name, double *buf,
  * string into str (len bytes, NUL-terminated); *is_string tells which. */
 NLS_API int nls_h5_read_attr(const char *path, const char *obj, const char *name, double *num,
                              char *str, int len, int *is_string);
+
+/* ---------------------------------------------------------------- dataset packaging */
+/* Per-class NetCDF-4 file of many trajectories, as the reference's aggregation writes it
+ * (aggregate_single_core.py:167-206, netCDF4.Dataset(format="NETCDF4")): dimensions
+ * total_temporal = num_trajectories * num_snapshots * cells and total_spatial =
+ * num_trajectories * cells; float variables u, v (complex classes: re_u, im_u) over
+ * total_temporal with chunks of 65536, u0, v0 (re_u0, im_u0) over total_spatial with
+ * chunks of 65536, m and c over total_spatial; global attributes num_snapshots,
+ * num_trajectories, spatial_shape ("(nx, ny)"), Lx, Ly[, Lz], is_complex, dims.  Written
+ * through libhdf5 (+ libhdf5_hl for the dimension scales) with the NetCDF-4 conventions;
+ * trajectory t's arrays go to offset t * their per-trajectory size. */
+typedef struct nls_nc_class nls_nc_class;
+NLS_API int nls_nc_class_create(const char *path, int dims, const int *shape, int num_snapshots,
+                                int num_trajectories, int is_complex, double Lx, double Ly,
+                                double Lz, nls_nc_class **out);
+/* One trajectory (float32 host arrays; complex classes: u = Re u, v = Im u, u0 = Re u0,
+ * v0 = Im u0); a NULL array leaves that part at netCDF's fill value. */
+NLS_API int nls_nc_class_put(nls_nc_class *h, int traj, const float *u, const float *v,
+                             const float *u0, const float *v0, const float *m, const float *c);
+NLS_API int nls_nc_class_close(nls_nc_class *h);
 
 /* ---------------------------------------------------------------- serving loop */
 /* Continuous batching of stochastic phi^4 trajectory requests of different

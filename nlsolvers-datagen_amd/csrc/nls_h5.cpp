@@ -83,6 +83,12 @@ struct Api {
   herr_t (*tclose)(hid_t);
   herr_t (*free_memory)(void *);
   hid_t *native_double, *native_llong, *c_s1, *dataset_create;
+  // NetCDF-4 class files
+  hid_t *native_float, *native_int, *ieee_f32be;
+  herr_t (*pset_fill_value)(hid_t, hid_t, const void *);
+  void *hl = nullptr; // libhdf5_hl (dimension scales); optional
+  herr_t (*ds_set_scale)(hid_t, const char *) = nullptr;
+  herr_t (*ds_attach_scale)(hid_t, hid_t, unsigned) = nullptr;
 };
 
 std::mutex g_mu; // every HDF5 call runs under this lock
@@ -136,8 +142,25 @@ Api *load() {
             bind(a, a.free_memory, "H5free_memory") &&
             bind(a, a.native_double, "H5T_NATIVE_DOUBLE_g") &&
             bind(a, a.native_llong, "H5T_NATIVE_LLONG_g") && bind(a, a.c_s1, "H5T_C_S1_g") &&
-            bind(a, a.dataset_create, "H5P_CLS_DATASET_CREATE_ID_g");
+            bind(a, a.dataset_create, "H5P_CLS_DATASET_CREATE_ID_g") &&
+            bind(a, a.native_float, "H5T_NATIVE_FLOAT_g") &&
+            bind(a, a.native_int, "H5T_NATIVE_INT_g") &&
+            bind(a, a.ieee_f32be, "H5T_IEEE_F32BE_g") &&
+            bind(a, a.pset_fill_value, "H5Pset_fill_value");
   if (!ok) return nullptr;
+  // the high-level library (dimension scales, for NetCDF-4 files), next to libhdf5
+  std::vector<std::string> hl;
+  if (const char *e = std::getenv("NLS_HDF5_HL_LIB")) hl.push_back(e);
+  hl.insert(hl.end(), {"libhdf5_hl.so", "libhdf5_hl.so.100", "/opt/conda/lib/libhdf5_hl.so"});
+  for (auto &n : hl) {
+    a.hl = dlopen(n.c_str(), RTLD_NOW | RTLD_LOCAL);
+    if (a.hl) break;
+  }
+  if (a.hl) {
+    a.ds_set_scale = reinterpret_cast<herr_t (*)(hid_t, const char *)>(dlsym(a.hl, "H5DSset_scale"));
+    a.ds_attach_scale =
+        reinterpret_cast<herr_t (*)(hid_t, hid_t, unsigned)>(dlsym(a.hl, "H5DSattach_scale"));
+  }
   if (a.open() < 0) {
     a.error = "H5open failed";
     return nullptr;
@@ -256,6 +279,9 @@ void write_header(Api &a, hid_t f, const Run &r, bool with_elapsed = true) {
     if (with_elapsed) attr_f64(a, g.id, "elapsed_time", r.elapsed_time);
     attr_str(a, g.id, "phenomenon", r.phenomenon);
     for (auto &kv : r.params) attr_str(a, g.id, ("phenomenon_" + kv.first).c_str(), kv.second);
+    // the conductivity class (the reference keeps it only in the run directory's name,
+    // c_<class>, parallel_generate_problem_configs.py:91-92)
+    if (!r.c_type.empty()) attr_str(a, g.id, "c_type", r.c_type);
   }
   {
     Id g(chk(a.gcreate(f, "grid", P_DEFAULT, P_DEFAULT, P_DEFAULT), "group grid"), a.gclose);
@@ -349,6 +375,7 @@ Run from_abi(const nls_h5_run &r) {
   o.run_id = str(r.run_id);
   o.phenomenon = str(r.phenomenon);
   o.m_type = str(r.m_type);
+  o.c_type = str(r.c_type);
   o.timestamp = str(r.timestamp);
   o.run_index = r.run_index;
   o.elapsed_time = r.elapsed_time;
@@ -465,6 +492,167 @@ void Stream::close(double elapsed_time) {
 Stream::~Stream() {
   try {
     close(0.0);
+  } catch (...) {
+  }
+}
+
+// ---------------------------------------------------------------- NetCDF-4 class files
+namespace {
+// NetCDF-4 conventions on top of HDF5: a dimension without a coordinate variable is a
+// dataset of its length (big-endian float, never written) made a dimension scale whose
+// NAME is the netCDF marker below, with an int attribute _Netcdf4Dimid; variables attach
+// the scales of their dimensions; attributes are one-element arrays (numbers) or
+// fixed-length strings.  netCDF's default fill value for float variables is
+// 9.9692099683868690e+36 (NC_FILL_FLOAT).
+constexpr float kFillFloat = 9.9692099683868690e+36f;
+
+void nc_attr_i32(Api &a, hid_t obj, const char *name, int v) {
+  const hsize_t one = 1;
+  Id sp(chk(a.screate_simple(1, &one, nullptr), "H5Screate_simple"), a.sclose);
+  Id at(chk(a.acreate(obj, name, *a.native_int, sp.id, P_DEFAULT, P_DEFAULT),
+            std::string("attribute ") + name),
+        a.aclose);
+  if (a.awrite(at.id, *a.native_int, &v) < 0) throw Error(std::string("HDF5: writing ") + name);
+}
+
+void nc_attr_f32(Api &a, hid_t obj, const char *name, float v) {
+  const hsize_t one = 1;
+  Id sp(chk(a.screate_simple(1, &one, nullptr), "H5Screate_simple"), a.sclose);
+  Id at(chk(a.acreate(obj, name, *a.native_float, sp.id, P_DEFAULT, P_DEFAULT),
+            std::string("attribute ") + name),
+        a.aclose);
+  if (a.awrite(at.id, *a.native_float, &v) < 0) throw Error(std::string("HDF5: writing ") + name);
+}
+
+void nc_attr_text(Api &a, hid_t obj, const char *name, const std::string &v) {
+  Id t(chk(a.tcopy(*a.c_s1), "H5Tcopy"), a.tclose);
+  a.tset_size(t.id, v.empty() ? 1 : v.size());
+  Id sp(chk(a.screate(S_SCALAR), "H5Screate"), a.sclose);
+  Id at(chk(a.acreate(obj, name, t.id, sp.id, P_DEFAULT, P_DEFAULT), std::string("attribute ") + name),
+        a.aclose);
+  const std::string buf = v.empty() ? std::string(1, '\0') : v;
+  if (a.awrite(at.id, t.id, buf.data()) < 0) throw Error(std::string("HDF5: writing ") + name);
+}
+
+hid_t nc_dimension(Api &a, hid_t f, const char *name, hsize_t len, int dimid) {
+  Id sp(chk(a.screate_simple(1, &len, nullptr), "H5Screate_simple"), a.sclose);
+  const hid_t d = chk(a.dcreate(f, name, *a.ieee_f32be, sp.id, P_DEFAULT, P_DEFAULT, P_DEFAULT),
+                      std::string("dimension ") + name);
+  char marker[96];
+  std::snprintf(marker, sizeof marker, "This is a netCDF dimension but not a netCDF variable.%10llu",
+                len);
+  if (a.ds_set_scale(d, marker) < 0) throw Error(std::string("HDF5: dimension scale ") + name);
+  nc_attr_i32(a, d, "_Netcdf4Dimid", dimid);
+  return d;
+}
+
+hid_t nc_variable(Api &a, hid_t f, const char *name, hid_t dim, hsize_t len, bool chunked) {
+  Id sp(chk(a.screate_simple(1, &len, nullptr), "H5Screate_simple"), a.sclose);
+  Id pl(chk(a.pcreate(*a.dataset_create), "H5Pcreate"), a.pclose);
+  if (chunked) {
+    // chunksizes=(65536,) (aggregate_single_core.py:180-188), at most the length
+    const hsize_t ch = len < 65536 ? (len > 0 ? len : 1) : 65536;
+    a.pset_chunk(pl.id, 1, &ch);
+  }
+  a.pset_fill_value(pl.id, *a.native_float, &kFillFloat);
+  const hid_t d = chk(a.dcreate(f, name, *a.native_float, sp.id, P_DEFAULT, pl.id, P_DEFAULT),
+                      std::string("variable ") + name);
+  if (a.ds_attach_scale(d, dim, 0) < 0) throw Error(std::string("HDF5: attaching ") + name);
+  return d;
+}
+
+void nc_put(Api &a, hid_t d, hsize_t off, hsize_t count, const float *x) {
+  if (!x || count == 0) return;
+  Id fs(chk(a.dget_space(d), "H5Dget_space"), a.sclose);
+  a.sselect_hyperslab(fs.id, S_SELECT_SET, &off, nullptr, &count, nullptr);
+  Id ms(chk(a.screate_simple(1, &count, nullptr), "H5Screate_simple"), a.sclose);
+  if (a.dwrite(d, *a.native_float, ms.id, fs.id, P_DEFAULT, x) < 0)
+    throw Error("HDF5: writing a trajectory slab");
+}
+} // namespace
+
+NcClass::NcClass(const std::string &path, int dims, const std::vector<int> &shape, int ns,
+                 int ntraj, bool is_complex, double Lx, double Ly, double Lz)
+    : ns_(ns), ntraj_(ntraj), complex_(is_complex) {
+  NLS_REQUIRE(dims == 2 || dims == 3, "NetCDF class: dims must be 2 or 3");
+  NLS_REQUIRE((int)shape.size() == dims, "NetCDF class: shape must have dims entries");
+  NLS_REQUIRE(ns > 0 && ntraj > 0, "NetCDF class: need snapshots and trajectories");
+  spat_ = 1;
+  for (int x : shape) {
+    NLS_REQUIRE(x > 0, "NetCDF class: bad shape");
+    spat_ *= (hsize_t)x;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  Api &a = need();
+  NLS_REQUIRE(a.ds_set_scale && a.ds_attach_scale,
+              "NetCDF class files need libhdf5_hl (dimension scales); set NLS_HDF5_HL_LIB");
+  file_ = chk(a.fcreate(path.c_str(), ACC_TRUNC, P_DEFAULT, P_DEFAULT), "create " + path);
+  try {
+    const hsize_t temp = (hsize_t)ns * spat_;
+    dt_ = nc_dimension(a, file_, "total_temporal", (hsize_t)ntraj * temp, 0);
+    ds_ = nc_dimension(a, file_, "total_spatial", (hsize_t)ntraj * spat_, 1);
+    const char *names_t[2] = {complex_ ? "re_u" : "u", complex_ ? "im_u" : "v"};
+    const char *names_s[2] = {complex_ ? "re_u0" : "u0", complex_ ? "im_u0" : "v0"};
+    for (int k = 0; k < 2; ++k) {
+      vt_[k] = nc_variable(a, file_, names_t[k], dt_, (hsize_t)ntraj * temp, true);
+      vs_[k] = nc_variable(a, file_, names_s[k], ds_, (hsize_t)ntraj * spat_, true);
+    }
+    vm_ = nc_variable(a, file_, "m", ds_, (hsize_t)ntraj * spat_, false);
+    vc_ = nc_variable(a, file_, "c", ds_, (hsize_t)ntraj * spat_, false);
+    // global attributes (aggregate_single_core.py:193-201)
+    nc_attr_i32(a, file_, "num_snapshots", ns);
+    nc_attr_i32(a, file_, "num_trajectories", ntraj);
+    std::string sh = "(";
+    for (int k = 0; k < dims; ++k) sh += std::to_string(shape[k]) + (k + 1 < dims ? ", " : "");
+    sh += dims == 1 ? ",)" : ")";
+    nc_attr_text(a, file_, "spatial_shape", sh);
+    nc_attr_f32(a, file_, "Lx", (float)Lx);
+    nc_attr_f32(a, file_, "Ly", (float)Ly);
+    if (dims == 3) nc_attr_f32(a, file_, "Lz", (float)Lz);
+    nc_attr_i32(a, file_, "is_complex", complex_ ? 1 : 0);
+    nc_attr_i32(a, file_, "dims", dims);
+    a.fflush(file_, 0);
+  } catch (...) {
+    close_locked();
+    throw;
+  }
+}
+
+void NcClass::put(int traj, const float *u, const float *v, const float *u0, const float *v0,
+                  const float *m, const float *c) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  NLS_REQUIRE(file_ >= 0, "NetCDF class: closed");
+  NLS_REQUIRE(traj >= 0 && traj < ntraj_, "NetCDF class: trajectory index out of range");
+  Api &a = need();
+  const hsize_t temp = (hsize_t)ns_ * spat_;
+  nc_put(a, vt_[0], (hsize_t)traj * temp, temp, u);
+  nc_put(a, vt_[1], (hsize_t)traj * temp, temp, v);
+  nc_put(a, vs_[0], (hsize_t)traj * spat_, spat_, u0);
+  nc_put(a, vs_[1], (hsize_t)traj * spat_, spat_, v0);
+  nc_put(a, vm_, (hsize_t)traj * spat_, spat_, m);
+  nc_put(a, vc_, (hsize_t)traj * spat_, spat_, c);
+}
+
+void NcClass::close_locked() {
+  if (file_ < 0) return;
+  Api &a = need();
+  for (int64_t *d : {&vt_[0], &vt_[1], &vs_[0], &vs_[1], &vm_, &vc_, &dt_, &ds_})
+    if (*d >= 0) {
+      a.dclose(*d);
+      *d = -1;
+    }
+  a.fclose(file_);
+  file_ = -1;
+}
+
+void NcClass::close() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  close_locked();
+}
+
+NcClass::~NcClass() {
+  try {
+    close();
   } catch (...) {
   }
 }
@@ -608,6 +796,40 @@ int nls_h5_read_dataset(const char *path, const char *name, double *buf, int64_t
     Id mt(cplx ? complex_type(a) : a.tcopy(*a.native_double), a.tclose);
     if (a.dread(d.id, mt.id, S_ALL, S_ALL, P_DEFAULT, buf) < 0)
       throw nls::Error(std::string("HDF5: reading ") + name);
+  });
+}
+
+struct nls_nc_class {
+  std::unique_ptr<nls::h5::NcClass> w;
+};
+
+int nls_nc_class_create(const char *path, int dims, const int *shape, int num_snapshots,
+                        int num_trajectories, int is_complex, double Lx, double Ly, double Lz,
+                        nls_nc_class **out) {
+  return nls::guard([&] {
+    NLS_REQUIRE(path && shape && out, "nls_nc_class_create: null argument");
+    NLS_REQUIRE(dims == 2 || dims == 3, "nls_nc_class_create: dims must be 2 or 3");
+    auto h = std::make_unique<nls_nc_class>();
+    h->w = std::make_unique<nls::h5::NcClass>(path, dims, std::vector<int>(shape, shape + dims),
+                                              num_snapshots, num_trajectories, is_complex != 0,
+                                              Lx, Ly, Lz);
+    *out = h.release();
+  });
+}
+
+int nls_nc_class_put(nls_nc_class *h, int traj, const float *u, const float *v, const float *u0,
+                     const float *v0, const float *m, const float *c) {
+  return nls::guard([&] {
+    NLS_REQUIRE(h && h->w, "nls_nc_class_put: null handle");
+    h->w->put(traj, u, v, u0, v0, m, c);
+  });
+}
+
+int nls_nc_class_close(nls_nc_class *h) {
+  return nls::guard([&] {
+    if (!h) return;
+    std::unique_ptr<nls_nc_class> own(h);
+    if (own->w) own->w->close();
   });
 }
 

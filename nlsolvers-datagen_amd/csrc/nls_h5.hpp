@@ -26,7 +26,7 @@ struct Run {
   int dim = 2, nx = 0, ny = 0, nz = 1;
   double Lx = 0, Ly = 0, Lz = 0, T = 0;
   int nt = 0, num_snapshots = 0;
-  std::string problem_type, run_id, phenomenon, m_type, timestamp;
+  std::string problem_type, run_id, phenomenon, m_type, c_type, timestamp;
   int run_index = 0;
   double elapsed_time = 0;
   std::vector<std::pair<std::string, std::string>> params; // metadata phenomenon_<key>
@@ -57,6 +57,31 @@ class Stream {
   int count_ = 0;
   bool complex_ = false, has_v_ = false;
   std::vector<uint64_t> dims_;
+};
+
+// A per-class NetCDF-4 file (aggregate_single_core.py:167-206): dimensions total_temporal
+// (trajectories x snapshots x cells) and total_spatial (trajectories x cells), float
+// variables u, v (re_u, im_u for complex classes) over total_temporal and u0, v0 (re_u0,
+// im_u0), m, c over total_spatial, trajectory t at offset t x its per-trajectory size,
+// and the global attributes num_snapshots, num_trajectories, spatial_shape, Lx, Ly[, Lz],
+// is_complex, dims.  Written through libhdf5 with the NetCDF-4 conventions (dimension
+// scales from libhdf5_hl).
+class NcClass {
+ public:
+  NcClass(const std::string &path, int dims, const std::vector<int> &shape, int ns, int ntraj,
+          bool is_complex, double Lx, double Ly, double Lz);
+  ~NcClass();
+  // one trajectory's arrays (any may be null: that part stays at the fill value)
+  void put(int traj, const float *u, const float *v, const float *u0, const float *v0,
+           const float *m, const float *c);
+  void close();
+
+ private:
+  void close_locked();
+  int64_t file_ = -1, dt_ = -1, ds_ = -1, vt_[2] = {-1, -1}, vs_[2] = {-1, -1}, vm_ = -1, vc_ = -1;
+  uint64_t spat_ = 0;
+  int ns_ = 0, ntraj_ = 0;
+  bool complex_ = false;
 };
 
 } // namespace h5

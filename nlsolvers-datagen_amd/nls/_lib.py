@@ -65,7 +65,8 @@ class H5Run(C.Structure):
                 ("u0", C.POINTER(C.c_double)), ("v0", C.POINTER(C.c_double)),
                 ("m", C.POINTER(C.c_double)), ("c", C.POINTER(C.c_double)),
                 ("ns_out", C.c_int), ("ox", C.c_int), ("oy", C.c_int), ("oz", C.c_int),
-                ("u", C.POINTER(C.c_double)), ("v", C.POINTER(C.c_double))]
+                ("u", C.POINTER(C.c_double)), ("v", C.POINTER(C.c_double)),
+                ("c_type", C.c_char_p)]
 
 
 class Request(C.Structure):
@@ -122,6 +123,10 @@ _SIGS = {
     "nls_matfunc_destroy": [_P],
     "nls_matfunc_apply_real": [_P, _P, _P, C.c_double, C.c_int],
     "nls_matfunc_apply_complex": [_P, _P, _P, C.c_double, C.c_double, C.c_int],
+    "nls_nc_class_create": [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_int,
+                            C.c_double, C.c_double, C.c_double, C.POINTER(C.c_void_p)],
+    "nls_nc_class_put": [_P, C.c_int, _F, _F, _F, _F, _F, _F],
+    "nls_nc_class_close": [_P],
     "nls_matfunc_apply_real_noalloc": [C.c_int64, _P, _P, _P, _P, _P, C.c_double, C.c_int,
                                        C.c_int, _P],
     "nls_matfunc_last_krylov": [_P, _D, _D],

@@ -31,7 +31,8 @@ class _Run:
 
     def __init__(self, path, *, u0, m, c, u, v=None, v0=None, Lx, Ly=None, Lz=None, T, nt,
                  num_snapshots, problem_type, run_id, run_index, phenomenon="",
-                 phenomenon_params=None, m_type="constant", elapsed_time=0.0, timestamp=None):
+                 phenomenon_params=None, m_type="constant", elapsed_time=0.0, timestamp=None,
+                 c_type=""):
         u0 = np.asarray(u0)
         cplx = np.iscomplexobj(u0)
         dim = u0.ndim
@@ -70,7 +71,7 @@ class _Run:
             int(run_index), float(elapsed_time), len(params), kt, vt,
             arr(u0, True), None if cplx else arr(v0), arr(m), arr(c),
             u.shape[0], u.shape[1], u.shape[2], u.shape[3] if dim == 3 else 1,
-            arr(u, True), None if cplx else arr(v))
+            arr(u, True), None if cplx else arr(v), _b(c_type) if c_type else None)
         self._keep = keep
 
 
@@ -78,8 +79,8 @@ def save_run(path, **kw):
     """Write one trajectory file now.  Keyword arguments: u0, m, c, u (stored
     trajectory, complex for NLSE runs), v0 and v for real runs, Lx[, Ly, Lz], T, nt,
     num_snapshots, problem_type, run_id, run_index, phenomenon, phenomenon_params
-    (dict, stored as metadata attributes phenomenon_<key> = str(value)), m_type,
-    elapsed_time, timestamp (default: now)."""
+    (dict, stored as metadata attributes phenomenon_<key> = str(value)), m_type, c_type
+    (metadata attribute c_type when set), elapsed_time, timestamp (default: now)."""
     r = _Run(path, **kw)
     call("nls_h5_write_run", C.byref(r.rec))
     return path
@@ -136,6 +137,15 @@ def read(path, name):
     call("nls_h5_read_dataset", _b(path), _b(name), dptr(buf), buf.size, dims, C.byref(nd),
          C.byref(cx))
     return (buf.view(np.complex128) if cx.value else buf).reshape(shape)
+
+
+def shape(path, name):
+    """(shape, is_complex) of dataset `name` without reading it."""
+    dims = (C.c_int64 * 4)()
+    nd = C.c_int()
+    cx = C.c_int()
+    call("nls_h5_read_dataset", _b(path), _b(name), None, 0, dims, C.byref(nd), C.byref(cx))
+    return tuple(dims[k] for k in range(nd.value)), bool(cx.value)
 
 
 def read_attr(path, obj, name):

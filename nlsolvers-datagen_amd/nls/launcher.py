@@ -203,12 +203,12 @@ class BatchLauncher:
         return os.path.join(self.h5_dir, f"run_{self.run_id}_{idx:04d}.h5")
 
     def sample(self, idx):
-        """(u0, v0, m, c, phenomenon params, m class) of run idx."""
+        """(u0, v0, m, c, phenomenon params, m class, c class) of run idx."""
         cfg, n, ph = self.cfg, self.cfg.nx, self.phenomenon
         if cfg.dim == 3:
             u0, v0 = S.bubbles_3d(n, cfg.Lx, cfg.seed, idx)
             one = np.ones_like(u0)
-            return u0, v0, one, one, {}, "constant"
+            return u0, v0, one, one, {}, "constant", "constant"
         c_kind = cfg.c_classes[idx % len(cfg.c_classes)]
         m_kind = cfg.m_classes[(idx // len(cfg.c_classes)) % len(cfg.m_classes)]
         c, _ = F.c_field(c_kind, n, cfg.Lx, cfg.seed, idx)
@@ -219,7 +219,7 @@ class BatchLauncher:
             else:
                 psi, pk = S.nlse_packets_2d(n, cfg.Lx, cfg.seed, idx)
                 prm = {"packets": pk}
-            return psi, None, m, c, prm, m_kind
+            return psi, None, m, c, prm, m_kind, c_kind
         if ph == "kink_field":
             u0, v0, prm = S.kink_field_2d(n, cfg.Lx, cfg.seed, idx)
         elif ph == "kink_ring":
@@ -227,16 +227,16 @@ class BatchLauncher:
         else:
             u0, v0, g = S.gaussian_sum_2d(n, cfg.Lx, cfg.seed, idx)
             prm = {"gaussians": g}
-        return u0, v0, m, c, prm, m_kind
+        return u0, v0, m, c, prm, m_kind, c_kind
 
     def _write(self, writer, idx, s, u, v, elapsed):
         cfg = self.cfg
-        u0, v0, m, c, prm, m_kind = s
+        u0, v0, m, c, prm, m_kind = s[:6]
         u = downsample(u, cfg)
         kw = dict(u0=u0, m=m, c=c, u=u, Lx=cfg.Lx, Ly=cfg.Ly, Lz=cfg.Lz, T=cfg.T, nt=cfg.nt,
                   num_snapshots=cfg.num_snapshots, run_id=self.run_id, run_index=idx,
                   phenomenon=self.phenomenon,
-                  phenomenon_params=prm, m_type=m_kind, elapsed_time=elapsed,
+                  phenomenon_params=prm, m_type=m_kind, c_type=s[6], elapsed_time=elapsed,
                   problem_type={"kge": "klein_gordon", "nlse": "cubic", "sp4": "phi4"}[self.kind])
         if self.kind != "nlse":
             kw.update(v0=v0, v=downsample(v, cfg))
