@@ -207,6 +207,11 @@ NLS_API int nls_nlse_apply_bc(nls_nlse *h);
 /* Driver loop (nlse_cubic_driver_2d.cpp:107-110) and the snapshot array (ns*N complex128). */
 NLS_API int nls_nlse_run(nls_nlse *h);
 NLS_API int nls_nlse_transfer_snapshots(nls_nlse *h, double *u);
+/* NLSESolverDevice::store_snapshot_online (nlse_dev.hpp:324-335): the current state goes to
+ * snapshot slot k (k = snapshots stored so far; nothing once num_snapshots are stored), into
+ * the library's snapshot array and, if host_dst is non-NULL, to host_dst + k * N
+ * (complex128, synchronous, as the reference's cudaMemcpy). */
+NLS_API int nls_nlse_store_snapshot_online(nls_nlse *h, double *host_dst);
 NLS_API int nls_nlse_get_state(nls_nlse *h, double *u);
 
 /* ---------------------------------------------------------------- stochastic phi^4 (plain FP64) */

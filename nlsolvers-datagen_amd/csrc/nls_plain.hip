@@ -1173,6 +1173,19 @@ int nls_nlse_transfer_snapshots(nls_nlse *h, double *u) {
     NLS_HIP(hipStreamSynchronize(h->s));
   });
 }
+int nls_nlse_store_snapshot_online(nls_nlse *h, double *host_dst) {
+  return guard([&] {
+    NLS_REQUIRE(h, "nls_nlse_store_snapshot_online: null handle");
+    const int slot = h->current_snapshot;
+    if (slot >= h->p.num_snapshots) return; // nlse_dev.hpp:325-327
+    h->store_snapshot_online();             // the library's own snapshot array
+    if (host_dst) {
+      NLS_HIP(hipMemcpyAsync(host_dst + (size_t)2 * slot * h->N, h->u->p, sizeof(c128) * h->N,
+                             hipMemcpyDeviceToHost, h->s));
+      NLS_HIP(hipStreamSynchronize(h->s));
+    }
+  });
+}
 int nls_nlse_get_state(nls_nlse *h, double *u) {
   return guard([&] {
     NLS_HIP(hipMemcpyAsync(u, h->u->p, sizeof(c128) * h->N, hipMemcpyDeviceToHost, h->s));

@@ -150,6 +150,7 @@ _SIGS = {
     "nls_nlse_run": [_P],
     "nls_nlse_transfer_snapshots": [_P, _D],
     "nls_nlse_get_state": [_P, _D],
+    "nls_nlse_store_snapshot_online": [_P, _D],
     "nls_sp4_create": [C.POINTER(SP4Params), _D, _D, _D, _D, C.POINTER(C.c_void_p)],
     "nls_sp4_destroy": [_P],
     "nls_sp4_step": [_P],

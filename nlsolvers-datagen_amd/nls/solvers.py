@@ -207,6 +207,15 @@ class NLSESolverDevice(_Handle):
         call("nls_nlse_transfer_snapshots", self.h, dptr(out.view(np.float64)))
         return out
 
+    def store_snapshot_online(self, host_dst=None):
+        """Stores the current state as the next snapshot (nlse_dev.hpp:324-335); with
+        host_dst, a complex128 array of (num_snapshots, *grid), also copies it there."""
+        if host_dst is not None:
+            assert host_dst.dtype == np.complex128 and host_dst.flags.c_contiguous
+            assert host_dst.shape == (self.ns,) + self.shape
+        call("nls_nlse_store_snapshot_online", self.h,
+             None if host_dst is None else dptr(host_dst.view(np.float64)))
+
     def state(self):
         out = np.empty(self.shape, np.complex128)
         call("nls_nlse_get_state", self.h, dptr(out.view(np.float64)))

@@ -277,3 +277,31 @@ def test_philox_normals_bitexact(orc):
         z = philox_normals(seed, stream, step, 10007)
         ref = orc.noise_field(seed, stream, step, 10007)
         assert np.array_equal(z, ref)
+
+
+def test_nlse_store_snapshot_online_to_host():
+    """NLSESolverDevice.store_snapshot_online(host_dst): the current state lands in the next
+    slot of the caller's host array (and of the solver's own snapshot array); nothing is
+    stored past num_snapshots (nlse_dev.hpp:324-335)."""
+    from nls.solvers import NLSESolverDevice
+    n, L = 32, 10.0
+    u0, _ = field(n, 2)
+    one = np.ones((n, n))
+    # nt = 10, 3 snapshots: snapshot 0 at creation, step_number % 3 == 0 stores more
+    s = NLSESolverDevice(n, L, 0.05, 10, 3, one, one, u0.astype(np.complex128), krylov_dim=8)
+    host = np.zeros((3, n, n), np.complex128)
+    s.step(1)
+    s.apply_bc()
+    s.store_snapshot_online(host)  # slot 1
+    st1 = s.state()
+    s.step(2)
+    s.apply_bc()
+    s.store_snapshot_online(host)  # slot 2
+    st2 = s.state()
+    s.store_snapshot_online(host)  # full: nothing stored
+    np.testing.assert_array_equal(host[1], st1)
+    np.testing.assert_array_equal(host[2], st2)
+    assert np.all(host[0] == 0)
+    snaps = s.transfer_snapshots()
+    np.testing.assert_array_equal(snaps[1], st1)
+    np.testing.assert_array_equal(snaps[2], st2)
