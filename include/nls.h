@@ -339,6 +339,30 @@ NLS_API int nls_ic_draw(int kind, uint64_t seed, int traj, double L, double *par
  * constant.  Stores snapshot 0. */
 NLS_API int nls_batch_set_initial_sampled(nls_batch *b, int t, int kind, uint64_t seed,
                                           int traj_index, double m_const, double c_const);
+/* Coefficient field of class `type` on the GPU (replaces the host generators
+ * scripts/c_fields_2d.py:169-248 generate_c_fields and scripts/m_fields_2d.py:147-238
+ * generate_m_fields, one field per call): family "c" (constant, periodic_structure,
+ * piecewise_constant, sign_changing_mass, layered, waveguide, quasiperiodic, turbulent;
+ * base = base_value) or "m" (constant, piecewise, gradient, phase, topological, defects,
+ * quasiperiodic, multiscale; base = m0; gradient / phase / topological derive from the
+ * device field c, n*n doubles, and fall back to constant when c is NULL, as the
+ * reference's dispatch does).  out: device, n*n doubles on linspace(-L, L, n)^2, "ij"
+ * order.  Parameters come from the trajectory's Philox stream (tags 0xC0C0 for c,
+ * 0x3A55 for m), the same draws as nls/fields.py.  stream: a hipStream_t or NULL;
+ * returns after the field is written. */
+NLS_API int nls_field_sample(const char *family, const char *type, int n, double L,
+                             uint64_t seed, int traj, double base, const double *c,
+                             double *out, void *stream);
+/* The reference launchers' phenomena on the GPU, their parameters drawn from the
+ * reference's spaces: "kink_field" (real_sampler.py:223-271 with the anisotropic random
+ * velocity field :48-65; u, v device n*n doubles: u0, v0) or "multi_soliton"
+ * (nlse_sampler.py:102-219; u device 2*n*n doubles, interleaved complex psi0, divided by
+ * max |psi0| when normalize != 0; v unused).  choices (host, may be NULL) receives the
+ * index of the value chosen for each key of the space, in key order (kink_field: 5;
+ * multi_soliton: 9, then the soliton count); *count their number. */
+NLS_API int nls_phenomenon_sample(const char *name, int n, double L, uint64_t seed, int traj,
+                                  int normalize, double *u, double *v, int *choices, int cap,
+                                  int *count, void *stream);
 /* Advances every trajectory by nsteps; *ms receives the HIP-event time of the
  * region on the batch stream (NULL to skip). */
 NLS_API int nls_batch_run(nls_batch *b, int nsteps, float *ms);

@@ -171,6 +171,10 @@ _SIGS = {
     "nls_batch_set_profiling": [_P, C.c_int],
     "nls_batch_monitor": [_P, C.c_int, C.POINTER(C.c_int), _D, _D, C.POINTER(C.c_int)],
     "nls_ic_draw": [C.c_int, C.c_uint64, C.c_int, C.c_double, _D, C.c_int, C.POINTER(C.c_int)],
+    "nls_field_sample": [C.c_char_p, C.c_char_p, C.c_int, C.c_double, C.c_uint64, C.c_int,
+                         C.c_double, _P, _P, _P],
+    "nls_phenomenon_sample": [C.c_char_p, C.c_int, C.c_double, C.c_uint64, C.c_int, C.c_int,
+                              _P, _P, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int), _P],
     "nls_batch_set_initial_sampled": [_P, C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_double,
                                       C.c_double],
     "nls_batch_tile": [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],

@@ -186,3 +186,28 @@ def m_field(kind, n, L, seed, traj, c=None, m0=1.0, rng=None):
             m += ps.uniform(a_lo, a_hi) * m0 * sm / np.max(np.abs(sm))
         return np.maximum(m, 0.1 * m0), {"type": kind, "num_scales": scales}
     raise ValueError(f"unknown m class {kind!r}; known: {M_CLASSES}")
+
+
+# ---------------------------------------------------------------- the same classes on the GPU
+def _field_device(family, kind, n, L, seed, traj, base, c=None):
+    import ctypes as C
+    from ._lib import call
+    from .device import DeviceArray
+    out = DeviceArray((n, n), np.float64)
+    call("nls_field_sample", family.encode(), kind.encode(), int(n), float(L), int(seed),
+         int(traj), float(base), C.c_void_p(c.ptr) if c is not None else None,
+         C.c_void_p(out.ptr), None)
+    return out
+
+
+def c_field_device(kind, n, L, seed, traj, base=1.0):
+    """c_field evaluated on the GPU (nls_field_sample): the same draws from the
+    trajectory's Philox stream, the field left in device memory (a DeviceArray)."""
+    return _field_device("c", kind, n, L, seed, traj, base)
+
+
+def m_field_device(kind, n, L, seed, traj, c=None, m0=1.0):
+    """m_field on the GPU; c is the device conductivity field (DeviceArray) the gradient,
+    phase and topological classes derive from (constant without it)."""
+    return _field_device("m", kind, n, L, seed, traj, m0, c)
+

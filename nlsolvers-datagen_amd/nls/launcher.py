@@ -58,6 +58,8 @@ class LaunchConfig:
     dr_strategy: str = "none"     # none | interpolation | FFT
     output_dir: str = "results"
     run_id: Optional[str] = None
+    sampler: str = "device"       # device: fields and phenomena drawn on the GPU
+                                  # (nls_field_sample / nls_phenomenon_sample); host: nls.fields
 
 
 def load_config(path, **overrides):
@@ -174,6 +176,8 @@ def resolve_phenomenon(name, family):
 class BatchLauncher:
     def __init__(self, cfg: LaunchConfig):
         self.cfg = cfg
+        if cfg.sampler not in ("device", "host"):
+            raise ValueError(f"sampler {cfg.sampler!r}: device or host")
         self.run_id = cfg.run_id or str(uuid.uuid4())[:8]
         self.h5_dir = os.path.join(cfg.output_dir, "hdf5")
         os.makedirs(self.h5_dir, exist_ok=True)
@@ -211,16 +215,28 @@ class BatchLauncher:
             return u0, v0, one, one, {}, "constant", "constant"
         c_kind = cfg.c_classes[idx % len(cfg.c_classes)]
         m_kind = cfg.m_classes[(idx // len(cfg.c_classes)) % len(cfg.m_classes)]
-        c, _ = F.c_field(c_kind, n, cfg.Lx, cfg.seed, idx)
-        m, _ = F.m_field(m_kind, n, cfg.Lx, cfg.seed, idx, c=c)
+        dev = cfg.sampler == "device"
+        if dev:
+            cd = F.c_field_device(c_kind, n, cfg.Lx, cfg.seed, idx)
+            c = cd.to_host()
+            m = F.m_field_device(m_kind, n, cfg.Lx, cfg.seed, idx, c=cd).to_host()
+        else:
+            c, _ = F.c_field(c_kind, n, cfg.Lx, cfg.seed, idx)
+            m, _ = F.m_field(m_kind, n, cfg.Lx, cfg.seed, idx, c=c)
         if self.kind == "nlse":
-            if ph == "multi_soliton":
+            if ph == "multi_soliton" and dev:
+                psi, prm = S.multi_soliton_device(n, cfg.Lx, cfg.seed, idx)
+                psi = psi.to_host()
+            elif ph == "multi_soliton":
                 psi, prm = S.multi_soliton_2d(n, cfg.Lx, cfg.seed, idx)
             else:
                 psi, pk = S.nlse_packets_2d(n, cfg.Lx, cfg.seed, idx)
                 prm = {"packets": pk}
             return psi, None, m, c, prm, m_kind, c_kind
-        if ph == "kink_field":
+        if ph == "kink_field" and dev:
+            ud, vd, prm = S.kink_field_device(n, cfg.Lx, cfg.seed, idx)
+            u0, v0 = ud.to_host(), vd.to_host()
+        elif ph == "kink_field":
             u0, v0, prm = S.kink_field_2d(n, cfg.Lx, cfg.seed, idx)
         elif ph == "kink_ring":
             u0, v0, prm = S.kink_ring_2d(n, cfg.Lx, cfg.seed, idx)

@@ -437,3 +437,39 @@ def multi_soliton_2d(n, L, seed, traj, system_type=None, amplitude_range=None, w
               "velocity_scale": velocity_scale, "chirp_range": chirp_range,
               "aspect_ratio_range": aspect_ratio_range, "n_solitons": n_solitons}
     return u, params
+
+
+# ---------------------------------------------------------------- the phenomena on the GPU
+def _space_choice(space, choices):
+    return {k: v[int(i)] for (k, v), i in zip(space.items(), choices)}
+
+
+def _phenomenon_device(name, n, L, seed, traj, normalize, complex_out):
+    import ctypes as C
+    from ._lib import call
+    from .device import DeviceArray
+    u = DeviceArray((n, n), np.complex128 if complex_out else np.float64)
+    v = None if complex_out else DeviceArray((n, n), np.float64)
+    ch = (C.c_int * 16)()
+    cnt = C.c_int()
+    call("nls_phenomenon_sample", name.encode(), int(n), float(L), int(seed), int(traj),
+         int(bool(normalize)), C.c_void_p(u.ptr), C.c_void_p(v.ptr) if v is not None else None,
+         ch, 16, C.byref(cnt), None)
+    return u, v, list(ch[:cnt.value])
+
+
+def kink_field_device(n, L, seed, traj):
+    """kink_field_2d on the GPU (nls_phenomenon_sample): (u0, v0) as DeviceArrays and
+    the chosen space parameters, the same draws and forms as the host sampler."""
+    u, v, ch = _phenomenon_device("kink_field", n, L, seed, traj, False, False)
+    return u, v, _space_choice(KINK_FIELD_SPACE, ch)
+
+
+def multi_soliton_device(n, L, seed, traj, normalize=True):
+    """multi_soliton_2d on the GPU: psi0 as a complex128 DeviceArray and the parameters
+    (the chosen space values and n_solitons)."""
+    u, _, ch = _phenomenon_device("multi_soliton", n, L, seed, traj, normalize, True)
+    params = _space_choice(MULTI_SOLITON_SPACE, ch[:len(MULTI_SOLITON_SPACE)])
+    params["n_solitons"] = ch[len(MULTI_SOLITON_SPACE)]
+    return u, params
+

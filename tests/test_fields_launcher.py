@@ -113,11 +113,11 @@ def test_launcher_refuses_unbuilt_phenomena_and_labels_what_it_generates(tmp_pat
         BatchLauncher(LaunchConfig(pde_type="klein_gordon", dim=3, nx=16, ny=16, nz=16,
                                    m_classes=["defects"], output_dir=str(tmp_path)))
     lk = BatchLauncher(LaunchConfig(pde_type="klein_gordon", nx=24, ny=24, Lx=3.0, Ly=3.0,
-                                    output_dir=str(tmp_path)))
+                                    output_dir=str(tmp_path), sampler="host"))
     u0, v0, m, c, prm, _, _ = lk.sample(0)
     assert lk.phenomenon == "kink_field" and "winding_x" in prm and np.any(v0 != 0)
     ln = BatchLauncher(LaunchConfig(pde_type="Nonlinear Schrodinger", nx=24, ny=24,
-                                    output_dir=str(tmp_path)))
+                                    output_dir=str(tmp_path), sampler="host"))
     psi, _, _, _, prm, _, _ = ln.sample(1)
     assert ln.phenomenon == "multi_soliton" and "n_solitons" in prm
     assert np.iscomplexobj(psi) and abs(np.abs(psi).max() - 1.0) < 1e-12
