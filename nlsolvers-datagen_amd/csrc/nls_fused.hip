@@ -1393,7 +1393,14 @@ __global__ __launch_bounds__(FBS, (CPT == 4 && UNI && STAGE == 2) ? 4 : 1) void 
   const int n = T.n, n2 = n * n, N = (int)T.N;
   const int64_t cells = a.cells;
   const size_t m2 = (size_t)(a.m - 2);
-  float *sc = lds, *sv0 = lds + (UNI ? 0 : emax), *sv1 = sv0 + emax; // no c window if UNI
+  // two windows (planes of even and odd z), each c and v_j of both sequences (no c if UNI)
+  const int wstride = (UNI ? 2 : 3) * emax;
+  auto win = [&](int z, float *&c_, float *&v0_, float *&v1_) {
+    float *b_ = lds + (z & 1) * wstride;
+    c_ = b_;
+    v0_ = b_ + (UNI ? 0 : emax);
+    v1_ = v0_ + emax;
+  };
   const float *cT = a.cf + T.off, *mT = a.mf + T.off;
   const double *uT = a.u + T.off;
   const float *X0 = a.W + ((size_t)0 * 2 + (STAGE == 1 ? 0 : ((j - 1) & 1))) * cells + T.off;
@@ -1473,7 +1480,7 @@ __global__ __launch_bounds__(FBS, (CPT == 4 && UNI && STAGE == 2) ? 4 : 1) void 
   float vm0[CPT], vm1[CPT], vc0[CPT], vc1[CPT], vn0[CPT], vn1[CPT];
   float cmz[CPT], ccz[CPT], cnz[CPT], pc0[CPT], pc1[CPT], pn0[CPT], pn1[CPT];
   Raw ro[CPT];        // own cells of plane z + 2 (in flight)
-  Raw rh[M3_HPT];     // window ends of plane z + 1 (in flight)
+  Raw rh[M3_HPT];     // window ends of plane z + 2 (in flight)
   // window end h in [0, 2n): window index e and flat cell of plane z
   auto hwin = [&](int h) { return h < n ? h : S + h; };
   auto hcell = [&](int h, int z) { return z * n2 + pa - n + hwin(h); };
@@ -1489,6 +1496,24 @@ __global__ __launch_bounds__(FBS, (CPT == 4 && UNI && STAGE == 2) ? 4 : 1) void 
       if (h < 2 * n) {
         const int g = hcell(h, z);
         if (g >= 0 && g < N) fetch(g, rh[k]);
+      }
+    }
+  };
+  // window ends of plane z from rh into plane z's window
+  auto put_halo = [&](int z) {
+    float *hc, *h0, *h1;
+    win(z, hc, h0, h1);
+#pragma unroll
+    for (int k = 0; k < M3_HPT; ++k) {
+      const int h = threadIdx.x + k * FBS;
+      if (h < 2 * n) {
+        const int g = hcell(h, z);
+        if (g >= 0 && g < N) {
+          const int e = hwin(h);
+          float t0, t1;
+          if (!UNI) hc[e] = rh[k].c;
+          form(rh[k], h0[e], h1[e], t0, t1);
+        }
       }
     }
   };
@@ -1514,10 +1539,18 @@ __global__ __launch_bounds__(FBS, (CPT == 4 && UNI && STAGE == 2) ? 4 : 1) void 
       }
     if (z0 + 1 < n) fetch_own(z0 + 1);
     fetch_halo(z0);
+    put_halo(z0);
+    if (z0 + 1 < z1) fetch_halo(z0 + 1);
   }
+  // The window ends of plane z + 2 are loaded with the own cells of plane z + 2, two planes
+  // ahead: the neighbouring ranges load the same cells as their own cells at about the same
+  // time, so one of the two reads is a cache hit.  They are deposited one plane ahead into
+  // the other window, which plane z + 1 then completes with its own cells.
   for (int z = z0; z < z1; ++z) {
-    __syncthreads(); // the previous plane's window reads are done
-    // window of plane z: own cells from registers, ends from the loads in flight
+    __syncthreads(); // plane z - 1's window reads are done: its window takes plane z + 1
+    float *sc, *sv0, *sv1;
+    win(z, sc, sv0, sv1);
+    // window of plane z: own cells from registers (its ends are in place)
 #pragma unroll
     for (int q = 0; q < CPT; ++q)
       if (own[q]) {
@@ -1526,19 +1559,8 @@ __global__ __launch_bounds__(FBS, (CPT == 4 && UNI && STAGE == 2) ? 4 : 1) void 
         sv0[e] = vc0[q];
         sv1[e] = vc1[q];
       }
-#pragma unroll
-    for (int k = 0; k < M3_HPT; ++k) {
-      const int h = threadIdx.x + k * FBS;
-      if (h < 2 * n) {
-        const int g = hcell(h, z);
-        if (g >= 0 && g < N) {
-          const int e = hwin(h);
-          float t0, t1;
-          if (!UNI) sc[e] = rh[k].c;
-          form(rh[k], sv0[e], sv1[e], t0, t1);
-        }
-      }
-    }
+    // ends of plane z + 1 (loaded one plane ago) into the other window
+    if (z + 1 < z1) put_halo(z + 1);
     // plane z + 1 of the own cells
     const bool up = z + 1 < n;
     if (up) {
@@ -1549,10 +1571,10 @@ __global__ __launch_bounds__(FBS, (CPT == 4 && UNI && STAGE == 2) ? 4 : 1) void 
           cnz[q] = ro[q].c;
         }
     }
-    // next loads: own cells of plane z + 2, window ends of plane z + 1
+    // next loads: own cells and window ends of plane z + 2
     if (z + 1 < z1) {
       if (z + 2 < n) fetch_own(z + 2);
-      fetch_halo(z + 1);
+      if (z + 2 < z1) fetch_halo(z + 2);
     }
     __syncthreads();
 #pragma unroll
@@ -3478,7 +3500,7 @@ struct nls_batch {
   void launch_pass(const KgeArgs &a, int j, hipStream_t s, unsigned ngrid) {
     const dim3 grid(ngrid), blk(FBS);
     if (pass_kind == 4) {
-      const size_t lds = march3d_lds_bytes(cpt, nmax, uni_all);
+      const size_t lds = 2 * march3d_lds_bytes(cpt, nmax, uni_all); // two plane windows
       const int emax = FBS * cpt + 2 * nmax;
       with_uni([&](auto u) {
         constexpr bool U = decltype(u)::value;
