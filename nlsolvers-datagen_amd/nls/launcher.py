@@ -48,6 +48,8 @@ class LaunchConfig:
     phenomenon: str = ""          # "" = the equation's default (PHENOMENA)
     integrator: str = ""          # gautschi | sewi | sv (KGE), strang (NLSE), damped | reference (phi^4)
     krylov_dim: int = 0           # 0: autotuner
+    lanczos_variant: str = "full"  # full: textbook Lanczos (alpha_{m-1} computed);
+                                   # reference: T[m-1][m-1] = 0 as lanczos.hpp:208-267
     mu2: float = 0.0
     gamma: float = 0.1
     sigma: float = 0.1
@@ -90,7 +92,7 @@ def load_config(path, **overrides):
     c.m_classes = list(y.get("m_classes", c.m_classes))
     c.c_classes = list(y.get("c_classes", c.c_classes))
     for k in ("integrator", "krylov_dim", "mu2", "seed", "phenomenon", "batch", "dr_x", "dr_y",
-              "dr_strategy", "gamma", "sigma"):
+              "dr_strategy", "gamma", "sigma", "lanczos_variant"):
         if k in y:
             setattr(c, k, type(getattr(c, k))(y[k]))
     for k, v in overrides.items():
@@ -327,7 +329,7 @@ class BatchLauncher:
                        snapshot_every=every, max_snapshots=cfg.num_snapshots,
                        cells_per_thread=tile["cells_per_thread"] if tile else 0,
                        pass_kernel=tile["pass_kind"] if tile else None,
-                       lanes=tile["lanes"] if tile else 0)
+                       lanes=tile["lanes"] if tile else 0, variant=cfg.lanczos_variant)
         for k, s in enumerate(samples):
             b.set_initial(k, s[2], s[3], s[0], s[1])
         b.run(cfg.nt - 1)  # reference cadence: nt - 1 steps (kg_driver_dev_2d.cpp:144)
@@ -364,6 +366,8 @@ def main(argv=None):
     ap.add_argument("--krylov-dim", dest="krylov_dim", type=int, default=None)
     ap.add_argument("--mu2", type=float, default=None)
     ap.add_argument("--batch", type=int, default=None)
+    ap.add_argument("--lanczos-variant", dest="lanczos_variant", default=None,
+                    choices=("full", "reference"))
     a = ap.parse_args(argv)
     ov = {k: v for k, v in vars(a).items() if k not in ("config", "m_type", "c_type")}
     if a.m_type:

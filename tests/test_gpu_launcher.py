@@ -23,9 +23,9 @@ def h5():
 
 @pytest.mark.parametrize("integrator", ["gautschi", "sv"])
 def test_launcher_kge_dataset_matches_oracle(orc, h5, tmp_path, integrator):
-    """Gautschi (FP32 Krylov) within the 1e-5 bar; Stormer-Verlet (FP64, the reference's
-    arithmetic) within 1e-6: the batch keeps m and c in FP32, the oracle reads the files'
-    FP64 fields."""
+    """Gautschi (FP32 Krylov, the launcher's full-T Lanczos against the oracle's) within the
+    1e-5 bar; Stormer-Verlet (FP64, the reference's arithmetic) within 1e-6: the batch keeps
+    m and c in FP32, the oracle reads the files' FP64 fields."""
     from nls.launcher import BatchLauncher, load_config
     cfg = load_config(os.path.join(GOLDEN, "dataset_kge_2d.yaml"), output_dir=str(tmp_path),
                       run_id="t", dr_strategy="none", batch=2, integrator=integrator)
@@ -38,7 +38,7 @@ def test_launcher_kge_dataset_matches_oracle(orc, h5, tmp_path, integrator):
         m, c = h5.read(p, "focusing/m"), h5.read(p, "focusing/c")
         assert u.shape == (5, 48, 48) and v.shape == (5, 48, 48)
         ur, _ = orc.kge_run(48, 10.0, 0.5, 100, 5, m, c, u0, v0, krylov=10, mu2=1.0,
-                            method=1 if integrator == "sv" else 0)
+                            method=1 if integrator == "sv" else 0, variant=1)
         for k in range(5):
             assert rel_l2(u[k], ur[k]) < (1e-6 if integrator == "sv" else 1e-5), (idx, k)
         assert h5.read_attr(p, "metadata", "run_index") == idx
