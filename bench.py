@@ -92,8 +92,8 @@ def fused_run(args, inputs):
         info["krylov_monitor"].update(
             max_orth=float(max(x["max_orth"] for x in mons)),
             reorth_on=int(sum(x["reorth_on"] for x in mons)))
-    # dominant-kernel timing: a few eager steps with HIP events around each
-    # Lanczos pass launch, on the batch stream
+    # dominant-kernel timing: a few eager steps on the batch stream, with HIP events around
+    # the back-to-back general Lanczos passes of each step (divided by their number)
     pt = {"avg_ms": 0.0, "launches": 0, "bytes_per_launch": 0.0}
     if args.prof_steps > 0:
         b.set_profiling(True)
@@ -666,7 +666,9 @@ def main():
                      "bytes_per_launch": pt["bytes_per_launch"],
                      "avg_launch_ms": pt["avg_ms"], "launches_timed": pt["launches"],
                      "measured_as": "the same kernel and tile over the whole batch on one "
-                                    "stream (the timed steps run the batch as concurrent lanes)",
+                                    "stream (the timed steps run the batch as concurrent lanes); "
+                                    "HIP events around the back-to-back general passes of "
+                                    "each profiled step, divided by their number",
                      "stream_copy_gbs": stream_gbs,
                      "step_bytes_per_cell": step_bytes,
                      "step_effective_gbs": step_gbs,

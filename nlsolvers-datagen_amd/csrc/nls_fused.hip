@@ -3195,6 +3195,7 @@ struct nls_batch {
   int ntrajL[MAXL] = {};
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
+  std::vector<int> ev_launches; // general passes between the events of each recorded pair
   double pass_ms = 0.0;
   int pass_launches = 0;
 
@@ -3343,14 +3344,17 @@ struct nls_batch {
       hipLaunchKernelGGL((k_nlse_pass<DIM, 1, CPT>), grid, blk, 0, s, a, 1);
     }
     reorth(1);
+    const bool each = reorth_live;
+    prof_open(prof, each, s);
     for (int j = 2; j <= m - 2; ++j) {
-      if (prof) NLS_HIP(hipEventRecord(next_event(), s));
+      prof_each(prof, each, s, false);
       if (march) march_pass(2, j);
       else if (plane) plane_pass(2, j);
       else hipLaunchKernelGGL((k_nlse_pass<DIM, 2, CPT>), grid, blk, 0, s, a, j);
-      if (prof) NLS_HIP(hipEventRecord(next_event(), s));
+      prof_each(prof, each, s, true);
       reorth(j);
     }
+    prof_close(prof, each, s, m - 3);
     if (p.variant == 1) { // full T: the partials of alpha_{m-1} (stage 3 = last pass)
       if (march) march_pass(3, m - 1);
       else if (plane) plane_pass(3, m - 1);
@@ -3494,6 +3498,26 @@ struct nls_batch {
     }
     return ev_pool[ev_used++];
   }
+  // Timing of the general passes: one event pair around the back-to-back general passes
+  // of a step (events between kernels add their own time to each launch), or one pair
+  // per pass when reorthogonalisation kernels run between them.
+  void prof_open(bool prof, bool each, hipStream_t st) {
+    if (prof && !each) NLS_HIP(hipEventRecord(next_event(), st));
+  }
+  void prof_close(bool prof, bool each, hipStream_t st, int passes) {
+    if (prof && !each && passes > 0) {
+      NLS_HIP(hipEventRecord(next_event(), st));
+      ev_launches.push_back(passes);
+    } else if (prof && !each) {
+      --ev_used; // no general pass in this step
+    }
+  }
+  void prof_each(bool prof, bool each, hipStream_t st, bool after) {
+    if (prof && each) {
+      NLS_HIP(hipEventRecord(next_event(), st));
+      if (after) ev_launches.push_back(1);
+    }
+  }
 
   // Pass kernels: kind 0 = direct global-memory stencil (any dim), 1 = LDS-tiled
   // chunk + halo rows (2D), 2 = row march with an LDS window (2D), 3 = row march
@@ -3561,23 +3585,24 @@ struct nls_batch {
       launch_rpass<0>(a, 0, st, ngrid);
       launch_rpass<1>(a, 1, st, ngrid);
       if (m - 2 >= 2) launch_rpass<2>(a, 2, st, ngrid);
-      for (int j = 3; j <= m - 2; ++j) {
-        if (prof) NLS_HIP(hipEventRecord(next_event(), st));
-        launch_rpass<3>(a, j, st, ngrid);
-        if (prof) NLS_HIP(hipEventRecord(next_event(), st));
-      }
+      prof_open(prof, false, st);
+      for (int j = 3; j <= m - 2; ++j) launch_rpass<3>(a, j, st, ngrid);
+      prof_close(prof, false, st, m - 4);
     } else {
       launch_pass<DIM, 0>(a, 0, st, ngrid);
       if (m - 2 >= 1) {
         launch_pass<DIM, 1>(a, 1, st, ngrid);
         launch_reorth(a, 1, st, ngrid);
       }
+      const bool each = reorth_live;
+      prof_open(prof, each, st);
       for (int j = 2; j <= m - 2; ++j) {
-        if (prof) NLS_HIP(hipEventRecord(next_event(), st));
+        prof_each(prof, each, st, false);
         launch_pass<DIM, 2>(a, j, st, ngrid);
-        if (prof) NLS_HIP(hipEventRecord(next_event(), st));
+        prof_each(prof, each, st, true);
         launch_reorth(a, j, st, ngrid);
       }
+      prof_close(prof, each, st, m - 3);
       if (p.variant == 1) { // full T: the partials of alpha_{m-1}
         if (m - 1 == 1) launch_pass<DIM, 1, true>(a, 1, st, ngrid);
         else launch_pass<DIM, 2, true>(a, m - 1, st, ngrid);
@@ -3815,6 +3840,7 @@ struct nls_batch {
     NLS_HIP(hipEventCreate(&e0));
     NLS_HIP(hipEventCreate(&e1));
     ev_used = 0;
+    ev_launches.clear();
     NLS_HIP(hipEventRecord(e0, s));
     for (int k = 0; k < nsteps; ++k) {
       const int64_t idx = step + 1; // reference step number i (kg_driver_dev_2d.cpp:144)
@@ -3849,11 +3875,11 @@ struct nls_batch {
     if (profiling) {
       pass_ms = 0.0;
       pass_launches = 0;
-      for (size_t k = 0; k + 1 < ev_used; k += 2) {
+      for (size_t k = 0; k + 1 < ev_used && k / 2 < ev_launches.size(); k += 2) {
         float d = 0;
         NLS_HIP(hipEventElapsedTime(&d, ev_pool[k], ev_pool[k + 1]));
         pass_ms += d;
-        ++pass_launches;
+        pass_launches += ev_launches[k / 2];
       }
     }
     check_status();

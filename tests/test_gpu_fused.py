@@ -863,3 +863,24 @@ def test_fused_kge_sv_bitwise_vs_oracle(orc, dim, q6):
             u, v = b.snapshots(t)
             assert np.array_equal(u, ur), (t, np.abs(u - ur).max())
             assert np.array_equal(v, vr), (t, np.abs(v - vr).max())
+
+
+def test_fused_pass_timing_counts_general_passes():
+    """The profiling leg times the back-to-back general passes of each step with one event
+    pair and reports their number: steps x (m - 3) passes, a positive average, and the
+    algorithmic bytes of one pass."""
+    from nls.batch import FusedBatch
+    n = 96
+    m, c, u0, v0 = make_case(n, 3, 0, ring=True)
+    for kernel in (0, 2):
+        b = FusedBatch([n, n], 10.0, 0.005, krylov_dim=8, mu2=1.0, cells_per_thread=8,
+                       pass_kernel=kernel, variant="full")
+        for t in range(2):
+            b.set_initial(t, m, c, u0, v0)
+        b.set_profiling(True)
+        b.run(4)
+        pt = b.pass_timing()
+        assert pt["launches"] == 4 * (8 - 3), pt
+        assert 0.0 < pt["avg_ms"] < 5.0, pt
+        assert pt["bytes_per_launch"] > 0, pt
+        b.close()
