@@ -2759,7 +2759,14 @@ __global__ __launch_bounds__(FBS) void k_nlse_march3d(NlseArgs a, int j, int ema
   }
   const int n = T.n, n2 = n * n, N = (int)T.N;
   const int64_t cells = a.cells;
-  float *sc = lds, *sv0 = lds + (UNI ? 0 : emax), *sv1 = sv0 + emax; // no c window if UNI
+  // two windows (planes of even and odd z), as k_kge_march3d; no c window if UNI
+  const int wstride = (UNI ? 2 : 3) * emax;
+  auto win = [&](int z, float *&c_, float *&v0_, float *&v1_) {
+    float *b_ = lds + (z & 1) * wstride;
+    c_ = b_;
+    v0_ = b_ + (UNI ? 0 : emax);
+    v1_ = v0_ + emax;
+  };
   const c32 *psiT = a.psi + T.off;
   const float *cT = a.cf + T.off, *mT = a.mf + T.off;
   const c32 *Xin = a.W + (STAGE == 1 ? 0 : (size_t)((j - 1) & 1) * cells) + T.off; // z or w'
@@ -2837,6 +2844,23 @@ __global__ __launch_bounds__(FBS) void k_nlse_march3d(NlseArgs a, int j, int ema
       }
     }
   };
+  auto put_halo = [&](int z) {
+    float *hc, *h0, *h1;
+    win(z, hc, h0, h1);
+#pragma unroll
+    for (int k = 0; k < M3_HPT; ++k) {
+      const int h = threadIdx.x + k * FBS;
+      if (h < 2 * n) {
+        const int g = hcell(h, z);
+        if (g >= 0 && g < N) {
+          const int e = hwin(h);
+          float t0, t1;
+          if (!UNI) hc[e] = rh[k].c;
+          form(rh[k], h0[e], h1[e], t0, t1);
+        }
+      }
+    }
+  };
 
   if (z0 < z1) {
     if (z0 > 0) {
@@ -2858,9 +2882,14 @@ __global__ __launch_bounds__(FBS) void k_nlse_march3d(NlseArgs a, int j, int ema
       }
     if (z0 + 1 < n) fetch_own(z0 + 1);
     fetch_halo(z0);
+    put_halo(z0);
+    if (z0 + 1 < z1) fetch_halo(z0 + 1);
   }
+  // window ends loaded with the own cells two planes ahead, deposited one plane ahead
   for (int z = z0; z < z1; ++z) {
-    __syncthreads();
+    __syncthreads(); // plane z - 1's window reads are done: its window takes plane z + 1
+    float *sc, *sv0, *sv1;
+    win(z, sc, sv0, sv1);
 #pragma unroll
     for (int q = 0; q < CPT; ++q)
       if (own[q]) {
@@ -2869,19 +2898,7 @@ __global__ __launch_bounds__(FBS) void k_nlse_march3d(NlseArgs a, int j, int ema
         sv0[e] = vc0[q];
         sv1[e] = vc1[q];
       }
-#pragma unroll
-    for (int k = 0; k < M3_HPT; ++k) {
-      const int h = threadIdx.x + k * FBS;
-      if (h < 2 * n) {
-        const int g = hcell(h, z);
-        if (g >= 0 && g < N) {
-          const int e = hwin(h);
-          float t0, t1;
-          if (!UNI) sc[e] = rh[k].c;
-          form(rh[k], sv0[e], sv1[e], t0, t1);
-        }
-      }
-    }
+    if (z + 1 < z1) put_halo(z + 1);
     const bool up = z + 1 < n;
     if (up) {
 #pragma unroll
@@ -2893,7 +2910,7 @@ __global__ __launch_bounds__(FBS) void k_nlse_march3d(NlseArgs a, int j, int ema
     }
     if (z + 1 < z1) {
       if (z + 2 < n) fetch_own(z + 2);
-      fetch_halo(z + 1);
+      if (z + 2 < z1) fetch_halo(z + 2);
     }
     __syncthreads();
 #pragma unroll
@@ -3305,7 +3322,7 @@ struct nls_batch {
     const bool plane = pass_kind == 4 && DIM == 3 && (CPT == 4 || CPT == 8);
     const size_t lds = march2d_lds_bytes(nmax, uni_all);
     const int emax = MSUB + 2 * nmax;
-    const size_t lds3 = march3d_lds_bytes(CPT, nmax, uni_all);
+    const size_t lds3 = 2 * march3d_lds_bytes(CPT, nmax, uni_all); // two plane windows
     const int emax3 = FBS * CPT + 2 * nmax;
     // pass j of the 3D plane march (tiles 4 and 8 only)
     auto plane_pass = [&](int stage, int j) {
