@@ -103,6 +103,29 @@ def fused_run(args, inputs):
     return b, ms, wall, info, pt
 
 
+def persistent_leg(make_batch, warmup, steps, graph_ms_per_step):
+    """The same batch stepped by the persistent step kernel (one cooperative launch per
+    run of steps without snapshots, per-trajectory arrival counters between the phases);
+    its tile is the smallest whose chunks fit the resident workgroups."""
+    b = make_batch()
+    b.run(1)
+    if warmup > 1:
+        b.run(warmup - 1)
+    ms = b.run(steps)
+    info = b.info()
+    info.update(b.tile())
+    out = {"value": info["cells"] * steps / (ms / 1e3), "unit": "cell-steps/s",
+           "ms_per_step": ms / steps, "steps": steps,
+           "steps_run": max(warmup, 1) + steps, "steps_run_persistent": b.persistent_steps(),
+           "cells_per_thread": info["cells_per_thread"], "krylov_dim": info["krylov_dim"],
+           "vs_graph_steps": graph_ms_per_step / (ms / steps)}
+    b.close()
+    log(f"[bench] persistent step kernel: {out['value']:.3e} cell-steps/s "
+        f"({out['ms_per_step']:.3f} ms/step, tile {out['cells_per_thread']}), "
+        f"{out['vs_graph_steps']:.2f}x the graph-captured steps")
+    return out
+
+
 def plain_run(args, inputs):
     """Plain drop-in: FP64 CSR SpMV + reference-granularity Lanczos, one trajectory
     after another (the rate does not depend on how many run in sequence)."""
@@ -287,6 +310,17 @@ def run_kge2d_uneven(args):
     log(f"[bench] kge2d uneven: {value:.3e} cell-steps/s ({ms / args.steps:.3f} ms/step), "
         f"m={info['krylov_dim']}, tile {info['cells_per_thread']} kind={info['pass_kind']} "
         f"lanes={info['lanes']}, pass {pt['avg_ms'] * 1e3:.1f} us -> {achieved or 0:.0f} GB/s")
+    b.close()
+    pers = None
+    if not args.skip_persistent:
+        def make_persistent():
+            q = FusedBatch(sizes, L, dt, krylov_dim=info["krylov_dim"], mu2=1.0,
+                           snapshot_every=every, max_snapshots=args.steps // every + 2,
+                           reorth=args.reorth, variant=args.variant, persistent=True)
+            for t in range(ntraj):
+                q.sample_initial(t, "kink_ring_2d", 2, m=1.0, c=1.0)
+            return q
+        pers = persistent_leg(make_persistent, args.warmup, args.steps, ms / args.steps)
     err = None
     if not args.skip_parity:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -307,7 +341,6 @@ def run_kge2d_uneven(args):
                             variant=1 if args.variant == "full" else 0)
         err = float(np.linalg.norm(u[1] - ur[1]) / np.linalg.norm(ur[1]))
         log(f"[bench] kge2d uneven rel-L2 ({n}^2, {S} steps) {err:.2e}")
-    b.close()
     return {"metric": "KGE-2D uneven batch cell-steps/s (fused, snapshots on the device)",
             "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": ms / args.steps, "higher_is_better": True,
@@ -328,7 +361,7 @@ def run_kge2d_uneven(args):
                          "traffic": None, "kernel": "dominant Lanczos pass (stage 2)",
                          "bytes_per_launch": pt["bytes_per_launch"],
                          "avg_launch_ms": pt["avg_ms"]},
-            "rel_l2_vs_fp64_oracle": err, "cpu_baseline": None}
+            "rel_l2_vs_fp64_oracle": err, "persistent": pers, "cpu_baseline": None}
 
 
 def run_nlse2d(args):
@@ -537,6 +570,9 @@ def main():
     ap.add_argument("--skip-parity", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--skip-persistent", action="store_true",
+                    help="skip timing the persistent step kernel beside the captured steps")
+    ap.add_argument("--persistent-steps", type=int, default=100)
     ap.add_argument("--serve-requests", type=int, default=500)
     ap.add_argument("--serve-warmup", type=int, default=50)
     ap.add_argument("--serve-rate", type=float, default=50.0)
@@ -581,6 +617,20 @@ def main():
         f"lanes={info['lanes']}, "
         f"pass avg {pt['avg_ms'] * 1e3:.1f} us -> {achieved or 0:.0f} GB/s")
     del b
+    pers = None
+    if not args.skip_persistent:
+        log("[bench] persistent step kernel, same workload")
+
+        def make_persistent():
+            from nls.batch import FusedBatch
+            q = FusedBatch([args.n] * args.ntraj, args.L, args.dt, krylov_dim=info["krylov_dim"],
+                           mu2=1.0, reorth=args.reorth, always_load_fields=args.load_fields,
+                           variant=args.variant, persistent=True)
+            for t, (m_, c_, u0, v0) in enumerate(inputs):
+                q.set_initial(t, m_, c_, u0, v0)
+            return q
+        pers = persistent_leg(make_persistent, args.warmup, args.persistent_steps,
+                              ms / args.steps)
     import ctypes as C
     from nls._lib import call as _call
     gb, cms = C.c_double(), C.c_double()
@@ -684,6 +734,7 @@ def main():
         "parity": par,
         "parity_other_variant": par_other,
         "krylov_monitor": info["krylov_monitor"],
+        "persistent": pers,
         "latency": serve,
         "cpu_baseline": cpu,
         "wall_s_timed": wall,

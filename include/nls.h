@@ -303,6 +303,16 @@ typedef struct {
                         on pass kernels 0, 2 and 4.  With krylov_dim = 0 the autotuner
                         picks m >= 10 for variant 0 (the reference drivers' m) and
                         m >= 8 for variant 1 */
+  int persistent;    /* 1: run the steps between snapshots as one persistent cooperative
+                        launch (2D Klein-Gordon Gautschi / sEWI on the row-march pass, reorth
+                        0 or 2): one workgroup per chunk for the whole launch, per-trajectory
+                        arrival counters between the phases of a step (bounded waits; an
+                        expired wait fails the run), the small stage run by the chunk that
+                        arrives last.  The tile is cells_per_thread, or the smallest whose
+                        chunks fit the workgroups resident at once.  Results are bitwise those
+                        of the graph-captured steps at the same tile.  Snapshot steps, and
+                        every step once reorthogonalisation has switched on, run the captured
+                        steps.  0: graph-captured steps only */
 } nls_batch_params;
 typedef struct {
   int n;             /* full side */
@@ -382,6 +392,8 @@ NLS_API int nls_batch_pass_timing(nls_batch *b, double *avg_ms, int *launches,
  * storing only the basis vectors, w' recomputed) and the number of lanes (parts of
  * the batch stepped concurrently on their own streams); any output may be NULL. */
 NLS_API int nls_batch_tile(nls_batch *b, int *cells_per_thread, int *pass_kind, int *lanes);
+/* Steps run so far by the persistent step kernel (nls_batch_params.persistent). */
+NLS_API int nls_batch_persistent_steps(nls_batch *b, int64_t *steps);
 /* Krylov monitor of trajectory t since creation: steps in which the weight of the last
  * Lanczos vector in f(T) e1, |y_{m-1}| / |y|, exceeded 1e-6 (the residual did not fall
  * within the Krylov dimension), the largest such weight, the largest loss of

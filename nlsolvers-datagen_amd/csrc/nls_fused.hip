@@ -380,29 +380,23 @@ __global__ __launch_bounds__(FBS) void k_kge_pass(KgeArgs a, int j) {
 // last_pass = m - 2: the reference's variant, alpha_{m-2} and b_{m-1} from the last
 // pass and T = tridiag(b, [alpha_0..alpha_{m-2}, 0], b) (lanczos.hpp:234-263, Q1).
 // last_pass = m - 1: full T, alpha_{m-1} from the extra pass (b_{m-1} was set by it).
-__global__ __launch_bounds__(kWave) void k_kge_small(KgeArgs a, int last_pass) {
-  const int t = a.tlist ? a.tlist[blockIdx.x / NSEQ] : (int)(blockIdx.x / NSEQ);
-  const int s = blockIdx.x % NSEQ;
+// One wavefront's small stage for (trajectory t, sequence s) with Z in LDS; no workgroup
+// barrier, so the persistent step kernel can run the two sequences on two waves.
+__device__ __forceinline__ void kge_small_wave(const KgeArgs &a, int t, int s, int last_pass,
+                                               double (*Z)[kMaxKrylov + 1]) {
   const TrajDev T = a.traj[t];
-  __shared__ SmallWS ws;
-  __shared__ double red[NSEQ * PN];
   const int m = a.m;
-  // reduce last-pass partials (one wave = whole block)
+  const int lane = threadIdx.x & (kWave - 1);
+  // reduce last-pass partials (the butterfly leaves the same sum in every lane)
+  double red[PN] = {0.0, 0.0, 0.0};
   {
-    const int lane = threadIdx.x;
-    double acc[PN] = {0.0, 0.0, 0.0};
     const double *p =
         a.part + ((size_t)(last_pass & 1) * a.total_chunks + T.chunk0) * (NSEQ * PN) + s * PN;
     for (int g = lane; g < T.nchunks; g += kWave)
-      for (int k = 0; k < PN; ++k) acc[k] += p[(size_t)g * (NSEQ * PN) + k];
-    for (int k = 0; k < PN; ++k) {
-      const double v = wave_sum(acc[k]);
-      if (lane == 0) red[k] = v;
-    }
+      for (int k = 0; k < PN; ++k) red[k] += p[(size_t)g * (NSEQ * PN) + k];
+    for (int k = 0; k < PN; ++k) red[k] = wave_sum(red[k]);
   }
-  __syncthreads();
   TrajScal &S = a.scal[t];
-  const int lane = threadIdx.x;
   if (a.reorth != 0 && m >= 3) {
     // orthogonality of the previous step's basis (combine partials, fixed order)
     double o = 0.0;
@@ -431,15 +425,22 @@ __global__ __launch_bounds__(kWave) void k_kge_small(KgeArgs a, int last_pass) {
   double dk = lane <= last_pass ? T.sd * alk + T.mu2d : 0.0;
   double ek = T.sd * bk;
   const double b0 = S.b0[s];
-  if (small_ql_wave(m, dk, ek, ws.Z)) {
+  if (small_ql_wave(m, dk, ek, Z)) {
     if (lane == 0) S.status = 1;
     return;
   }
   const int kind = s == 0 ? F_COSM1_SQRT : a.kind1;
   double y, yi;
-  small_apply_f_wave(m, dk, ws.Z, kind, T.tau, 0.0, b0, 1.0, 0.0, y, yi);
+  small_apply_f_wave(m, dk, Z, kind, T.tau, 0.0, b0, 1.0, 0.0, y, yi);
   if (lane < m) S.y[s][lane] = y;
   monitor_tail(S, s, m, y, 0.0, s == 0 ? b0 : 0.0);
+}
+
+__global__ __launch_bounds__(kWave) void k_kge_small(KgeArgs a, int last_pass) {
+  const int t = a.tlist ? a.tlist[blockIdx.x / NSEQ] : (int)(blockIdx.x / NSEQ);
+  const int s = blockIdx.x % NSEQ;
+  __shared__ SmallWS ws;
+  kge_small_wave(a, t, s, last_pass, ws.Z);
 }
 
 // ---------------------------------------------------------------- combine
@@ -1070,12 +1071,12 @@ template <class T> __device__ inline const T &at32(const T *base, uint32_t idx) 
 // UNI: every trajectory of the batch has uniform m and c (values from the descriptor).
 // LAST: pass j = m - 1 of the full-T Lanczos variant: v_{m-1} is formed and A applied for
 // the partials that give alpha_{m-1}; nothing is stored (the combine folds v_{m-1}).
-template <int STAGE, bool UNI = false, bool LAST = false>
-__global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax, int alt) {
+// The pass body for the chunk cm (trajectory, range) with the dynamic LDS window lds;
+// shared by the pass kernel and the persistent step kernel.
+template <int STAGE, bool UNI, bool LAST>
+__device__ __forceinline__ void march2d_body(const KgeArgs a, const int2 cm, int j, int emax,
+                                             int alt, float *lds) {
   static_assert(!(LAST && STAGE == 0), "the last pass follows pass 0");
-  extern __shared__ float lds[];
-  const int2 cm = a.cmap[blockIdx.x];
-  if (cm.x < 0) return;
   const TrajDev T = a.traj[cm.x];
   __shared__ double red[NSEQ * PN];
   __shared__ double shn[NSEQ * PN * (FBS / kWave)];
@@ -1324,6 +1325,14 @@ __global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax,
 #pragma unroll
     for (int k = 0; k < NSEQ * PN; ++k) dst[k] = acc[k];
   }
+}
+
+template <int STAGE, bool UNI = false, bool LAST = false>
+__global__ __launch_bounds__(FBS) void k_kge_march2d(KgeArgs a, int j, int emax, int alt) {
+  extern __shared__ float lds[];
+  const int2 cm = a.cmap[blockIdx.x];
+  if (cm.x < 0) return;
+  march2d_body<STAGE, UNI, LAST>(a, cm, j, emax, alt, lds);
 }
 
 // ---------------------------------------------------------------- 3D plane march, stored w'
@@ -1652,10 +1661,8 @@ __global__ __launch_bounds__(FBS, (CPT == 4 && UNI && STAGE == 2) ? 4 : 1) void 
 // coefficients: d = y0 x/b0 + sum_{k<m-2} y_k v_k + (y_{m-2} - y_{m-1} alpha/b) v_{m-2}
 // + (y_{m-1}/b) w'_{m-2}, so each stored vector is read exactly once.
 // ORTH: also the orthogonality partials <v_{m-2}, v_0> (reorthogonalisation modes 1, 2).
-template <int DIM, bool ORTH, bool UNI = false>
-__global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a) {
-  const int2 cm = a.cmap[blockIdx.x];
-  if (cm.x < 0) return;
+template <int DIM, bool ORTH, bool UNI>
+__device__ __forceinline__ void combine_rt_body(const KgeArgs a, const int2 cm) {
   const TrajDev T = a.traj[cm.x];
   const TrajScal &S = a.scal[cm.x];
   __shared__ double cv[NSEQ][kMaxKrylov]; // coefficient of V slot k-1 (k = 1..m-2); [0]: seed
@@ -1751,6 +1758,142 @@ __global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a) {
     block_sum_n<NSEQ>(o, sho);
     if (threadIdx.x == 0) store_orth(a, T, S, cm.y, o);
   }
+}
+
+template <int DIM, bool ORTH, bool UNI = false>
+__global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a) {
+  const int2 cm = a.cmap[blockIdx.x];
+  if (cm.x < 0) return;
+  combine_rt_body<DIM, ORTH, UNI>(a, cm);
+}
+
+// ---------------------------------------------------------------- persistent step kernel
+// One cooperative launch advances a 2D row-march batch (Gautschi / sEWI) by nsteps steps
+// that take no snapshot.  Workgroup b owns chunk cmap[b] in every phase of every step
+// (the grid is the chunk map; a cooperative launch keeps every workgroup resident).  Per
+// trajectory, an arrival counter orders the phases: pass j (j = 0 .. np - 1) of a chunk
+// starts once every chunk of its trajectory has arrived from the phase before; the chunk
+// that arrives last from the last pass runs the small stage (waves 0 and 1, one Krylov
+// sequence each, Z in the LDS of the pass window) and arrives once more, which releases
+// the combine; the combine's arrivals release pass 0 of the next step.  Per step the
+// counter rises by (np + 1) * nchunks + 1.  An arrival is an agent-scope release
+// read-modify-write after a workgroup barrier; a wait polls with relaxed loads and ends
+// with one agent-scope acquire fence.  Waits are bounded: one that gives up sets the
+// error flag, and every other wait then ends too, so every workgroup exits.
+constexpr int kCntStride = 64;          // ints between two trajectories' counters (256 B)
+constexpr int64_t kSpinMax = 1ll << 21; // poll rounds before a wait gives up (seconds)
+// Waves per SIMD the kernel is compiled for: every phase inlined needs 203-243 VGPRs
+// uncapped (2 waves, 512 resident workgroups); capped at 128 VGPRs (4 waves, 1024
+// workgroups) it spills 252-340 bytes per lane, nearly all outside the phases' loops.
+constexpr int kPersistWaves = 4;
+
+struct PersistArgs {
+  double *state[3]; // the FP64 state ring
+  int *cnt;         // [ntraj * kCntStride], zero at launch
+  int *err;         // set to 1 by a wait that gives up
+  int64_t step0;    // steps taken before this launch (position in the ring)
+  int nsteps;
+  int emax;         // LDS window cells of the pass
+  int last_pass;    // m - 2 (reference variant) or m - 1 (full T)
+};
+
+__device__ inline bool persist_wait(const int *cnt, int target, const PersistArgs &q, int *flag) {
+  if (threadIdx.x == 0) {
+    int ok = 0;
+    for (int64_t it = 0; it < kSpinMax; ++it) {
+      if (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) {
+        ok = 1;
+        break;
+      }
+      if (__hip_atomic_load(q.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+      __builtin_amdgcn_s_sleep(2);
+    }
+    if (ok) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    else __hip_atomic_store(q.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *flag = ok;
+  }
+  __syncthreads();
+  return *flag != 0;
+}
+
+// Arrival of a chunk at the end of a phase (after the workgroup barrier that completes
+// every store of the phase): agent-scope release, or acquire-release for the last pass,
+// whose last arrival also acquires the other chunks' partials.  Returns, in thread 0,
+// whether this was the last arrival of the phase (target: the count it completes).
+__device__ inline bool persist_arrive(int *cnt, bool acq, int target) {
+  __syncthreads();
+  bool last = false;
+  if (threadIdx.x == 0) {
+    const int old = acq ? __hip_atomic_fetch_add(cnt, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT)
+                        : __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    last = old == target - 1;
+  }
+  return last;
+}
+
+// An opaque copy of the (uniform) chunk descriptor: what a phase derives from it cannot be
+// hoisted out of the step loop, so the values of all phases are not live at once (203
+// against 223 VGPRs uncapped; under the 4-wave cap 252-340 against 336-424 bytes of spill).
+__device__ inline int2 opaque_chunk(int2 v) {
+  asm volatile("" : "+s"(v.x), "+s"(v.y));
+  return v;
+}
+
+template <bool UNI, bool ORTH>
+__global__ __launch_bounds__(FBS, kPersistWaves) void k_kge_persist2d(KgeArgs a0, PersistArgs q) {
+  extern __shared__ float lds[];
+  __shared__ int wait_ok, last_arriver;
+  const int2 cm = a0.cmap[blockIdx.x];
+  if (cm.x < 0) return;
+  const int C = a0.traj[cm.x].nchunks;
+  int *cnt = q.cnt + (size_t)cm.x * kCntStride;
+  const int np = q.last_pass + 1; // passes per step (at least 3: m >= 3)
+  const bool full = q.last_pass == a0.m - 1;
+  const int per_step = (np + 1) * C + 1;
+  for (int st = 0; st < q.nsteps; ++st) {
+    KgeArgs a = a0;
+    const int cur = (int)((q.step0 + st) % 3);
+    a.u = q.state[cur];
+    a.up = q.state[(cur + 2) % 3];
+    a.un = q.state[(cur + 1) % 3];
+    const int base = st * per_step;
+    // pass 0 on the seeds, pass 1, the general passes 2 .. np - 2
+    if (!persist_wait(cnt, base, q, &wait_ok)) return;
+    march2d_body<0, UNI, false>(a, opaque_chunk(cm), 0, q.emax, 1, lds);
+    persist_arrive(cnt, false, 0);
+    if (!persist_wait(cnt, base + C, q, &wait_ok)) return;
+    march2d_body<1, UNI, false>(a, opaque_chunk(cm), 1, q.emax, 1, lds);
+    persist_arrive(cnt, false, 0);
+    for (int j = 2; j < np - 1; ++j) {
+      if (!persist_wait(cnt, base + j * C, q, &wait_ok)) return;
+      march2d_body<2, UNI, false>(a, opaque_chunk(cm), j, q.emax, 1, lds);
+      persist_arrive(cnt, false, 0);
+    }
+    // the last pass: j = m - 1 of the full T (partials only) or j = m - 2
+    const int jl = np - 1;
+    if (!persist_wait(cnt, base + jl * C, q, &wait_ok)) return;
+    if (full) march2d_body<2, UNI, true>(a, opaque_chunk(cm), jl, q.emax, 1, lds);
+    else march2d_body<2, UNI, false>(a, opaque_chunk(cm), jl, q.emax, 1, lds);
+    const bool last = persist_arrive(cnt, true, base + np * C);
+    if (threadIdx.x == 0) last_arriver = last;
+    __syncthreads();
+    if (last_arriver) {
+      // small stage: wave w < NSEQ for sequence w, Z rows [kMaxKrylov w, kMaxKrylov (w + 1))
+      // in the pass window's LDS
+      const int w = threadIdx.x / kWave;
+      double(*Z)[kMaxKrylov + 1] = reinterpret_cast<double(*)[kMaxKrylov + 1]>(lds);
+      if (w < NSEQ) kge_small_wave(a, opaque_chunk(cm).x, w, q.last_pass, Z + w * kMaxKrylov);
+      persist_arrive(cnt, false, 0);
+    }
+    if (!persist_wait(cnt, base + np * C + 1, q, &wait_ok)) return;
+    combine_rt_body<2, ORTH, UNI>(a, opaque_chunk(cm));
+    persist_arrive(cnt, false, 0);
+  }
+}
+
+inline size_t persist2d_lds_bytes(int nmax, bool uni) {
+  return std::max(march2d_lds_bytes(nmax, uni),
+                  sizeof(double) * NSEQ * kMaxKrylov * (kMaxKrylov + 1));
 }
 
 // Snapshot combine over the workgroup's range: also the velocity (u+ - u)/tau before
@@ -3215,6 +3358,9 @@ struct nls_batch {
   std::vector<int> ev_launches; // general passes between the events of each recorded pair
   double pass_ms = 0.0;
   int pass_launches = 0;
+  // Persistent step kernel (p.persistent): arrival counters, error flag, steps it has run.
+  int *d_cnt = nullptr, *d_err = nullptr;
+  int64_t persist_steps = 0;
 
   template <class T> T *alloc(size_t count) {
     void *q = nullptr;
@@ -3231,7 +3377,7 @@ struct nls_batch {
     for (int k = 1; k < MAXL; ++k)
       if (sl[k]) (void)hipStreamSynchronize(sl[k]);
     void *ptrs[] = {d_traj, d_cmap, d_scal, d_part, d_mf, d_cf, d_V, d_W, d_X,
-                    d_state[0], d_state[1], d_state[2], d_snapU, d_snapV};
+                    d_state[0], d_state[1], d_state[2], d_snapU, d_snapV, d_cnt, d_err};
     for (void *q : ptrs)
       if (q) (void)hipFree(q);
     for (int k = 0; k < MAXL; ++k) {
@@ -3692,6 +3838,90 @@ struct nls_batch {
     else launch_step_t<3>(cur, snapshot, slot, prof);
   }
 
+  // ------------------------------------------------------------ persistent step kernel
+  // Workgroups of the persistent kernel resident on the whole chip at once.
+  int persist_slots() const {
+    int per_cu = 0, dev = 0, ncu = 0;
+    const size_t lds = persist2d_lds_bytes(nmax_of(), uni_all);
+    with_uni([&](auto u) {
+      constexpr bool U = decltype(u)::value;
+      if (p.reorth)
+        NLS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_kge_persist2d<U, true>, FBS, lds));
+      else
+        NLS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_kge_persist2d<U, false>, FBS, lds));
+    });
+    NLS_HIP(hipGetDevice(&dev));
+    NLS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    return per_cu * ncu;
+  }
+
+  // Tile of the persistent kernel: the given one, or the smallest whose chunk map fits
+  // the resident workgroups (one chunk per workgroup for the whole run).
+  void setup_persistent() {
+    NLS_REQUIRE(p.equation == NLS_EQ_KGE && !sv() && p.dim == 2,
+                "persistent: the persistent step kernel runs 2D Klein-Gordon Gautschi / sEWI batches");
+    NLS_REQUIRE(p.reorth != 1, "persistent: not with reorthogonalisation in every iteration (reorth 1)");
+    NLS_REQUIRE(p.pass_kernel == 0 || p.pass_kernel == 3,
+                "persistent: the persistent step kernel runs the row-march pass (pass_kernel 2)");
+    int dev = 0, coop = 0;
+    NLS_HIP(hipGetDevice(&dev));
+    NLS_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    NLS_REQUIRE(coop != 0, "persistent: the device has no cooperative launch");
+    NLS_REQUIRE(m >= (p.variant == 1 ? 3 : 4),
+                "persistent: Krylov dimension below 4 (reference variant) or 3 (full T)");
+    pass_kind = 2;
+    NLS_REQUIRE(kind_ok(2, 64), "persistent: a grid side exceeds what the row-march window holds");
+    lanes = std::min(p.lanes > 0 ? p.lanes : 2, max_lanes); // snapshot steps
+    const size_t slots = (size_t)persist_slots();
+    int c = p.cells_per_thread;
+    if (c > 0) {
+      NLS_REQUIRE(kind_ok(2, c), "persistent: cells_per_thread outside [4, 64]");
+      set_layout(c);
+    } else {
+      for (c = 4; c <= 64; ++c) {
+        set_layout(c);
+        if (lay.cmap.size() <= slots) break;
+      }
+    }
+    NLS_REQUIRE(lay.cmap.size() <= slots,
+                "persistent: " + std::to_string(lay.cmap.size()) + " chunks exceed the " +
+                    std::to_string(slots) + " workgroups resident at once (larger tile needed)");
+    if (!d_cnt) {
+      d_cnt = alloc<int>((size_t)kCntStride * ntraj);
+      d_err = alloc<int>(1);
+      NLS_HIP(hipMemsetAsync(d_err, 0, sizeof(int), s));
+    }
+  }
+
+  bool persist_live() const { return p.persistent && !reorth_live; }
+
+  // nsteps steps without snapshots in one cooperative launch.
+  void launch_persist(int nsteps) {
+    const size_t slots = (size_t)persist_slots();
+    NLS_REQUIRE(lay.cmap.size() <= slots, "persistent: the chunk map exceeds the resident workgroups");
+    KgeArgs a = args(0); // the kernel sets the ring pointers per step
+    PersistArgs q{};
+    for (int k = 0; k < 3; ++k) q.state[k] = d_state[k];
+    q.cnt = d_cnt;
+    q.err = d_err;
+    q.step0 = step;
+    q.nsteps = nsteps;
+    q.emax = MSUB + 2 * nmax;
+    q.last_pass = p.variant == 1 ? m - 1 : m - 2;
+    NLS_HIP(hipMemsetAsync(d_cnt, 0, sizeof(int) * kCntStride * ntraj, s));
+    void *kargs[] = {&a, &q};
+    const size_t lds = persist2d_lds_bytes(nmax, uni_all);
+    const dim3 grid((unsigned)lay.cmap.size()), blk(FBS);
+    with_uni([&](auto u) {
+      constexpr bool U = decltype(u)::value;
+      const void *fn = p.reorth ? (const void *)k_kge_persist2d<U, true>
+                                : (const void *)k_kge_persist2d<U, false>;
+      NLS_HIP(hipLaunchCooperativeKernel(fn, grid, blk, kargs, (unsigned)lds, s));
+    });
+    check_launch("persistent kge steps");
+    persist_steps += nsteps;
+  }
+
   // Tile autotuner: times one step per cells-per-thread candidate on the live
   // buffers (a step only writes the next-state buffer and Krylov scratch, so
   // timing runs leave the trajectory untouched) and keeps the fastest.
@@ -3815,6 +4045,12 @@ struct nls_batch {
   }
 
   void check_status() {
+    if (d_err) {
+      int err = 0;
+      NLS_HIP(hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, s));
+      NLS_HIP(hipStreamSynchronize(s));
+      NLS_REQUIRE(err == 0, "persistent step kernel: a per-trajectory wait gave up");
+    }
     std::vector<TrajScal> h(ntraj);
     NLS_HIP(hipMemcpyAsync(h.data(), d_scal, sizeof(TrajScal) * ntraj, hipMemcpyDeviceToHost, s));
     NLS_HIP(hipStreamSynchronize(s));
@@ -3832,6 +4068,12 @@ struct nls_batch {
       lanes = 1;
       set_layout(p.cells_per_thread > 0 ? p.cells_per_thread : 16);
       tuned = true;
+      recapture = false;
+    }
+    if (!tuned && p.persistent) {
+      setup_persistent();
+      tuned = true;
+      if (p.use_graph) capture_graphs(); // snapshot steps and the steps after reorth switches on
       recapture = false;
     }
     if (!tuned) {
@@ -3859,11 +4101,35 @@ struct nls_batch {
     ev_used = 0;
     ev_launches.clear();
     NLS_HIP(hipEventRecord(e0, s));
+    auto snap_at = [&](int64_t idx) {
+      return p.snapshot_every > 0 && idx % p.snapshot_every == 0 &&
+             idx / p.snapshot_every < p.max_snapshots;
+    };
     for (int k = 0; k < nsteps; ++k) {
       const int64_t idx = step + 1; // reference step number i (kg_driver_dev_2d.cpp:144)
       const int cur = (int)(step % nstates());
-      const bool snap = p.snapshot_every > 0 && idx % p.snapshot_every == 0 &&
-                        idx / p.snapshot_every < p.max_snapshots;
+      const bool snap = snap_at(idx);
+      if (!snap && !profiling && persist_live()) {
+        // the steps up to the next snapshot step, the end of the run or the next check of
+        // the orthogonality flag, in one persistent launch
+        int len = 1;
+        while (k + len < nsteps && !snap_at(idx + len) &&
+               (p.reorth != 2 || (step + len) % kReorthCheck != 0))
+          ++len;
+        launch_persist(len);
+        step += len;
+        k += len - 1;
+        if (p.reorth == 2 && !reorth_live && step % kReorthCheck == 0) {
+          int any = 0;
+          NLS_HIP(hipMemcpyAsync(&any, d_any, sizeof(int), hipMemcpyDeviceToHost, s));
+          NLS_HIP(hipStreamSynchronize(s));
+          if (any) {
+            reorth_live = true;
+            if (p.use_graph) capture_graphs();
+          }
+        }
+        continue;
+      }
       if (snap || !p.use_graph || profiling || !graphs[cur])
         launch_step(cur, snap, snap ? (int)(idx / p.snapshot_every) : 0, profiling);
       else
@@ -4258,6 +4524,13 @@ int nls_batch_tile(nls_batch *b, int *cells_per_thread, int *pass_kind, int *lan
     if (cells_per_thread) *cells_per_thread = b->cpt;
     if (pass_kind) *pass_kind = b->pass_kind;
     if (lanes) *lanes = b->lanes;
+  });
+}
+
+int nls_batch_persistent_steps(nls_batch *b, int64_t *steps) {
+  return guard([&] {
+    NLS_REQUIRE(b && steps, "nls_batch_persistent_steps: null argument");
+    *steps = b->persist_steps;
   });
 }
 

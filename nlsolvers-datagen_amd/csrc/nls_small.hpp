@@ -255,7 +255,12 @@ __device__ inline void small_apply_f_wave(int m, double lam, double (*Z)[kMaxKry
   const int lane = threadIdx.x & (kWave - 1);
   cplx f = {0.0, 0.0};
   if (lane < m) f = eval_f(kind, t_re, t_im, lam);
-  __syncthreads(); // rows of Z written by every lane are read below
+  // rows of Z written by every lane of this wavefront are read below: a wavefront-scope
+  // barrier (LDS operations of one wavefront complete in order), so a workgroup may run
+  // this stage on some of its waves only
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   double ar = 0.0, ai = 0.0;
   for (int i = 0; i < m; ++i) {
     const double fr = lane_read(f.re, i), fi = lane_read(f.im, i);

@@ -44,7 +44,7 @@ class BatchParams(C.Structure):
                 ("max_snapshots", C.c_int), ("cells_per_thread", C.c_int),
                 ("pass_kernel", C.c_int), ("lanes", C.c_int), ("reorth", C.c_int),
                 ("reorth_tol", C.c_double), ("always_load_fields", C.c_int),
-                ("variant", C.c_int)]
+                ("variant", C.c_int), ("persistent", C.c_int)]
 
 
 class ServerParams(C.Structure):
@@ -169,6 +169,7 @@ _SIGS = {
     "nls_batch_get_state": [_P, C.c_int, _D, _D],
     "nls_batch_info": [_P, C.POINTER(C.c_int), _I64, C.POINTER(C.c_size_t)],
     "nls_batch_set_profiling": [_P, C.c_int],
+    "nls_batch_persistent_steps": [_P, _I64],
     "nls_batch_monitor": [_P, C.c_int, C.POINTER(C.c_int), _D, _D, C.POINTER(C.c_int)],
     "nls_ic_draw": [C.c_int, C.c_uint64, C.c_int, C.c_double, _D, C.c_int, C.POINTER(C.c_int)],
     "nls_field_sample": [C.c_char_p, C.c_char_p, C.c_int, C.c_double, C.c_uint64, C.c_int,

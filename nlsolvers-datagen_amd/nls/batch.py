@@ -37,7 +37,7 @@ class FusedBatch:
                  mu2=0.0, q6=True, gamma=0.0, use_graph=True, snapshot_every=0,
                  max_snapshots=0, sigmas=None, seeds=None, cells_per_thread=0,
                  pass_kernel=None, lanes=0, reorth=0, reorth_tol=0.0,
-                 always_load_fields=False, variant="reference"):
+                 always_load_fields=False, variant="reference", persistent=False):
         eq = {"kge": L.EQ_KGE, "nlse": L.EQ_NLSE, "sp4": L.EQ_SP4}[equation]
         meth = {"gautschi": L.KGE_GAUTSCHI, "sewi": L.KGE_SEWI, "sv": L.KGE_SV,
                 "strang": L.NLSE_STRANG, "reference": L.SP4_REFERENCE,
@@ -53,7 +53,8 @@ class FusedBatch:
                                snapshot_every, max_snapshots, cells_per_thread,
                                0 if pass_kernel is None else pass_kernel + 1, lanes,
                                {"never": 0, "always": 1, "auto": 2}.get(reorth, reorth),
-                               reorth_tol, int(always_load_fields), lanczos_variant(variant))
+                               reorth_tol, int(always_load_fields), lanczos_variant(variant),
+                               int(persistent))
         self.td = (L.TrajDesc * k)(*[L.TrajDesc(n, l, s, sd)
                                      for n, l, s, sd in zip(self.ns, Ls, sigmas, seeds)])
         self.h = C.c_void_p()
@@ -136,6 +137,12 @@ class FusedBatch:
         lanes = C.c_int()
         call("nls_batch_tile", self.h, C.byref(cpt), C.byref(kind), C.byref(lanes))
         return {"cells_per_thread": cpt.value, "pass_kind": kind.value, "lanes": lanes.value}
+
+    def persistent_steps(self):
+        """Steps run so far by the persistent step kernel (``persistent=True``)."""
+        n = C.c_int64()
+        call("nls_batch_persistent_steps", self.h, C.byref(n))
+        return n.value
 
     def monitor(self, t):
         """Krylov monitor of trajectory t: steps whose last-vector weight |y_{m-1}|/|y|
