@@ -219,6 +219,19 @@ __device__ inline float seed_x(int s, const KgeArgs &a, int64_t g) {
   return (float)(-(double)a.mf[g] * uu * uu * uu);
 }
 
+// ---------------------------------------------------------------- LDS-only barrier
+// A workgroup barrier that orders LDS accesses only.  __syncthreads() also waits for every
+// outstanding global memory operation of the wave (s_waitcnt vmcnt(0)), so in a march the
+// previous sub-chunk's stores and the prefetched loads would complete before each barrier.
+// The passes exchange nothing through global memory inside a launch (each workgroup reads
+// what earlier launches wrote and stores its own cells), so their barriers need only LDS
+// ordering (s_waitcnt lgkmcnt(0); s_barrier) and the stores drain while the march goes on.
+__device__ inline void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // ---------------------------------------------------------------- per-trajectory reduction
 // Sums the partials of this trajectory's chunks (fixed order) into red[0..NSEQ*PN).
 __device__ inline void reduce_partials(const KgeArgs &a, const TrajDev &T, int parity,
@@ -239,7 +252,7 @@ __device__ inline void reduce_partials(const KgeArgs &a, const TrajDev &T, int p
       if (lane == 0) red[k] = v;
     }
   }
-  __syncthreads();
+  lds_barrier();
 }
 
 struct Coef {
@@ -624,11 +637,11 @@ __device__ inline void block_sum_n(double (&v)[NV], double *sh /* NV * (FBS/64) 
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
 #pragma unroll
   for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
-  __syncthreads();
+  lds_barrier();
   if (lane == 0)
 #pragma unroll
     for (int k = 0; k < NV; ++k) sh[k * (FBS / kWave) + w] = v[k];
-  __syncthreads();
+  lds_barrier();
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     double t = sh[k * (FBS / kWave)];
@@ -1295,13 +1308,13 @@ __device__ __forceinline__ void march2d_body(const KgeArgs a, const int2 cm, int
         fetch(clampi(i), nc[q], nx0[q], nx1[q], np0[q], np1[q], nu[q], nm[q]);
       }
     }
-    __syncthreads();
+    lds_barrier();
     if (!more) {
       compute(std::false_type{}, cb, len);
       break;
     }
     compute(std::true_type{}, cb, cb + MSUB);
-    __syncthreads(); // every read of this window is done
+    lds_barrier(); // every read of this window is done
     // keep [cb' - n, cb' + n) (disjoint source and destination: 2n <= SUB)
     for (int e = threadIdx.x; e < 2 * n; e += FBS) {
       if (!UNI) sc[e] = sc[MSUB + e];
@@ -1312,7 +1325,7 @@ __device__ __forceinline__ void march2d_body(const KgeArgs a, const int2 cm, int
         sp1[e] = sp1[MSUB + e];
       }
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int q = 0; q < MSPT; ++q) {
       if (!nok[q]) nc[q] = nx0[q] = nx1[q] = np0[q] = np1[q] = nm[q] = 0.0f, nu[q] = 0.0;
@@ -1561,7 +1574,7 @@ __global__ __launch_bounds__(FBS, (CPT == 4 && UNI && STAGE == 2) ? 4 : 1) void 
   // time, so one of the two reads is a cache hit.  They are deposited one plane ahead into
   // the other window, which plane z + 1 then completes with its own cells.
   for (int z = z0; z < z1; ++z) {
-    __syncthreads(); // plane z - 1's window reads are done: its window takes plane z + 1
+    lds_barrier(); // plane z - 1's window reads are done: its window takes plane z + 1
     float *sc, *sv0, *sv1;
     win(z, sc, sv0, sv1);
     // window of plane z: own cells from registers (its ends are in place)
@@ -1590,7 +1603,7 @@ __global__ __launch_bounds__(FBS, (CPT == 4 && UNI && STAGE == 2) ? 4 : 1) void 
       if (z + 2 < n) fetch_own(z + 2);
       if (z + 2 < z1) fetch_halo(z + 2);
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int q = 0; q < CPT; ++q) {
       if (!own[q]) continue;
@@ -2802,7 +2815,7 @@ __global__ __launch_bounds__(FBS) void k_nlse_march2d(NlseArgs a, int j, int ema
         if (i >= 0 && i < N) fetch(i, nc[q], nx[q], np[q]);
       }
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int kk = 0; kk < MSPT; ++kk) {
       const int l = cb + threadIdx.x + kk * FBS;
@@ -2830,7 +2843,7 @@ __global__ __launch_bounds__(FBS) void k_nlse_march2d(NlseArgs a, int j, int ema
       }
     }
     if (!more) break;
-    __syncthreads(); // every read of this window is done
+    lds_barrier(); // every read of this window is done
     for (int e = threadIdx.x; e < 2 * n; e += FBS) { // keep [cb' - n, cb' + n); 2n <= SUB
       if (!UNI) sc[e] = sc[MSUB + e];
       sv0[e] = sv0[MSUB + e];
@@ -2840,7 +2853,7 @@ __global__ __launch_bounds__(FBS) void k_nlse_march2d(NlseArgs a, int j, int ema
         sp1[e] = sp1[MSUB + e];
       }
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int q = 0; q < MSPT; ++q) {
       const int i = cell(cb + MSUB + n + threadIdx.x + q * FBS);
@@ -3030,7 +3043,7 @@ __global__ __launch_bounds__(FBS) void k_nlse_march3d(NlseArgs a, int j, int ema
   }
   // window ends loaded with the own cells two planes ahead, deposited one plane ahead
   for (int z = z0; z < z1; ++z) {
-    __syncthreads(); // plane z - 1's window reads are done: its window takes plane z + 1
+    lds_barrier(); // plane z - 1's window reads are done: its window takes plane z + 1
     float *sc, *sv0, *sv1;
     win(z, sc, sv0, sv1);
 #pragma unroll
@@ -3055,7 +3068,7 @@ __global__ __launch_bounds__(FBS) void k_nlse_march3d(NlseArgs a, int j, int ema
       if (z + 2 < n) fetch_own(z + 2);
       if (z + 2 < z1) fetch_halo(z + 2);
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int q = 0; q < CPT; ++q) {
       if (!own[q]) continue;
