@@ -1371,12 +1371,14 @@ inline size_t march3d_lds_bytes(int cpt, int nmax, bool uni = false) {
   return sizeof(float) * (uni ? 2 : 3) * (size_t)(FBS * cpt + 2 * nmax);
 }
 
-// The tile-4 uniform-field stage-2 pass is held to 128 VGPRs (4 waves per SIMD instead of 3,
-// at the cost of a 12-byte spill): 3.64 -> 3.09-3.31 ms per pass on 40 x 220^3.  Stage 1 under
-// the same cap spills 36 bytes and slows from 3.68 to 4.82 ms, so it is left uncapped.
+// Register caps, measured on one box on 40 x 220^3 (tile 4, uniform fields): the general
+// stage-2 pass runs uncapped (132 VGPRs, 3 waves per SIMD, no spill): 2.44 ms, against 2.90 ms
+// held to 4 waves (128 VGPRs, a 12-byte spill) and 4.54 ms held to 5 (a 120-byte spill).  The
+// last pass of the full-T variant is held to 4 waves: 1.46 ms, against 2.25 ms uncapped at
+// 126 VGPRs (the same occupancy, scheduled for fewer waves).
 // LAST: pass j = m - 1 of the full-T Lanczos variant (partials only, nothing stored).
 template <int STAGE, int CPT, bool UNI = false, bool LAST = false>
-__global__ __launch_bounds__(FBS, (CPT == 4 && UNI && STAGE == 2) ? 4 : 1) void k_kge_march3d(KgeArgs a, int j, int emax) {
+__global__ __launch_bounds__(FBS, (CPT == 4 && UNI && STAGE == 2 && LAST) ? 4 : 1) void k_kge_march3d(KgeArgs a, int j, int emax) {
   static_assert(!(LAST && STAGE == 0), "the last pass follows pass 0");
   extern __shared__ float lds[];
   const int2 cm = a.cmap[blockIdx.x];
