@@ -131,7 +131,8 @@ NLS_API int nls_matfunc_last_krylov(nls_matfunc *h, double *T_host, double *beta
 
 /* FP64 GPU small stage on host data: eigen-decomposition of the symmetric
  * tridiagonal (d[m], e[m-1]) and y = beta0 Q f(L) Q^T e1 (yr, yi host arrays of m).
- * kind + 0x100 (real kinds) runs the fused batch's single-wavefront QL instead.
+ * kind + 0x100 (real kinds) runs the fused batch's single-wavefront QL instead;
+ * kind + 0x200 (kinds 1, 3, 4 on a T with no negative eigenvalue) its power-series route.
  * Replaces cusolverDnDsyevd + f(lambda) + GEMMs (matfunc_real.hpp:252-292,385-392). */
 NLS_API int nls_small_matfunc(int m, const double *d, const double *e, int kind, double t_re,
                               double t_im, double beta0, double *yr, double *yi);

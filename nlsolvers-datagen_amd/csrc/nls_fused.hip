@@ -438,13 +438,23 @@ __device__ __forceinline__ void kge_small_wave(const KgeArgs &a, int t, int s, i
   double dk = lane <= last_pass ? T.sd * alk + T.mu2d : 0.0;
   double ek = T.sd * bk;
   const double b0 = S.b0[s];
-  if (small_ql_wave(m, dk, ek, Z)) {
-    if (lane == 0) S.status = 1;
-    return;
-  }
   const int kind = s == 0 ? F_COSM1_SQRT : a.kind1;
-  double y, yi;
-  small_apply_f_wave(m, dk, Z, kind, T.tau, 0.0, b0, 1.0, 0.0, y, yi);
+  double y;
+  // Full T with no negative eigenvalue (the shifted Laplacian's): f is entire in lambda
+  // and f(|lambda|) = f(lambda), so the power series gives y without the QL (7 terms at
+  // tau^2 ||T|| = 0.05) when tau^2 ||T||_inf <= 4.  The reference variant's Q1 matrix can
+  // have a negative eigenvalue, where the reference takes f at |lambda|: the QL route.
+  if (full && T.tau * T.tau * tridiag_norm_inf_wave(m, dk, ek) <= 4.0 &&
+      tridiag_nonneg_wave(m, dk, ek)) {
+    y = small_kge_series_wave(m, dk, ek, kind, T.tau, b0);
+  } else {
+    if (small_ql_wave(m, dk, ek, Z)) {
+      if (lane == 0) S.status = 1;
+      return;
+    }
+    double yi;
+    small_apply_f_wave(m, dk, Z, kind, T.tau, 0.0, b0, 1.0, 0.0, y, yi);
+  }
   if (lane < m) S.y[s][lane] = y;
   monitor_tail(S, s, m, y, 0.0, s == 0 ? b0 : 0.0);
 }

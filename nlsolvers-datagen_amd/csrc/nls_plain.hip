@@ -335,6 +335,17 @@ __global__ __launch_bounds__(kWave) void k_small_de(int m, const double *__restr
     }
     return;
   }
+  if (kind & F_KGE_SERIES) { // the fused batch's power series (T with no negative eigenvalue)
+    const int lane = threadIdx.x;
+    const double y = small_kge_series_wave(m, lane < m ? d[lane] : 0.0,
+                                           lane < m - 1 ? e[lane] : 0.0, kind & ~F_KGE_SERIES,
+                                           t_re, beta0);
+    if (lane < m) {
+      yr[lane] = y;
+      yi[lane] = 0.0;
+    }
+    return;
+  }
   if (kind & F_WAVE_QL) { // the fused batch's register-resident QL (real symmetric T)
     const int lane = threadIdx.x;
     double dk = lane < m ? d[lane] : 0.0, ek = lane < m - 1 ? e[lane] : 0.0;

@@ -33,6 +33,9 @@ enum FuncKind : int {
 // Added to a real kind: the register-resident single-wavefront QL of the fused batch
 // (small_ql_wave + small_apply_f_wave) instead of the workgroup version.
 constexpr int F_WAVE_QL = 0x100;
+// Added to F_COSM1_SQRT, F_SINC2_SQRT or F_SINC2_HALF: the fused batch's power-series
+// route (small_kge_series_wave), for a T with no negative eigenvalue.
+constexpr int F_KGE_SERIES = 0x200;
 
 struct cplx {
   double re, im;
@@ -272,6 +275,72 @@ __device__ inline void small_apply_f_wave(int m, double lam, double (*Z)[kMaxKry
   }
   yr = beta0 * (phr * ar - phi * ai);
   yi = beta0 * (phr * ai + phi * ar);
+}
+
+// Largest row sum of |T| for the wavefront layout of small_ql_wave (lane k: d_k, e_k =
+// T[k][k+1]; lanes >= m ignored), in every lane.
+__device__ inline double tridiag_norm_inf_wave(int m, double dk, double ek) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const double e = lane < m - 1 ? ek : 0.0;
+  const double el = __shfl_up(e, 1, kWave);
+  double nu = lane < m ? fabs(dk) + fabs(lane > 0 ? el : 0.0) + fabs(e) : 0.0;
+  for (int off = 32; off >= 1; off >>= 1) nu = fmax(nu, __shfl_xor(nu, off, kWave));
+  return nu;
+}
+
+// Whether the symmetric tridiagonal T (wavefront layout of small_ql_wave) has no negative
+// eigenvalue, from the signs of the pivots of T = L D L^T (Sylvester's law of inertia: the
+// number of negative pivots is the number of negative eigenvalues).  A zero pivot answers
+// false.  Uniform across the wavefront.
+__device__ inline bool tridiag_nonneg_wave(int m, double dk, double ek) {
+  double p = lane_read(dk, 0);
+  bool ok = p > 0.0;
+  for (int k = 1; k < m && ok; ++k) {
+    const double e = lane_read(ek, k - 1);
+    p = lane_read(dk, k) - e * e / p;
+    ok = p > 0.0;
+  }
+  return ok;
+}
+
+// y = beta0 f(T) e1 for the Klein-Gordon kinds F_COSM1_SQRT, F_SINC2_SQRT and
+// F_SINC2_HALF on a symmetric tridiagonal T with nonnegative spectrum (the full
+// Lanczos T of a shifted Laplacian), lane k < m receiving y_k.  These f are entire
+// functions of lambda (power series in t^2 lambda), so on such a T the series is the
+// same function of T the eigen-decomposition gives:
+//   cos(t sqrt l) - 1    = sum_{k>=1} (-t^2 l)^k / (2k)!
+//   (sin x / x)^2, x^2 = t^2 l = sum_{k>=0} (-1)^k 2^(2k+1) (t^2 l)^k / (2k+2)!
+// (F_SINC2_HALF: t/2 in place of t).  Term k is ratio_k * T * term_{k-1}, summed until a
+// term drops below 1e-17 of the sum (at most 60 terms).  Meant for t^2 ||T||_inf <= 4,
+// where no term exceeds the sum by more than a factor of a few.
+__device__ inline double small_kge_series_wave(int m, double dk, double ek, int kind, double t,
+                                               double beta0) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const bool in = lane < m;
+  const double d = in ? dk : 0.0;
+  const double e = lane < m - 1 ? ek : 0.0;  // T[lane][lane+1]
+  const double el = __shfl_up(e, 1, kWave);  // T[lane-1][lane]
+  const double eu = lane > 0 ? el : 0.0;
+  const double tt = kind == F_SINC2_HALF ? 0.25 * t * t : t * t;
+  const bool cosm1 = kind == F_COSM1_SQRT;
+  double term = lane == 0 ? 1.0 : 0.0; // e1
+  double acc = cosm1 ? 0.0 : term;
+  for (int k = 1; k <= 60; ++k) {
+    const double up = __shfl_down(term, 1, kWave), dn = __shfl_up(term, 1, kWave);
+    double x = d * term + e * (lane + 1 < m ? up : 0.0);
+    if (lane > 0) x += eu * dn;
+    const double ratio = cosm1 ? -tt / ((2.0 * k) * (2.0 * k - 1.0))
+                               : -4.0 * tt / ((2.0 * k + 2.0) * (2.0 * k + 1.0));
+    term = in ? ratio * x : 0.0;
+    acc += term;
+    double tn = fabs(term), an = fabs(acc);
+    for (int off = 32; off >= 1; off >>= 1) {
+      tn = fmax(tn, __shfl_xor(tn, off, kWave));
+      an = fmax(an, __shfl_xor(an, off, kWave));
+    }
+    if (tn <= 1e-17 * an) break;
+  }
+  return beta0 * acc;
 }
 
 // y = beta0 exp(t T) e1 (t = t_re + i t_im) for the symmetric tridiagonal T of the

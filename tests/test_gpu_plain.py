@@ -83,6 +83,32 @@ def test_small_matfunc_gpu_vs_oracle(orc):
             assert np.abs(y - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), (m, dt)
 
 
+def test_small_kge_series_vs_oracle_ql(orc):
+    """The fused batch's power-series small stage (full Lanczos T of a shifted Laplacian:
+    no negative eigenvalue) against the oracle's QL evaluation of the same f, for the three
+    Klein-Gordon kinds, m = 2..32, t^2 ||T||_inf from 1e-3 up to the route's limit of 4."""
+    from nls.solvers import small_matfunc
+    rng = np.random.default_rng(17)
+    for m in (2, 3, 5, 8, 10, 12, 16, 24, 32):
+        # SPD tridiagonal with the structure of a Lanczos T of sd * B + mu2 (B in [0, 8])
+        d = 3.0 + 2.0 * rng.random(m)
+        e = 0.5 + 1.5 * rng.random(m - 1)
+        T = np.diag(d) + np.diag(e, -1) + np.diag(e, 1)
+        lam = np.linalg.eigvalsh(T)
+        shift = max(0.0, 1e-3 - lam.min())
+        d = d + shift
+        T = np.diag(d) + np.diag(e, -1)
+        Tcm = np.ascontiguousarray(T.T)
+        norm = np.abs(np.diag(d) + np.diag(e, -1) + np.diag(e, 1)).sum(axis=1).max()
+        for target in (1e-3, 0.05, 0.5, 4.0):
+            t = np.sqrt(target / norm)
+            for kind in (orc.F_COSM1_SQRT, orc.F_SINC2_SQRT, orc.F_SINC2_HALF):
+                ref = orc.small_matfunc_real(Tcm, kind, t)
+                y = small_matfunc(d, e, kind + 0x200, t).real
+                err = np.abs(y - ref).max() / max(np.abs(ref).max(), 1e-300)
+                assert err <= 1e-13, (m, target, kind, err)
+
+
 def test_matfunc_real_plain_vs_oracle(orc):
     from nls.solvers import MatrixFunctionApplicatorReal
     n = 64
