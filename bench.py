@@ -722,7 +722,16 @@ def main():
                      "stream_copy_gbs": stream_gbs,
                      "step_bytes_per_cell": step_bytes,
                      "step_effective_gbs": step_gbs,
-                     "step_frac": step_gbs / HBM_PEAK_GBS},
+                     "step_frac": step_gbs / HBM_PEAK_GBS,
+                     # the compulsory bytes of a KGE Gautschi step by 3-term matrix-free
+                     # Lanczos with FP32 vectors, state and fields (p = q = 4): two
+                     # matrix-function applications of 4mp + (m-1)q and 5p + q pointwise,
+                     # 40m + 16 B per cell-step
+                     "step_bytes_per_cell_fp32_lanczos": 40 * m + 16,
+                     "step_effective_gbs_fp32_lanczos": (40 * m + 16) * cells
+                                                        / (ms / args.steps / 1e3) / 1e9,
+                     "step_frac_fp32_lanczos": (40 * m + 16) * cells
+                                               / (ms / args.steps / 1e3) / 1e9 / HBM_PEAK_GBS},
         "plain": {"value": plain_rate, "unit": "cell-steps/s",
                   "path": "FP64 CSR SpMV + reference-granularity Lanczos (CGS), one trajectory "
                           "at a time", "steps": plain_steps},
