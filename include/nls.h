@@ -395,6 +395,13 @@ NLS_API int nls_batch_pass_timing(nls_batch *b, double *avg_ms, int *launches,
 NLS_API int nls_batch_tile(nls_batch *b, int *cells_per_thread, int *pass_kind, int *lanes);
 /* Steps run so far by the persistent step kernel (nls_batch_params.persistent). */
 NLS_API int nls_batch_persistent_steps(nls_batch *b, int64_t *steps);
+/* Ends the persistent launch in progress, or the next one, from any host thread: every
+ * per-trajectory wait of the kernel reads this flag (pinned host memory) while it polls,
+ * so every workgroup exits, and that nls_batch_run fails with "aborted by the host".
+ * The flag is cleared when the run returns.  The batch then holds a partly advanced
+ * step; destroy it.  Captured (non-persistent) steps are not affected.  No reference
+ * counterpart (the reference runs one process per trajectory and stops it). */
+NLS_API int nls_batch_abort(nls_batch *b);
 /* Krylov monitor of trajectory t since creation: steps in which the weight of the last
  * Lanczos vector in f(T) e1, |y_{m-1}| / |y|, exceeded 1e-6 (the residual did not fall
  * within the Krylov dimension), the largest such weight, the largest loss of

@@ -1815,26 +1815,49 @@ constexpr int kPersistWaves = 4;
 struct PersistArgs {
   double *state[3]; // the FP64 state ring
   int *cnt;         // [ntraj * kCntStride], zero at launch
-  int *err;         // set to 1 by a wait that gives up
+  int *err;         // 1: a wait gave up; 2: the host set the abort flag
+  const int *abort; // pinned host memory: nonzero ends every wait
   int64_t step0;    // steps taken before this launch (position in the ring)
   int nsteps;
   int emax;         // LDS window cells of the pass
   int last_pass;    // m - 2 (reference variant) or m - 1 (full T)
 };
 
-__device__ inline bool persist_wait(const int *cnt, int target, const PersistArgs &q, int *flag) {
+// step_start: also read the error and abort flags before polling (once per step), so an
+// abort ends every workgroup at its next step even where no counter falls short.
+__device__ inline bool persist_wait(const int *cnt, int target, const PersistArgs &q, int *flag,
+                                    bool step_start = false) {
   if (threadIdx.x == 0) {
-    int ok = 0;
-    for (int64_t it = 0; it < kSpinMax; ++it) {
+    // code: 1 a wait gave up, 2 the host set the abort flag, 0 the error flag is already set
+    int ok = 0, code = 1;
+    bool stop = false;
+    if (step_start) {
+      if (__hip_atomic_load(q.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        stop = true;
+        code = 0;
+      } else if (__hip_atomic_load(q.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) {
+        stop = true;
+        code = 2;
+      }
+    }
+    for (int64_t it = 0; it < kSpinMax && !stop; ++it) {
       if (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) {
         ok = 1;
         break;
       }
-      if (__hip_atomic_load(q.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+      // the flags are read only once the counter has not reached the target
+      if (__hip_atomic_load(q.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        code = 0;
+        break;
+      }
+      if (__hip_atomic_load(q.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) {
+        code = 2;
+        break;
+      }
       __builtin_amdgcn_s_sleep(2);
     }
     if (ok) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    else __hip_atomic_store(q.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else if (code != 0) __hip_atomic_store(q.err, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     *flag = ok;
   }
   __syncthreads();
@@ -1883,7 +1906,7 @@ __global__ __launch_bounds__(FBS, kPersistWaves) void k_kge_persist2d(KgeArgs a0
     a.un = q.state[(cur + 1) % 3];
     const int base = st * per_step;
     // pass 0 on the seeds, pass 1, the general passes 2 .. np - 2
-    if (!persist_wait(cnt, base, q, &wait_ok)) return;
+    if (!persist_wait(cnt, base, q, &wait_ok, true)) return;
     march2d_body<0, UNI, false>(a, opaque_chunk(cm), 0, q.emax, 1, lds);
     persist_arrive(cnt, false, 0);
     if (!persist_wait(cnt, base + C, q, &wait_ok)) return;
@@ -3383,8 +3406,10 @@ struct nls_batch {
   std::vector<int> ev_launches; // general passes between the events of each recorded pair
   double pass_ms = 0.0;
   int pass_launches = 0;
-  // Persistent step kernel (p.persistent): arrival counters, error flag, steps it has run.
+  // Persistent step kernel (p.persistent): arrival counters, error flag, the host-set abort
+  // flag (pinned host memory), steps it has run.
   int *d_cnt = nullptr, *d_err = nullptr;
+  int *h_abort = nullptr;
   int64_t persist_steps = 0;
 
   template <class T> T *alloc(size_t count) {
@@ -3412,6 +3437,7 @@ struct nls_batch {
       if (k > 0 && sl[k]) (void)hipStreamDestroy(sl[k]);
     }
     if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (h_abort) (void)hipHostFree(h_abort);
     if (s) (void)hipStreamDestroy(s);
   }
 
@@ -3916,6 +3942,11 @@ struct nls_batch {
       d_err = alloc<int>(1);
       NLS_HIP(hipMemsetAsync(d_err, 0, sizeof(int), s));
     }
+    if (!h_abort) {
+      NLS_HIP(hipHostMalloc((void **)&h_abort, sizeof(int),
+                            hipHostMallocCoherent | hipHostMallocMapped));
+      __atomic_store_n(h_abort, 0, __ATOMIC_RELEASE);
+    }
   }
 
   bool persist_live() const { return p.persistent && !reorth_live; }
@@ -3929,6 +3960,9 @@ struct nls_batch {
     for (int k = 0; k < 3; ++k) q.state[k] = d_state[k];
     q.cnt = d_cnt;
     q.err = d_err;
+    int *abort_dev = nullptr; // the device's address of the pinned flag
+    NLS_HIP(hipHostGetDevicePointer((void **)&abort_dev, h_abort, 0));
+    q.abort = abort_dev;
     q.step0 = step;
     q.nsteps = nsteps;
     q.emax = MSUB + 2 * nmax;
@@ -4074,6 +4108,12 @@ struct nls_batch {
       int err = 0;
       NLS_HIP(hipMemcpyAsync(&err, d_err, sizeof(int), hipMemcpyDeviceToHost, s));
       NLS_HIP(hipStreamSynchronize(s));
+      if (err != 0) { // the batch's state is that of a partly advanced step
+        NLS_HIP(hipMemsetAsync(d_err, 0, sizeof(int), s));
+        NLS_HIP(hipStreamSynchronize(s));
+        if (h_abort) __atomic_store_n(h_abort, 0, __ATOMIC_RELEASE);
+      }
+      NLS_REQUIRE(err != 2, "persistent step kernel: aborted by the host (nls_batch_abort)");
       NLS_REQUIRE(err == 0, "persistent step kernel: a per-trajectory wait gave up");
     }
     std::vector<TrajScal> h(ntraj);
@@ -4549,6 +4589,16 @@ int nls_batch_tile(nls_batch *b, int *cells_per_thread, int *pass_kind, int *lan
     if (cells_per_thread) *cells_per_thread = b->cpt;
     if (pass_kind) *pass_kind = b->pass_kind;
     if (lanes) *lanes = b->lanes;
+  });
+}
+
+int nls_batch_abort(nls_batch *b) {
+  return guard([&] {
+    NLS_REQUIRE(b, "nls_batch_abort: null batch");
+    if (!b->h_abort) // before the first run: the flag the next persistent launch reads
+      NLS_HIP(hipHostMalloc((void **)&b->h_abort, sizeof(int),
+                            hipHostMallocCoherent | hipHostMallocMapped));
+    __atomic_store_n(b->h_abort, 1, __ATOMIC_RELEASE);
   });
 }
 

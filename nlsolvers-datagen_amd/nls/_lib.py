@@ -170,6 +170,7 @@ _SIGS = {
     "nls_batch_info": [_P, C.POINTER(C.c_int), _I64, C.POINTER(C.c_size_t)],
     "nls_batch_set_profiling": [_P, C.c_int],
     "nls_batch_persistent_steps": [_P, _I64],
+    "nls_batch_abort": [_P],
     "nls_batch_monitor": [_P, C.c_int, C.POINTER(C.c_int), _D, _D, C.POINTER(C.c_int)],
     "nls_ic_draw": [C.c_int, C.c_uint64, C.c_int, C.c_double, _D, C.c_int, C.POINTER(C.c_int)],
     "nls_field_sample": [C.c_char_p, C.c_char_p, C.c_int, C.c_double, C.c_uint64, C.c_int,

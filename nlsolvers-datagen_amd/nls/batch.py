@@ -144,6 +144,11 @@ class FusedBatch:
         call("nls_batch_persistent_steps", self.h, C.byref(n))
         return n.value
 
+    def abort(self):
+        """Ends the persistent launch in progress (or the next one): that run raises
+        NLSError; the batch is then left mid-step and should be closed."""
+        call("nls_batch_abort", self.h)
+
     def monitor(self, t):
         """Krylov monitor of trajectory t: steps whose last-vector weight |y_{m-1}|/|y|
         exceeded 1e-6, the largest weight seen, the largest loss of orthogonality

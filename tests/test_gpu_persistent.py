@@ -127,3 +127,26 @@ def test_persistent_refuses_unsupported_batches(kw):
         b.set_initial(0, np.ones(shape), np.ones(shape), np.zeros(shape), np.zeros(shape))
     with pytest.raises(NLSError, match="persistent"):
         b.run(2)
+
+
+def test_persistent_abort_flag_ends_the_launch():
+    """The host sets the abort flag before a run: every wait of the persistent kernel that
+    finds its counter short reads the flag and gives up, every workgroup exits, and the run
+    fails with a message.  A new persistent batch afterwards runs normally (bitwise the
+    captured steps)."""
+    from nls._lib import NLSError
+    ns_list = [64, 96]
+    cases = [make_case(n, 8, t) for t, n in enumerate(ns_list)]
+    from nls.batch import FusedBatch
+    b = FusedBatch(ns_list, 10.0, 0.005, krylov_dim=8, mu2=1.0, cells_per_thread=16,
+                   pass_kernel=2, variant="full", persistent=True)
+    for t, cs in enumerate(cases):
+        b.set_initial(t, *cs)
+    b.abort()
+    with pytest.raises(NLSError, match="aborted by the host"):
+        b.run(8)
+    b.close()
+    _, up, _ = _run(ns_list, cases, 20, True, "full")
+    _, ug, _ = _run(ns_list, cases, 20, False, "full")
+    for t in range(len(ns_list)):
+        assert np.array_equal(up[t], ug[t]), t
