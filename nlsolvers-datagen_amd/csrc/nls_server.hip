@@ -49,6 +49,8 @@ constexpr int SBS = 256;
 // and many workgroups finish it sooner (4 cells per thread: p50 62 ms, 1 cell: 30 ms on
 // the Poisson 50 req/s workload).
 constexpr int SCPT = 1;
+// Largest grid side of a request: its cells are indexed with 32-bit integers.
+constexpr int kMaxSide = 46340;
 constexpr int NXCD = 8;
 
 struct ReqDev {
@@ -64,12 +66,12 @@ struct ReqDev {
   float mval, cval;
 };
 
-// UNI: the request's m and c are uniform (values from the descriptor, no field loads)
+// UNI: the request's m and c are uniform (values from the descriptor, no field loads).
+// g: the cell's index (row, col), which also keys its noise.
 template <bool UNI>
-__device__ inline double sp4_new(const ReqDev &R, const double *u, const double *up, int64_t g,
-                                 int64_t i, uint32_t it) {
+__device__ inline double sp4_new(const ReqDev &R, const double *u, const double *up, int g,
+                                 int row, int col, uint32_t it) {
   const int n = R.n;
-  const int64_t row = i / n, col = i - row * n;
   const double ui = u[g];
   const double ci = UNI ? (double)R.cval : (double)R.cf[g];
   auto cn = [&](int64_t o) { return UNI ? ci : (double)R.cf[o]; };
@@ -80,7 +82,7 @@ __device__ inline double sp4_new(const ReqDev &R, const double *u, const double 
   if (col < n - 1) lap += 0.5 * (ci + cn(g + 1)) * (u[g + 1] - ui);
   if (row < n - 1) lap += 0.5 * (ci + cn(g + n)) * (u[g + n] - ui);
   lap *= R.s;
-  const double z = noise1(R.seed, R.stream, it, (uint64_t)i);
+  const double z = noise1(R.seed, R.stream, it, (uint64_t)g);
   const double mv = UNI ? (double)R.mval : (double)R.mf[g];
   if (R.mode == NLS_SP4_REFERENCE) {
     const double nl = -mv * (ui - ui * ui * ui + R.s_eff * z);
@@ -103,15 +105,19 @@ __global__ __launch_bounds__(SBS) void k_sp4_group_step(const ReqDev *__restrict
   double *un = ring[(st + 1) % 3];
   const uint32_t it = (uint32_t)(st + 1);
   const int n = R.n;
-  auto cl = [n](int64_t a) { return a < 1 ? (int64_t)1 : (a > n - 2 ? (int64_t)n - 2 : a); };
-  const int64_t base = (int64_t)cm.y * (SBS * SCPT);
+  auto cl = [n](int a) { return a < 1 ? 1 : (a > n - 2 ? n - 2 : a); };
+  // 32-bit cell indices (a request has at most kMaxSide^2 cells): the division that gives
+  // the row is on the chain from the thread's index to its loads
+  const int base = cm.y * (SBS * SCPT);
 #pragma unroll
   for (int kk = 0; kk < SCPT; ++kk) {
-    const int64_t i = base + threadIdx.x + (int64_t)kk * SBS;
-    if (i >= R.N) break;
-    const int64_t r = i / n, c = i - r * n;
-    const int64_t src = cl(r) * n + cl(c); // Neumann ghost copy (boundaries.cuh:9-19)
-    un[i] = R.uni ? sp4_new<true>(R, u, up, src, src, it) : sp4_new<false>(R, u, up, src, src, it);
+    const int i = base + (int)threadIdx.x + kk * SBS;
+    if (i >= (int)R.N) break;
+    const int r = (int)((unsigned)i / (unsigned)n), c = i - r * n;
+    const int rs = cl(r), cs = cl(c);
+    const int src = rs * n + cs; // Neumann ghost copy (boundaries.cuh:9-19)
+    un[i] = R.uni ? sp4_new<true>(R, u, up, src, rs, cs, it)
+                  : sp4_new<false>(R, u, up, src, rs, cs, it);
   }
 }
 
@@ -534,6 +540,7 @@ int nls_server_submit(nls_server *sv, const nls_request *q, const double *m, con
   return guard([&] {
     NLS_REQUIRE(sv && q && id, "nls_server_submit: null argument");
     NLS_REQUIRE(q->n >= 3 && q->nt >= 2 && q->dt > 0, "nls_server_submit: bad request");
+    NLS_REQUIRE(q->n <= kMaxSide, "nls_server_submit: grid side above " + std::to_string(kMaxSide));
     NLS_REQUIRE(q->num_snapshots >= 0 && q->num_snapshots <= q->nt, "bad num_snapshots");
     auto r = std::make_shared<Request>();
     r->q = *q;
