@@ -395,9 +395,11 @@ NLS_API int nls_batch_pass_timing(nls_batch *b, double *avg_ms, int *launches,
 NLS_API int nls_batch_tile(nls_batch *b, int *cells_per_thread, int *pass_kind, int *lanes);
 /* Steps run so far by the persistent step kernel (nls_batch_params.persistent). */
 NLS_API int nls_batch_persistent_steps(nls_batch *b, int64_t *steps);
-/* Ends the persistent launch in progress, or the next one, from any host thread: every
- * per-trajectory wait of the kernel reads this flag (pinned host memory) while it polls,
- * so every workgroup exits, and that nls_batch_run fails with "aborted by the host".
+/* Ends the persistent launch in progress, or the next one, from any host thread: one
+ * workgroup reads this flag (pinned host memory) at the start of every step and turns
+ * it into the kernel's error flag, which every wait reads while it polls (a waiting
+ * workgroup also reads the host flag every 4096 polls), so every workgroup exits and
+ * that nls_batch_run fails with "aborted by the host".
  * The flag is cleared when the run returns.  The batch then holds a partly advanced
  * step; destroy it.  Captured (non-persistent) steps are not affected.  No reference
  * counterpart (the reference runs one process per trajectory and stops it). */

@@ -1807,6 +1807,10 @@ __global__ __launch_bounds__(FBS) void k_kge_combine_rt(KgeArgs a) {
 // error flag, and every other wait then ends too, so every workgroup exits.
 constexpr int kCntStride = 64;          // ints between two trajectories' counters (256 B)
 constexpr int64_t kSpinMax = 1ll << 21; // poll rounds before a wait gives up (seconds)
+// Poll rounds between two reads of the host's abort flag in a wait: the flag is in host
+// memory, and a read of it in every round by every waiting workgroup made the step 3.3x
+// slower (1.05 -> 3.5 ms on 100 x 300^2).
+constexpr int64_t kAbortPoll = 4096;
 // Waves per SIMD the kernel is compiled for: every phase inlined needs 203-243 VGPRs
 // uncapped (2 waves, 512 resident workgroups); capped at 128 VGPRs (4 waves, 1024
 // workgroups) it spills 252-340 bytes per lane, nearly all outside the phases' loops.
@@ -1823,10 +1827,12 @@ struct PersistArgs {
   int last_pass;    // m - 2 (reference variant) or m - 1 (full T)
 };
 
-// step_start: also read the error and abort flags before polling (once per step), so an
-// abort ends every workgroup at its next step even where no counter falls short.
+// step_start: also read the error flag before polling (once per step), so an abort ends
+// every workgroup at its next step even where no counter falls short; watch: this
+// workgroup reads the host's abort flag there too (one workgroup per launch does, and
+// turns the abort into the error flag for the others).
 __device__ inline bool persist_wait(const int *cnt, int target, const PersistArgs &q, int *flag,
-                                    bool step_start = false) {
+                                    bool step_start = false, bool watch = false) {
   if (threadIdx.x == 0) {
     // code: 1 a wait gave up, 2 the host set the abort flag, 0 the error flag is already set
     int ok = 0, code = 1;
@@ -1835,7 +1841,7 @@ __device__ inline bool persist_wait(const int *cnt, int target, const PersistArg
       if (__hip_atomic_load(q.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
         stop = true;
         code = 0;
-      } else if (__hip_atomic_load(q.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) {
+      } else if (watch && __hip_atomic_load(q.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) {
         stop = true;
         code = 2;
       }
@@ -1845,12 +1851,14 @@ __device__ inline bool persist_wait(const int *cnt, int target, const PersistArg
         ok = 1;
         break;
       }
-      // the flags are read only once the counter has not reached the target
+      // the flags are read only once the counter has not reached the target; the host's
+      // flag once every kAbortPoll rounds
       if (__hip_atomic_load(q.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
         code = 0;
         break;
       }
-      if (__hip_atomic_load(q.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) {
+      if ((it & (kAbortPoll - 1)) == kAbortPoll - 1 &&
+          __hip_atomic_load(q.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)) {
         code = 2;
         break;
       }
@@ -1898,6 +1906,7 @@ __global__ __launch_bounds__(FBS, kPersistWaves) void k_kge_persist2d(KgeArgs a0
   const int np = q.last_pass + 1; // passes per step (at least 3: m >= 3)
   const bool full = q.last_pass == a0.m - 1;
   const int per_step = (np + 1) * C + 1;
+  const bool watch = cm.x == 0 && cm.y == 0; // reads the host's abort flag at each step start
   for (int st = 0; st < q.nsteps; ++st) {
     KgeArgs a = a0;
     const int cur = (int)((q.step0 + st) % 3);
@@ -1906,7 +1915,7 @@ __global__ __launch_bounds__(FBS, kPersistWaves) void k_kge_persist2d(KgeArgs a0
     a.un = q.state[(cur + 1) % 3];
     const int base = st * per_step;
     // pass 0 on the seeds, pass 1, the general passes 2 .. np - 2
-    if (!persist_wait(cnt, base, q, &wait_ok, true)) return;
+    if (!persist_wait(cnt, base, q, &wait_ok, true, watch)) return;
     march2d_body<0, UNI, false>(a, opaque_chunk(cm), 0, q.emax, 1, lds);
     persist_arrive(cnt, false, 0);
     if (!persist_wait(cnt, base + C, q, &wait_ok)) return;
