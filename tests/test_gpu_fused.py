@@ -241,6 +241,33 @@ def test_fused_nlse_vs_oracle(orc, kernel, cpt):
 
 
 
+@pytest.mark.parametrize("amp,dt,k", [(0.9, 0.005, 50), (12.0, 0.01, 5)])
+@pytest.mark.parametrize("variant", ["reference", "full"])
+def test_fused_nlse_constant_state_closed_form(amp, dt, k, variant):
+    """A constant field is an eigenvector of the Neumann Laplacian with eigenvalue 0, so the
+    Krylov factor is the identity and each Strang step multiplies psi by
+    exp(i dt m |psi|^2): after k steps psi = A exp(i k dt m |A|^2).  The half-step phase
+    0.5 dt m |A|^2 is 2e-3 for |A| = 0.9 (small-phase series) and 0.72 for |A| = 12 at
+    dt = 0.01 (the library sincos).  The complex64 field's rounding moves |psi|^2 by about
+    1e-7 per step, which the phase of 1.44 rad per step at |A| = 12 turns into 3.6e-5 after
+    50 steps: that case runs 5 steps."""
+    from nls.batch import FusedBatch
+    n, mval = 40, 1.0
+    A = amp * np.exp(0.7j)
+    for kernel, cpt in [(2, 8), (0, 8)]:
+        b = FusedBatch([n], 10.0, dt, equation="nlse", method="strang", krylov_dim=8,
+                       cells_per_thread=cpt, pass_kernel=kernel, variant=variant)
+        b.set_initial(0, np.full((n, n), mval), np.ones((n, n)),
+                      np.full((n, n), A, dtype=np.complex128))
+        b.run(k)
+        psi, _ = b.state(0)
+        a32 = complex(np.complex64(A))  # the field is held in complex64
+        exact = a32 * np.exp(1j * k * dt * mval * abs(a32) ** 2)
+        err = np.max(np.abs(psi - exact)) / abs(A)
+        b.close()
+        assert err < 1e-5, (kernel, err)
+
+
 @pytest.mark.parametrize("krylov,stalls", [(3, 2), (5, 2), (8, 0), (10, 0)])
 def test_fused_krylov_monitor_vs_oracle(orc, krylov, stalls):
     """Krylov monitor after one step equals the last-vector weight |y_{m-1}| / |y| of the
