@@ -1108,9 +1108,10 @@ __device__ __forceinline__ void march2d_body(const KgeArgs a, const int2 cm, int
   const int n = T.n;
   const int N = (int)T.N;
   const int64_t cells = a.cells;
-  // no c window for uniform fields
-  float *sc = lds, *sv0 = lds + (UNI ? 0 : emax), *sv1 = sv0 + emax, *sp0 = sv1 + emax,
-        *sp1 = sp0 + emax;
+  // no c window for uniform fields; v_j and the previous vector of the two sequences are
+  // interleaved (one 8-byte LDS access per cell for both)
+  float *sc = lds;
+  float2 *sv = reinterpret_cast<float2 *>(lds + (UNI ? 0 : emax)), *sp = sv + emax;
   const size_t m2 = (size_t)(a.m - 2);
   const float *cT = a.cf + T.off, *mT = a.mf + T.off;
   const double *uT = a.u + T.off;
@@ -1170,13 +1171,10 @@ __device__ __forceinline__ void march2d_body(const KgeArgs a, const int2 cm, int
       }
     }
     if (STAGE == 0) {
-      sv0[e] = x0;
-      sv1[e] = x1;
+      sv[e] = make_float2(x0, x1);
     } else {
-      sv0[e] = (x0 - cA[0] * p0) * cB[0];
-      sv1[e] = (x1 - cA[1] * p1) * cB[1];
-      sp0[e] = p0;
-      sp1[e] = p1;
+      sv[e] = make_float2((x0 - cA[0] * p0) * cB[0], (x1 - cA[1] * p1) * cB[1]);
+      sp[e] = make_float2(p0, p1);
     }
   };
   // Lanczos scalars of the previous pass; formed after the first window loads are
@@ -1280,18 +1278,24 @@ __device__ __forceinline__ void march2d_body(const KgeArgs a, const int2 cm, int
           w.wr = w.hr ? 0.5f * (ci + sc[e + oR]) : 0.0f;
           w.wd = w.hd ? 0.5f * (ci + sc[e + oD]) : 0.0f;
         }
+        const float2 vc = sv[e], vU = sv[e + oU], vL = sv[e + oL], vR = sv[e + oR],
+                     vD = sv[e + oD];
+        const float2 pc = STAGE == 0 ? make_float2(0.0f, 0.0f) : sp[e];
 #pragma unroll
         for (int s = 0; s < NSEQ; ++s) {
-          const float *sv = s == 0 ? sv0 : sv1;
-          const float vi = sv[e];
-          const float Av = apply5z(w, sv, e, oU, oL, oR, oD);
+          // apply5z's sum, term by term in the same order
+          const float vi = s == 0 ? vc.x : vc.y;
+          float Av = w.wu * (vi - (s == 0 ? vU.x : vU.y));
+          Av += w.wl * (vi - (s == 0 ? vL.x : vL.y));
+          Av += w.wr * (vi - (s == 0 ? vR.x : vR.y));
+          Av += w.wd * (vi - (s == 0 ? vD.x : vD.y));
           if (STAGE == 0) {
             at32(s == 0 ? Wout0 : Wout1, (uint32_t)i) = Av;
             acc[s * PN + 0] += (double)vi * vi;
             acc[s * PN + 1] += (double)vi * Av;
             acc[s * PN + 2] += (double)Av * Av;
           } else {
-            const float wp = Av - cC[s] * (s == 0 ? sp0 : sp1)[e];
+            const float wp = Av - cC[s] * (s == 0 ? pc.x : pc.y);
             if (!LAST) {
               at32(s == 0 ? Vout0 : Vout1, (uint32_t)i) = vi;
               at32(s == 0 ? Wout0 : Wout1, (uint32_t)i) = wp;
@@ -1328,12 +1332,9 @@ __device__ __forceinline__ void march2d_body(const KgeArgs a, const int2 cm, int
     // keep [cb' - n, cb' + n) (disjoint source and destination: 2n <= SUB)
     for (int e = threadIdx.x; e < 2 * n; e += FBS) {
       if (!UNI) sc[e] = sc[MSUB + e];
-      sv0[e] = sv0[MSUB + e];
-      sv1[e] = sv1[MSUB + e];
-      if (STAGE >= 1 && e >= n) { // the previous vector is read at own cells only
-        sp0[e] = sp0[MSUB + e];
-        sp1[e] = sp1[MSUB + e];
-      }
+      sv[e] = sv[MSUB + e];
+      if (STAGE >= 1 && e >= n) // the previous vector is read at own cells only
+        sp[e] = sp[MSUB + e];
     }
     lds_barrier();
 #pragma unroll
