@@ -63,6 +63,28 @@ template <class T> __device__ inline T wave_sum(T v) {
   return v;
 }
 
+// FP64 sum across a 64-lane wavefront from data-parallel-primitive (DPP) lane moves inside
+// each 16-lane row (lane pairs i^1, i^2, then the mirrored lanes of each 8 and of each 16)
+// and the four row sums read from lanes 0, 16, 32, 48 and added in that order.  Every lane
+// returns the same total, bitwise reproducible for identical inputs; no LDS traffic (the
+// butterfly of wave_sum moves each level through LDS with ds_bpermute).
+template <int CTRL> __device__ inline double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ inline double wave_sum_dpp(double v) {
+  v += dpp_move<0xB1>(v);  // quad_perm [1,0,3,2]: lane i^1
+  v += dpp_move<0x4E>(v);  // quad_perm [2,3,0,1]: lane i^2
+  v += dpp_move<0x141>(v); // row_half_mirror: lane 7-i of each 8
+  v += dpp_move<0x140>(v); // row_mirror: lane 15-i of each 16
+  auto lane_d = [&](int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+  };
+  return ((lane_d(0) + lane_d(16)) + lane_d(32)) + lane_d(48);
+}
+
 // Sum over a workgroup of BS threads (BS a multiple of 64).  Fixed order: each
 // wave reduces by butterfly, wave results are added in wave-index order by
 // wave 0.  Every thread returns the total.  sh must hold BS/64 entries.

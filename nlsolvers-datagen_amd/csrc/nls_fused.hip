@@ -248,7 +248,7 @@ __device__ inline void reduce_partials(const KgeArgs &a, const TrajDev &T, int p
     }
 #pragma unroll
     for (int k = 0; k < NSEQ * PN; ++k) {
-      const double v = wave_sum(acc[k]);
+      const double v = wave_sum_dpp(acc[k]);
       if (lane == 0) red[k] = v;
     }
   }
@@ -646,7 +646,7 @@ template <int NV>
 __device__ inline void block_sum_n(double (&v)[NV], double *sh /* NV * (FBS/64) */) {
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
 #pragma unroll
-  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum_dpp(v[k]);
   lds_barrier();
   if (lane == 0)
 #pragma unroll
