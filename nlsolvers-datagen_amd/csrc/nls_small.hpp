@@ -277,6 +277,25 @@ __device__ inline void small_apply_f_wave(int m, double lam, double (*Z)[kMaxKry
   yi = beta0 * (phr * ai + phi * ar);
 }
 
+// T * x for the tridiagonal held one row per lane (lane k: d = T[k][k], e = T[k][k+1],
+// eu = T[k][k-1]; lanes >= m hold zeros): the neighbours' x come from DPP row shifts when
+// every row lies in the first 16 lanes (m <= 16; lane 15's right neighbour and lane 0's
+// left one are not needed there), from cross-lane shuffles otherwise.
+__device__ inline double tridiag_apply_wave(int m, double d, double e, double eu, double x) {
+  const int lane = threadIdx.x & (kWave - 1);
+  double up, dn;
+  if (m <= 16) {
+    up = dpp_move<0x101>(x); // row_shl:1, lane k + 1
+    dn = dpp_move<0x111>(x); // row_shr:1, lane k - 1
+  } else {
+    up = __shfl_down(x, 1, kWave);
+    dn = __shfl_up(x, 1, kWave);
+  }
+  double r = d * x + e * (lane + 1 < m ? up : 0.0);
+  if (lane > 0) r += eu * dn;
+  return r;
+}
+
 // Largest row sum of |T| for the wavefront layout of small_ql_wave (lane k: d_k, e_k =
 // T[k][k+1]; lanes >= m ignored), in every lane.
 __device__ inline double tridiag_norm_inf_wave(int m, double dk, double ek) {
@@ -310,9 +329,13 @@ __device__ inline bool tridiag_nonneg_wave(int m, double dk, double ek) {
 // same function of T the eigen-decomposition gives:
 //   cos(t sqrt l) - 1    = sum_{k>=1} (-t^2 l)^k / (2k)!
 //   (sin x / x)^2, x^2 = t^2 l = sum_{k>=0} (-1)^k 2^(2k+1) (t^2 l)^k / (2k+2)!
-// (F_SINC2_HALF: t/2 in place of t).  Term k is ratio_k * T * term_{k-1}, summed until a
-// term drops below 1e-17 of the sum (at most 60 terms).  Meant for t^2 ||T||_inf <= 4,
-// where no term exceeds the sum by more than a factor of a few.
+// (F_SINC2_HALF: t/2 in place of t).  Term k is ratio_k * T * term_{k-1}.  With
+// rho = tt ||T||_inf, ||term_k|| is at most b_k = b_{k-1} rho / ((2k)(2k - 1)) (cos - 1,
+// b_0 = 1, the sum starting at term 1) or b_{k-1} 4 rho / ((2k + 2)(2k + 1)) (sinc^2, the
+// sum starting at term 0 = e1); the sum takes K terms, K the first with b_{K+1} <= 1e-17
+// of the first summed term's bound, fixed before the loop, so no term needs a reduction
+// across the wavefront (at most 60 terms).  Meant for t^2 ||T||_inf <= 4, where no term
+// exceeds the sum by more than a factor of a few.
 __device__ inline double small_kge_series_wave(int m, double dk, double ek, int kind, double t,
                                                double beta0) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -323,22 +346,28 @@ __device__ inline double small_kge_series_wave(int m, double dk, double ek, int 
   const double eu = lane > 0 ? el : 0.0;
   const double tt = kind == F_SINC2_HALF ? 0.25 * t * t : t * t;
   const bool cosm1 = kind == F_COSM1_SQRT;
+  const double rho = tt * tridiag_norm_inf_wave(m, dk, ek);
+  auto ratio_bound = [&](int k) {
+    return cosm1 ? rho / ((2.0 * k) * (2.0 * k - 1.0))
+                 : 4.0 * rho / ((2.0 * k + 2.0) * (2.0 * k + 1.0));
+  };
+  double b = ratio_bound(1); // bound of term 1
+  const double first = cosm1 ? b : 1.0;
+  int K = 1;
+  while (K < 60) {
+    const double next = b * ratio_bound(K + 1);
+    if (next <= 1e-17 * first) break;
+    b = next;
+    ++K;
+  }
   double term = lane == 0 ? 1.0 : 0.0; // e1
   double acc = cosm1 ? 0.0 : term;
-  for (int k = 1; k <= 60; ++k) {
-    const double up = __shfl_down(term, 1, kWave), dn = __shfl_up(term, 1, kWave);
-    double x = d * term + e * (lane + 1 < m ? up : 0.0);
-    if (lane > 0) x += eu * dn;
+  for (int k = 1; k <= K; ++k) {
+    const double x = tridiag_apply_wave(m, d, e, eu, term);
     const double ratio = cosm1 ? -tt / ((2.0 * k) * (2.0 * k - 1.0))
                                : -4.0 * tt / ((2.0 * k + 2.0) * (2.0 * k + 1.0));
     term = in ? ratio * x : 0.0;
     acc += term;
-    double tn = fabs(term), an = fabs(acc);
-    for (int off = 32; off >= 1; off >>= 1) {
-      tn = fmax(tn, __shfl_xor(tn, off, kWave));
-      an = fmax(an, __shfl_xor(an, off, kWave));
-    }
-    if (tn <= 1e-17 * an) break;
   }
   return beta0 * acc;
 }
@@ -380,14 +409,8 @@ __device__ inline void small_exp_series_wave(int m, double dk, double ek, double
     double ar = vr, ai = vi, tr = vr, ti = vi;
     for (int k = 1; k <= K; ++k) {
       // T * term
-      const double tr_dn = __shfl_up(tr, 1, kWave), ti_dn = __shfl_up(ti, 1, kWave);
-      const double tr_up = __shfl_down(tr, 1, kWave), ti_up = __shfl_down(ti, 1, kWave);
-      double xr = d * tr + e * (lane + 1 < m ? tr_up : 0.0);
-      double xi = d * ti + e * (lane + 1 < m ? ti_up : 0.0);
-      if (lane > 0) {
-        xr += eu * tr_dn;
-        xi += eu * ti_dn;
-      }
+      const double xr = tridiag_apply_wave(m, d, e, eu, tr);
+      const double xi = tridiag_apply_wave(m, d, e, eu, ti);
       // term <- (c / k) * (T term)
       const double inv = lane_read(rk, k - 1);
       tr = in ? (cr * xr - ci * xi) * inv : 0.0;
