@@ -1218,26 +1218,33 @@ __device__ __forceinline__ void march2d_body(const KgeArgs a, const int2 cm, int
   // number of stores of a full sub-chunk, are fixed, so waiting for the prefetched window
   // does not have to wait for the stores issued after it.
   auto clampi = [N](int i) { return i < 0 ? 0 : (i >= N ? N - 1 : i); };
-  // first window [-n, SUB + n), MSPT loads per thread in flight before their deposits
-  for (int e0 = 0; e0 < MSUB + 2 * n; e0 += MSPT * FBS) {
-    float rc[MSPT], x0[MSPT], x1[MSPT], p0[MSPT], p1[MSPT], rm[MSPT];
-    double ru[MSPT];
-    bool ok[MSPT];
+  // first window [-n, SUB + n), at most 2 SUB cells (2n <= SUB): the loads of both halves
+  // are in flight before any deposit
+  {
+    float rc[2][MSPT], x0[2][MSPT], x1[2][MSPT], p0[2][MSPT], p1[2][MSPT], rm[2][MSPT];
+    double ru[2][MSPT];
+    bool ok[2][MSPT];
 #pragma unroll
-    for (int q = 0; q < MSPT; ++q) {
-      const int e = e0 + threadIdx.x + q * FBS, i = cell(e - n);
-      ok[q] = i >= 0 && i < N;
-      if (e < MSUB + 2 * n) fetch(clampi(i), rc[q], x0[q], x1[q], p0[q], p1[q], ru[q], rm[q]);
-    }
-    if (e0 == 0) coefs();
+    for (int r = 0; r < 2; ++r)
 #pragma unroll
-    for (int q = 0; q < MSPT; ++q) {
-      const int e = e0 + threadIdx.x + q * FBS;
-      if (e < MSUB + 2 * n) {
-        if (!ok[q]) rc[q] = x0[q] = x1[q] = p0[q] = p1[q] = rm[q] = 0.0f, ru[q] = 0.0;
-        deposit(e, rc[q], x0[q], x1[q], p0[q], p1[q], ru[q], rm[q]);
+      for (int q = 0; q < MSPT; ++q) {
+        const int e = r * MSUB + threadIdx.x + q * FBS, i = cell(e - n);
+        ok[r][q] = i >= 0 && i < N;
+        if (e < MSUB + 2 * n)
+          fetch(clampi(i), rc[r][q], x0[r][q], x1[r][q], p0[r][q], p1[r][q], ru[r][q], rm[r][q]);
       }
-    }
+    coefs();
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int q = 0; q < MSPT; ++q) {
+        const int e = r * MSUB + threadIdx.x + q * FBS;
+        if (e < MSUB + 2 * n) {
+          if (!ok[r][q])
+            rc[r][q] = x0[r][q] = x1[r][q] = p0[r][q] = p1[r][q] = rm[r][q] = 0.0f, ru[r][q] = 0.0;
+          deposit(e, rc[r][q], x0[r][q], x1[r][q], p0[r][q], p1[r][q], ru[r][q], rm[r][q]);
+        }
+      }
   }
   // Grid column of this thread's cell at local coordinate l = cb + tid + kk * FBS, from
   // the column of l = tid and the offset (cb + kk * FBS) mod n, which is the same for the
