@@ -462,8 +462,9 @@ NLS_API int nls_h5_writer_destroy(nls_h5_writer *w);
  * doubles (buf may be NULL to query dims and type). */
 NLS_API int nls_h5_read_dataset(const char *path, const char *name, double *buf, int64_t cap,
                                 int64_t *dims, int *ndim, int *is_complex);
-/* Attribute `name` of object `obj` ("metadata", "grid", ...): numeric into *num, or
- * string into str (len bytes, NUL-terminated); *is_string tells which. */
+/* Attribute `name` of object `obj` ("metadata", "grid", ..., "/" for the file's global
+ * attributes): numeric into *num, or string (variable- or fixed-length, as NetCDF-4 text
+ * attributes are) into str (len bytes, NUL-terminated); *is_string tells which. */
 NLS_API int nls_h5_read_attr(const char *path, const char *obj, const char *name, double *num,
                              char *str, int len, int *is_string);
 
@@ -486,6 +487,26 @@ NLS_API int nls_nc_class_create(const char *path, int dims, const int *shape, in
 NLS_API int nls_nc_class_put(nls_nc_class *h, int traj, const float *u, const float *v,
                              const float *u0, const float *v0, const float *m, const float *c);
 NLS_API int nls_nc_class_close(nls_nc_class *h);
+/* The canonical per-class NetCDF-4 file that canonicalize_data.py:174-262 makes from a
+ * class file: dimensions member, time, x, y[, z] with coordinate variables (member int
+ * 0..num_trajectories-1, time = linspace(0, T, num_snapshots), x/y/z = linspace(-L, L, n)),
+ * float variables u, v (complex classes: u_real, u_imag) over (member, time, x, y[, z])
+ * chunked one trajectory per chunk, m and c over (member, x, y[, z]), and the global
+ * attributes pde_type, boundary_condition, num_trajectories, num_snapshots, nt,
+ * spatial_shape, dims, is_complex, Lx, Ly[, Lz], T and, when not NULL, m_classes and
+ * c_classes.  Same NetCDF-4 conventions as nls_nc_class. */
+typedef struct nls_nc_std nls_nc_std;
+NLS_API int nls_nc_std_create(const char *path, int dims, const int *shape, int num_snapshots,
+                              int num_trajectories, int is_complex, double Lx, double Ly,
+                              double Lz, double T, int nt, const char *pde_type,
+                              const char *boundary_condition, const char *m_classes,
+                              const char *c_classes, nls_nc_std **out);
+/* One trajectory (float32 host arrays): u, v of num_snapshots x cells (complex classes:
+ * the real and imaginary parts of u), m, c of cells; a NULL array leaves that part at
+ * netCDF's fill value. */
+NLS_API int nls_nc_std_put(nls_nc_std *h, int traj, const float *u, const float *v,
+                           const float *m, const float *c);
+NLS_API int nls_nc_std_close(nls_nc_std *h);
 
 /* ---------------------------------------------------------------- serving loop */
 /* Continuous batching of stochastic phi^4 trajectory requests of different

@@ -32,7 +32,7 @@ using htri_t = int;
 using hsize_t = unsigned long long;
 constexpr hid_t P_DEFAULT = 0, S_ALL = 0, E_DEFAULT = 0;
 constexpr unsigned ACC_RDONLY = 0u, ACC_TRUNC = 2u;
-constexpr int S_SCALAR = 0, S_SELECT_SET = 0, T_COMPOUND = 6, T_CSET_UTF8 = 1;
+constexpr int S_SCALAR = 0, S_SELECT_SET = 0, T_STRING = 3, T_COMPOUND = 6, T_CSET_UTF8 = 1;
 constexpr hsize_t S_UNLIMITED = ~0ull;
 constexpr size_t T_VARIABLE = ~(size_t)0;
 
@@ -79,6 +79,7 @@ struct Api {
   hid_t (*tcreate)(int, size_t);
   herr_t (*tinsert)(hid_t, const char *, size_t, hid_t);
   int (*tget_class)(hid_t);
+  size_t (*tget_size)(hid_t);
   htri_t (*tis_variable_str)(hid_t);
   herr_t (*tclose)(hid_t);
   herr_t (*free_memory)(void *);
@@ -137,7 +138,7 @@ Api *load() {
             bind(a, a.aclose, "H5Aclose") && bind(a, a.tcopy, "H5Tcopy") &&
             bind(a, a.tset_size, "H5Tset_size") && bind(a, a.tset_cset, "H5Tset_cset") &&
             bind(a, a.tcreate, "H5Tcreate") && bind(a, a.tinsert, "H5Tinsert") &&
-            bind(a, a.tget_class, "H5Tget_class") &&
+            bind(a, a.tget_class, "H5Tget_class") && bind(a, a.tget_size, "H5Tget_size") &&
             bind(a, a.tis_variable_str, "H5Tis_variable_str") && bind(a, a.tclose, "H5Tclose") &&
             bind(a, a.free_memory, "H5free_memory") &&
             bind(a, a.native_double, "H5T_NATIVE_DOUBLE_g") &&
@@ -657,6 +658,199 @@ NcClass::~NcClass() {
   }
 }
 
+// ---------------------------------------------------------------- canonical class files
+namespace {
+constexpr int kFillInt = -2147483647; // NC_FILL_INT
+
+void nc_attr_f64(Api &a, hid_t obj, const char *name, double v) {
+  const hsize_t one = 1;
+  Id sp(chk(a.screate_simple(1, &one, nullptr), "H5Screate_simple"), a.sclose);
+  Id at(chk(a.acreate(obj, name, *a.native_double, sp.id, P_DEFAULT, P_DEFAULT),
+            std::string("attribute ") + name),
+        a.aclose);
+  if (a.awrite(at.id, *a.native_double, &v) < 0) throw Error(std::string("HDF5: writing ") + name);
+}
+
+void nc_attr_i32v(Api &a, hid_t obj, const char *name, const std::vector<int> &v) {
+  const hsize_t len = v.size();
+  Id sp(chk(a.screate_simple(1, &len, nullptr), "H5Screate_simple"), a.sclose);
+  Id at(chk(a.acreate(obj, name, *a.native_int, sp.id, P_DEFAULT, P_DEFAULT),
+            std::string("attribute ") + name),
+        a.aclose);
+  if (a.awrite(at.id, *a.native_int, v.data()) < 0) throw Error(std::string("HDF5: writing ") + name);
+}
+
+// A coordinate variable: the dimension's own dataset holds its values, is a dimension scale
+// named after itself and carries the dimension id (NetCDF-4's layout of a variable that
+// shares its dimension's name).
+hid_t nc_coordinate(Api &a, hid_t f, const char *name, bool is_int, const void *values,
+                    hsize_t len, int dimid) {
+  Id sp(chk(a.screate_simple(1, &len, nullptr), "H5Screate_simple"), a.sclose);
+  Id pl(chk(a.pcreate(*a.dataset_create), "H5Pcreate"), a.pclose);
+  const hid_t ty = is_int ? *a.native_int : *a.native_float;
+  if (is_int) a.pset_fill_value(pl.id, ty, &kFillInt);
+  else a.pset_fill_value(pl.id, ty, &kFillFloat);
+  const hid_t d = chk(a.dcreate(f, name, ty, sp.id, P_DEFAULT, pl.id, P_DEFAULT),
+                      std::string("coordinate ") + name);
+  if (a.dwrite(d, ty, sp.id, sp.id, P_DEFAULT, values) < 0)
+    throw Error(std::string("HDF5: writing ") + name);
+  if (a.ds_set_scale(d, name) < 0) throw Error(std::string("HDF5: dimension scale ") + name);
+  nc_attr_i32(a, d, "_Netcdf4Dimid", dimid);
+  return d;
+}
+
+// A float variable over the dimensions `dims` (their datasets, in order) with the extents
+// `ext`, chunked as `chunk` when given, contiguous otherwise.
+hid_t nc_variable_nd(Api &a, hid_t f, const char *name, const std::vector<hid_t> &dims,
+                     const std::vector<int> &dimids, const std::vector<hsize_t> &ext,
+                     const std::vector<hsize_t> &chunk) {
+  Id sp(chk(a.screate_simple((int)ext.size(), ext.data(), nullptr), "H5Screate_simple"),
+        a.sclose);
+  Id pl(chk(a.pcreate(*a.dataset_create), "H5Pcreate"), a.pclose);
+  if (!chunk.empty()) a.pset_chunk(pl.id, (int)chunk.size(), chunk.data());
+  a.pset_fill_value(pl.id, *a.native_float, &kFillFloat);
+  const hid_t d = chk(a.dcreate(f, name, *a.native_float, sp.id, P_DEFAULT, pl.id, P_DEFAULT),
+                      std::string("variable ") + name);
+  for (size_t k = 0; k < dims.size(); ++k)
+    if (a.ds_attach_scale(d, dims[k], (unsigned)k) < 0)
+      throw Error(std::string("HDF5: attaching ") + name);
+  nc_attr_i32v(a, d, "_Netcdf4Coordinates", dimids);
+  return d;
+}
+
+// rows [row, row + 1) of a variable whose first extent is the trajectory: count floats
+void nc_put_row(Api &a, hid_t d, int rank, hsize_t row, const std::vector<hsize_t> &ext,
+                const float *x) {
+  if (!x) return;
+  std::vector<hsize_t> off(rank, 0), cnt(ext.begin(), ext.end());
+  off[0] = row;
+  cnt[0] = 1;
+  Id fs(chk(a.dget_space(d), "H5Dget_space"), a.sclose);
+  a.sselect_hyperslab(fs.id, S_SELECT_SET, off.data(), nullptr, cnt.data(), nullptr);
+  Id ms(chk(a.screate_simple(rank, cnt.data(), nullptr), "H5Screate_simple"), a.sclose);
+  if (a.dwrite(d, *a.native_float, ms.id, fs.id, P_DEFAULT, x) < 0)
+    throw Error("HDF5: writing a trajectory");
+}
+} // namespace
+
+NcStandard::NcStandard(const std::string &path, int dims, const std::vector<int> &shape, int ns,
+                       int ntraj, bool is_complex, const Attrs &at)
+    : dims_(dims), ns_(ns), ntraj_(ntraj), shape_(shape) {
+  NLS_REQUIRE(dims == 2 || dims == 3, "NetCDF standard file: dims must be 2 or 3");
+  NLS_REQUIRE((int)shape.size() == dims, "NetCDF standard file: shape must have dims entries");
+  NLS_REQUIRE(ns > 0 && ntraj > 0, "NetCDF standard file: need snapshots and trajectories");
+  spat_ = 1;
+  for (int x : shape) {
+    NLS_REQUIRE(x > 0, "NetCDF standard file: bad shape");
+    spat_ *= (hsize_t)x;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  Api &a = need();
+  NLS_REQUIRE(a.ds_set_scale && a.ds_attach_scale,
+              "NetCDF files need libhdf5_hl (dimension scales); set NLS_HDF5_HL_LIB");
+  file_ = chk(a.fcreate(path.c_str(), ACC_TRUNC, P_DEFAULT, P_DEFAULT), "create " + path);
+  try {
+    // coordinates (canonicalize_data.py:218-223)
+    std::vector<int> member(ntraj);
+    for (int t = 0; t < ntraj; ++t) member[t] = t;
+    auto linspace = [](double lo, double hi, int n) {
+      std::vector<float> v(n);
+      for (int k = 0; k < n; ++k)
+        v[k] = (float)(n == 1 ? lo : lo + (hi - lo) * (double)k / (double)(n - 1));
+      return v;
+    };
+    const std::vector<float> tv = linspace(0.0, at.T, ns);
+    const double Ls[3] = {at.Lx, at.Ly, at.Lz};
+    const char *axes[3] = {"x", "y", "z"};
+    dim_[0] = nc_coordinate(a, file_, "member", true, member.data(), (hsize_t)ntraj, 0);
+    dim_[1] = nc_coordinate(a, file_, "time", false, tv.data(), (hsize_t)ns, 1);
+    for (int k = 0; k < dims; ++k) {
+      const std::vector<float> xv = linspace(-Ls[k], Ls[k], shape[k]);
+      dim_[2 + k] = nc_coordinate(a, file_, axes[k], false, xv.data(), (hsize_t)shape[k], 2 + k);
+    }
+    // fields (member, time, x, y[, z]), one trajectory per chunk; m, c (member, x, y[, z])
+    std::vector<hid_t> fd = {dim_[0], dim_[1]}, pd = {dim_[0]};
+    std::vector<int> fid = {0, 1}, pid = {0};
+    std::vector<hsize_t> fext = {(hsize_t)ntraj, (hsize_t)ns}, pext = {(hsize_t)ntraj};
+    for (int k = 0; k < dims; ++k) {
+      fd.push_back(dim_[2 + k]);
+      pd.push_back(dim_[2 + k]);
+      fid.push_back(2 + k);
+      pid.push_back(2 + k);
+      fext.push_back((hsize_t)shape[k]);
+      pext.push_back((hsize_t)shape[k]);
+    }
+    std::vector<hsize_t> fch(fext);
+    fch[0] = 1;
+    const char *names[2] = {is_complex ? "u_real" : "u", is_complex ? "u_imag" : "v"};
+    for (int k = 0; k < 2; ++k) vu_[k] = nc_variable_nd(a, file_, names[k], fd, fid, fext, fch);
+    vm_ = nc_variable_nd(a, file_, "m", pd, pid, pext, {});
+    vc_ = nc_variable_nd(a, file_, "c", pd, pid, pext, {});
+    // global attributes (canonicalize_data.py:264-279)
+    nc_attr_text(a, file_, "pde_type", at.pde_type);
+    nc_attr_text(a, file_, "boundary_condition", at.boundary_condition);
+    nc_attr_i32(a, file_, "num_trajectories", ntraj);
+    nc_attr_i32(a, file_, "num_snapshots", ns);
+    nc_attr_i32(a, file_, "nt", at.nt);
+    std::string sh = "(";
+    for (int k = 0; k < dims; ++k) sh += std::to_string(shape[k]) + (k + 1 < dims ? ", " : "");
+    sh += ")";
+    nc_attr_text(a, file_, "spatial_shape", sh);
+    nc_attr_i32(a, file_, "dims", dims);
+    nc_attr_i32(a, file_, "is_complex", is_complex ? 1 : 0);
+    nc_attr_f64(a, file_, "Lx", at.Lx);
+    nc_attr_f64(a, file_, "Ly", at.Ly);
+    if (dims == 3 && at.Lz != 0.0) nc_attr_f64(a, file_, "Lz", at.Lz);
+    nc_attr_f64(a, file_, "T", at.T);
+    if (!at.m_classes.empty()) nc_attr_text(a, file_, "m_classes", at.m_classes);
+    if (!at.c_classes.empty()) nc_attr_text(a, file_, "c_classes", at.c_classes);
+    a.fflush(file_, 0);
+  } catch (...) {
+    close_locked();
+    throw;
+  }
+}
+
+void NcStandard::put(int traj, const float *u, const float *v, const float *m, const float *c) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  NLS_REQUIRE(file_ >= 0, "NetCDF standard file: closed");
+  NLS_REQUIRE(traj >= 0 && traj < ntraj_, "NetCDF standard file: trajectory index out of range");
+  Api &a = need();
+  std::vector<hsize_t> fext = {(hsize_t)ntraj_, (hsize_t)ns_}, pext = {(hsize_t)ntraj_};
+  for (int x : shape_) {
+    fext.push_back((hsize_t)x);
+    pext.push_back((hsize_t)x);
+  }
+  nc_put_row(a, vu_[0], dims_ + 2, (hsize_t)traj, fext, u);
+  nc_put_row(a, vu_[1], dims_ + 2, (hsize_t)traj, fext, v);
+  nc_put_row(a, vm_, dims_ + 1, (hsize_t)traj, pext, m);
+  nc_put_row(a, vc_, dims_ + 1, (hsize_t)traj, pext, c);
+}
+
+void NcStandard::close_locked() {
+  if (file_ < 0) return;
+  Api &a = need();
+  for (int64_t *d : {&vu_[0], &vu_[1], &vm_, &vc_, &dim_[0], &dim_[1], &dim_[2], &dim_[3], &dim_[4]})
+    if (*d >= 0) {
+      a.dclose(*d);
+      *d = -1;
+    }
+  a.fclose(file_);
+  file_ = -1;
+}
+
+void NcStandard::close() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  close_locked();
+}
+
+NcStandard::~NcStandard() {
+  try {
+    close();
+  } catch (...) {
+  }
+}
+
 } // namespace h5
 } // namespace nls
 
@@ -833,6 +1027,52 @@ int nls_nc_class_close(nls_nc_class *h) {
   });
 }
 
+struct nls_nc_std {
+  std::unique_ptr<nls::h5::NcStandard> w;
+};
+
+int nls_nc_std_create(const char *path, int dims, const int *shape, int num_snapshots,
+                      int num_trajectories, int is_complex, double Lx, double Ly, double Lz,
+                      double T, int nt, const char *pde_type, const char *boundary_condition,
+                      const char *m_classes, const char *c_classes, nls_nc_std **out) {
+  return nls::guard([&] {
+    NLS_REQUIRE(path && shape && out && pde_type && boundary_condition,
+                "nls_nc_std_create: null argument");
+    NLS_REQUIRE(dims == 2 || dims == 3, "nls_nc_std_create: dims must be 2 or 3");
+    nls::h5::NcStandard::Attrs at;
+    at.pde_type = pde_type;
+    at.boundary_condition = boundary_condition;
+    at.m_classes = m_classes ? m_classes : "";
+    at.c_classes = c_classes ? c_classes : "";
+    at.Lx = Lx;
+    at.Ly = Ly;
+    at.Lz = Lz;
+    at.T = T;
+    at.nt = nt;
+    auto h = std::make_unique<nls_nc_std>();
+    h->w = std::make_unique<nls::h5::NcStandard>(path, dims, std::vector<int>(shape, shape + dims),
+                                                 num_snapshots, num_trajectories, is_complex != 0,
+                                                 at);
+    *out = h.release();
+  });
+}
+
+int nls_nc_std_put(nls_nc_std *h, int traj, const float *u, const float *v, const float *m,
+                   const float *c) {
+  return nls::guard([&] {
+    NLS_REQUIRE(h && h->w, "nls_nc_std_put: null handle");
+    h->w->put(traj, u, v, m, c);
+  });
+}
+
+int nls_nc_std_close(nls_nc_std *h) {
+  return nls::guard([&] {
+    if (!h) return;
+    std::unique_ptr<nls_nc_std> own(h);
+    if (own->w) own->w->close();
+  });
+}
+
 int nls_h5_read_attr(const char *path, const char *obj, const char *name, double *num, char *str,
                      int len, int *is_string) {
   using namespace nls::h5;
@@ -846,8 +1086,19 @@ int nls_h5_read_attr(const char *path, const char *obj, const char *name, double
           a.aclose);
     Id ty(chk(a.aget_type(at.id), "H5Aget_type"), a.tclose);
     const bool s = a.tis_variable_str(ty.id) > 0;
-    if (is_string) *is_string = s ? 1 : 0;
-    if (s) {
+    const bool fixed = !s && a.tget_class(ty.id) == T_STRING; // NetCDF-4 text attributes
+    if (is_string) *is_string = (s || fixed) ? 1 : 0;
+    if (fixed) {
+      const size_t sz = a.tget_size(ty.id);
+      std::string buf(sz + 1, '\0');
+      Id mt(chk(a.tcopy(*a.c_s1), "H5Tcopy"), a.tclose);
+      a.tset_size(mt.id, sz);
+      if (a.aread(at.id, mt.id, &buf[0]) < 0) throw nls::Error("HDF5: reading attribute");
+      if (str && len > 0) {
+        std::strncpy(str, buf.c_str(), (size_t)len - 1);
+        str[len - 1] = '\0';
+      }
+    } else if (s) {
       Id mt(chk(a.tcopy(*a.c_s1), "H5Tcopy"), a.tclose);
       a.tset_size(mt.id, T_VARIABLE);
       a.tset_cset(mt.id, T_CSET_UTF8);

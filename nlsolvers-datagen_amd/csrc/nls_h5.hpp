@@ -84,5 +84,35 @@ class NcClass {
   bool complex_ = false;
 };
 
+// The canonical per-class NetCDF-4 file (canonicalize_data.py:174-262): dimensions member,
+// time, x, y[, z] with coordinate variables (member int 0..; time linspace(0, T, ns); x, y,
+// z linspace(-L, L, n)), float variables u, v (complex classes: u_real, u_imag) over
+// (member, time, x, y[, z]) in chunks of one trajectory, m and c over (member, x, y[, z]),
+// and the global attributes pde_type, boundary_condition, num_trajectories,
+// num_snapshots, nt, spatial_shape, dims, is_complex, Lx, Ly[, Lz], T[, m_classes,
+// c_classes].  NetCDF-4 conventions as NcClass.
+class NcStandard {
+ public:
+  struct Attrs {
+    std::string pde_type, boundary_condition, m_classes, c_classes;
+    double Lx = 0, Ly = 0, Lz = 0, T = 0;
+    int nt = 0;
+  };
+  NcStandard(const std::string &path, int dims, const std::vector<int> &shape, int ns,
+             int ntraj, bool is_complex, const Attrs &at);
+  ~NcStandard();
+  // one trajectory: u, v of ns x cells (complex: real and imaginary parts), m, c of cells
+  // (any may be null: that part stays at the fill value)
+  void put(int traj, const float *u, const float *v, const float *m, const float *c);
+  void close();
+
+ private:
+  void close_locked();
+  int64_t file_ = -1, dim_[5] = {-1, -1, -1, -1, -1}, vu_[2] = {-1, -1}, vm_ = -1, vc_ = -1;
+  uint64_t spat_ = 0;
+  int dims_ = 0, ns_ = 0, ntraj_ = 0;
+  std::vector<int> shape_;
+};
+
 } // namespace h5
 } // namespace nls

@@ -7,21 +7,25 @@ The reference packages a generated dataset in two steps after the runs:
     c_classes), named config_<pde>_<dims>D_<shape>_ns<ns>_nt<nt>.yaml; the descriptors are
     what canonicalize_data.py:174-262 and the dataset's users read;
   * aggregate_single_core.py:57-206 groups the run files by (dims, stored shape,
-    snapshots, domain, complex) and writes one NetCDF-4 file per group.
-Here both read the files this library writes (nls.h5, the launchers' layout); the
-NetCDF-4 file is written by the C ABI (nls_nc_class_*) through libhdf5 with the
-NetCDF-4 conventions.  The class keys, file names, variables, chunking, attribute
+    snapshots, domain, complex) and writes one NetCDF-4 file per group;
+  * canonicalize_data.py:39-262 rewrites a class file in the canonical layout
+    (member, time, x, y[, z]) with coordinate variables and the descriptor's attributes.
+Here they read the files this library writes (nls.h5, the launchers' layout); the
+NetCDF-4 files are written by the C ABI (nls_nc_class_*, nls_nc_std_*) through libhdf5
+with the NetCDF-4 conventions.  The class keys, file names, variables, chunking, attribute
 names and YAML key order are the reference's; the problem type and the coefficient
 classes come from the files' metadata (problem_type, focusing/m type, c_type) where
 the reference parses directory names, which are used as a fallback (m_<class>,
 c_<class> path components, kge / nlse in the path).
 """
+import ast
 import ctypes as C
 import os
 from collections import defaultdict
 from pathlib import Path
 
 import numpy as np
+import yaml
 
 from . import _lib as L
 from . import h5
@@ -270,10 +274,124 @@ def aggregate(run_dir, outdir, target_shape=None):
     return out
 
 
+def _class_attrs(path):
+    """Global attributes of a class file (aggregate_single_core.py:193-201)."""
+    def ra(k, default=None):
+        try:
+            return h5.read_attr(path, "/", k)
+        except L.NLSError:
+            if default is None:
+                raise
+            return default
+    dims = int(ra("dims"))
+    shape = tuple(int(x) for x in ast.literal_eval(str(ra("spatial_shape"))))
+    return {"num_trajectories": int(ra("num_trajectories")),
+            "num_snapshots": int(ra("num_snapshots")), "spatial_shape": shape, "dims": dims,
+            "is_complex": bool(int(ra("is_complex"))), "Lx": float(ra("Lx")),
+            "Ly": float(ra("Ly")), "Lz": float(ra("Lz", 0.0)) if dims == 3 else None}
+
+
+def standard_header(path, cfg):
+    """The header of the canonical file canonicalize() writes for the class file `path`
+    and the dataset descriptor `cfg`, as canonicalize_data.py's dry run prints it (ncdump
+    style): dimensions, variables with their attributes, global attributes."""
+    a = _class_attrs(path)
+    shape, dims = a["spatial_shape"], a["dims"]
+    axes = ("x", "y", "z")[:dims]
+    out = [f"netcdf {os.path.splitext(os.path.basename(path))[0]} {{", "dimensions:"]
+    for name, size in [("member", a["num_trajectories"]), ("time", a["num_snapshots"])] + \
+            list(zip(axes, shape)):
+        out.append(f"    {name} = {size} ;")
+    out += ["", "variables:", "    int32 member(member) ;", "    float32 time(time) ;"]
+    out += [f"    float32 {x}({x}) ;" for x in axes]
+    fd = ", ".join(("member", "time") + axes)
+    for v in (("u_real", "u_imag") if a["is_complex"] else ("u", "v")):
+        out.append(f"    float32 {v}({fd}) ;")
+    for v in ("m", "c"):
+        out.append(f"    float32 {v}({', '.join(('member',) + axes)}) ;")
+    out += ["", "// global attributes:"]
+    for k, v in _standard_attrs(a, cfg):
+        out.append(f"    :{k} = {v} ;")
+    out.append("}")
+    return "\n".join(out)
+
+
+def _standard_attrs(a, cfg):
+    attrs = [("pde_type", cfg["pde_type"]), ("boundary_condition", cfg["boundary_condition"]),
+             ("num_trajectories", a["num_trajectories"]),
+             ("num_snapshots", a["num_snapshots"]), ("nt", cfg.get("nt", a["num_snapshots"])),
+             ("spatial_shape", str(a["spatial_shape"])), ("dims", a["dims"]),
+             ("is_complex", int(a["is_complex"])), ("Lx", a["Lx"]), ("Ly", a["Ly"])]
+    if a["Lz"]:
+        attrs.append(("Lz", a["Lz"]))
+    attrs.append(("T", cfg["domain"]["T"][1]))
+    for k in ("m_classes", "c_classes"):
+        if k in cfg:
+            attrs.append((k, ", ".join(cfg[k])))
+    return attrs
+
+
+def canonicalize(input_file, config_file, dry_run=False):
+    """The canonical layout of a per-class NetCDF file (canonicalize_data.py:39-262): the
+    flat u, v (re_u, im_u), m, c of `input_file` reshaped to (member, time, x, y[, z]) and
+    (member, x, y[, z]) with coordinate variables and the descriptor's global attributes,
+    written to `input_file` with `.nc` replaced by `_standard.nc`.  dry_run: print the
+    future header and write nothing.  Returns the output path (None for a dry run)."""
+    with open(config_file) as f:
+        cfg = yaml.safe_load(f)
+    if dry_run:
+        print(standard_header(str(input_file), cfg))
+        return None
+    path = str(input_file)
+    a = _class_attrs(path)
+    n, ns, shape, dims = a["num_trajectories"], a["num_snapshots"], a["spatial_shape"], a["dims"]
+    spat = int(np.prod(shape))
+    out = path.replace(".nc", "_standard.nc")
+    attrs = dict(_standard_attrs(a, cfg))
+    sh = (C.c_int * dims)(*shape)
+    hd = C.c_void_p()
+
+    def b(k):
+        return attrs[k].encode() if k in attrs else None
+    call("nls_nc_std_create", out.encode(), dims, sh, ns, n, int(a["is_complex"]),
+         a["Lx"], a["Ly"], a["Lz"] or 0.0, float(attrs["T"]), int(attrs["nt"]),
+         b("pde_type"), b("boundary_condition"), b("m_classes"), b("c_classes"), C.byref(hd))
+
+    def fp(x):
+        return x.ctypes.data_as(C.POINTER(C.c_float))
+    try:
+        names = ("re_u", "im_u") if a["is_complex"] else ("u", "v")
+        fu, fv = (h5.read(path, k) for k in names)
+        fm, fc = h5.read(path, "m"), h5.read(path, "c")
+        for t in range(n):
+            tt, ts = slice(t * ns * spat, (t + 1) * ns * spat), slice(t * spat, (t + 1) * spat)
+            call("nls_nc_std_put", hd, t, fp(_f32(fu[tt])), fp(_f32(fv[tt])), fp(_f32(fm[ts])),
+                 fp(_f32(fc[ts])))
+    finally:
+        call("nls_nc_std_close", hd)
+    return out
+
+
 def main(argv=None):
     import argparse
+    import sys
+    argv = sys.argv[1:] if argv is None else argv
+    if argv and argv[0] == "canonicalize":
+        # canonicalize_data.py's command line: <netcdf_file> <config_yaml> [--dry-run]
+        cp = argparse.ArgumentParser(description="canonical layout of a class file")
+        cp.add_argument("command")
+        cp.add_argument("netcdf_file")
+        cp.add_argument("config_yaml")
+        cp.add_argument("--dry-run", action="store_true")
+        c = cp.parse_args(argv)
+        p = canonicalize(c.netcdf_file, c.config_yaml, c.dry_run)
+        if p:
+            print(p)
+        return
     ap = argparse.ArgumentParser(description="package generated runs: YAML descriptors "
-                                             "(describe) or per-class NetCDF-4 files (aggregate)")
+                                             "(describe), per-class NetCDF-4 files (aggregate), "
+                                             "or the canonical layout of a class file "
+                                             "(canonicalize <file> <config> [--dry-run])")
     ap.add_argument("command", choices=("describe", "aggregate"))
     ap.add_argument("run_dir")
     ap.add_argument("outdir")

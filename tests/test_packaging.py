@@ -181,3 +181,72 @@ def test_netcdf_class_file_layout_and_contents(h5, tmp_path, cplx):
         cname = "c" if cplx else "focusing/c"
         np.testing.assert_allclose(rd["c"][ss], rs(h5.read(p, cname)).ravel(), rtol=1e-6,
                                    atol=1e-6)
+
+
+@pytest.mark.skipif(H5DUMP is None, reason="h5dump not found")
+@pytest.mark.parametrize("cplx", [False, True])
+def test_canonical_class_file_layout_and_contents(h5, tmp_path, cplx, capsys):
+    """canonicalize() on a class file and its run descriptor (canonicalize_data.py:39-262):
+    coordinate variables member, time, x, y holding 0.., linspace(0, T, ns) and
+    linspace(-L, L, n); u, v (u_real, u_imag) over (member, time, x, y) in one-trajectory
+    chunks and m, c over (member, x, y), equal to the class file's flat slices reshaped;
+    the descriptor's global attributes; the dry run prints the header and writes nothing."""
+    from nls.packaging import aggregate, canonicalize, write_descriptors
+    runs = tmp_path / "runs"
+    runs.mkdir()
+    ns, n, out_n, L, T = 5, 16, 6, 4.0, 2.5
+    classes = [("constant", "layered"), ("defects", "constant"), ("phase", "waveguide")]
+    _write_runs(h5, str(runs), n, L, T, 50, ns, classes, out_n=out_n, cplx=cplx,
+                problem_type="cubic" if cplx else "klein_gordon")
+    (cls, _), = aggregate(str(runs), str(tmp_path / "nc"), target_shape=(out_n, out_n)).items()
+    cfg_path, = write_descriptors(str(runs), str(tmp_path / "cfg"))
+    with open(cfg_path) as f:
+        cfg = yaml.safe_load(f)
+    assert canonicalize(cls, cfg_path, dry_run=True) is None
+    hdr_txt = capsys.readouterr().out
+    fields = ("u_real", "u_imag") if cplx else ("u", "v")
+    assert f"float32 {fields[0]}(member, time, x, y) ;" in hdr_txt
+    assert "float32 m(member, x, y) ;" in hdr_txt and "member = 3 ;" in hdr_txt
+    assert not os.path.exists(cls.replace(".nc", "_standard.nc"))
+    out = canonicalize(cls, cfg_path)
+    assert out == cls.replace(".nc", "_standard.nc")
+    hdr = _h5dump_header(out)
+    for dim, length in (("member", 3), ("time", ns), ("x", out_n), ("y", out_n)):
+        block = hdr[hdr.index(f'DATASET "{dim}"'):]
+        block = block[:block.index("\n   }")]
+        assert "DIMENSION_SCALE" in block and "_Netcdf4Dimid" in block, dim
+        assert f"( {length} ) / ( {length} )" in block, dim
+        assert ("H5T_STD_I32LE" if dim == "member" else "H5T_IEEE_F32LE") in block, dim
+    for v in fields:
+        block = hdr[hdr.index(f'DATASET "{v}"'):]
+        block = block[:block.index("\n   }")]
+        assert f"( 3, {ns}, {out_n}, {out_n} )" in block and "DIMENSION_LIST" in block, v
+        assert f"CHUNKED ( 1, {ns}, {out_n}, {out_n} )" in block, v
+    for v in ("m", "c"):
+        block = hdr[hdr.index(f'DATASET "{v}"'):]
+        block = block[:block.index("\n   }")]
+        assert f"( 3, {out_n}, {out_n} )" in block and "CONTIGUOUS" in block, v
+    # coordinates and data
+    np.testing.assert_array_equal(h5.read(out, "member"), np.arange(3))
+    np.testing.assert_allclose(h5.read(out, "time"), np.linspace(0, cfg["domain"]["T"][1], ns),
+                               rtol=1e-6)
+    np.testing.assert_allclose(h5.read(out, "x"), np.linspace(-L, L, out_n), rtol=1e-6)
+    spat = out_n * out_n
+    flat = ("re_u", "im_u") if cplx else ("u", "v")
+    for fv, sv in zip(flat, fields):
+        a, b = h5.read(cls, fv), h5.read(out, sv)
+        for t in range(3):
+            np.testing.assert_array_equal(b[t], a[t * ns * spat:(t + 1) * ns * spat]
+                                          .reshape(ns, out_n, out_n))
+    for v in ("m", "c"):
+        a, b = h5.read(cls, v), h5.read(out, v)
+        np.testing.assert_array_equal(b, a.reshape(3, out_n, out_n))
+    # global attributes from the class file and the descriptor
+    ra = lambda k: h5.read_attr(out, "/", k)
+    assert ra("pde_type") == cfg["pde_type"] and ra("boundary_condition") == cfg["boundary_condition"]
+    assert int(ra("num_trajectories")) == 3 and int(ra("num_snapshots")) == ns
+    assert int(ra("nt")) == cfg["nt"] and ra("spatial_shape") == f"({out_n}, {out_n})"
+    assert int(ra("dims")) == 2 and int(ra("is_complex")) == int(cplx)
+    assert ra("Lx") == pytest.approx(L) and ra("T") == pytest.approx(cfg["domain"]["T"][1])
+    assert ra("m_classes") == ", ".join(cfg["m_classes"])
+    assert ra("c_classes") == ", ".join(cfg["c_classes"])
