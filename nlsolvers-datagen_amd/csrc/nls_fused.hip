@@ -400,7 +400,7 @@ __device__ __forceinline__ void kge_small_wave(const KgeArgs &a, int t, int s, i
   const TrajDev T = a.traj[t];
   const int m = a.m;
   const int lane = threadIdx.x & (kWave - 1);
-  // reduce last-pass partials (the butterfly leaves the same sum in every lane)
+  // reduce last-pass partials (the wave sum leaves the same total in every lane)
   double red[PN] = {0.0, 0.0, 0.0};
   {
     const double *p =
