@@ -462,6 +462,10 @@ NLS_API int nls_h5_writer_destroy(nls_h5_writer *w);
  * doubles (buf may be NULL to query dims and type). */
 NLS_API int nls_h5_read_dataset(const char *path, const char *name, double *buf, int64_t cap,
                                 int64_t *dims, int *ndim, int *is_complex);
+/* Elements [offset, offset + count) of the 1-D real dataset `name` as float32 (the flat
+ * per-class files' u, v, m, c are read one trajectory at a time this way). */
+NLS_API int nls_h5_read_range_f32(const char *path, const char *name, int64_t offset,
+                                  int64_t count, float *buf);
 /* Attribute `name` of object `obj` ("metadata", "grid", ..., "/" for the file's global
  * attributes): numeric into *num, or string (variable- or fixed-length, as NetCDF-4 text
  * attributes are) into str (len bytes, NUL-terminated); *is_string tells which. */

@@ -993,6 +993,33 @@ int nls_h5_read_dataset(const char *path, const char *name, double *buf, int64_t
   });
 }
 
+int nls_h5_read_range_f32(const char *path, const char *name, int64_t offset, int64_t count,
+                          float *buf) {
+  using namespace nls::h5;
+  return nls::guard([&] {
+    NLS_REQUIRE(path && name && buf, "nls_h5_read_range_f32: null argument");
+    NLS_REQUIRE(offset >= 0 && count >= 0, "nls_h5_read_range_f32: negative range");
+    if (count == 0) return;
+    std::lock_guard<std::mutex> lk(g_mu);
+    Api &a = need();
+    Id f(chk(a.fopen(path, ACC_RDONLY, P_DEFAULT), std::string("open ") + path), a.fclose);
+    Id d(chk(a.dopen(f.id, name, P_DEFAULT), std::string("dataset ") + name), a.dclose);
+    Id fs(chk(a.dget_space(d.id), "H5Dget_space"), a.sclose);
+    NLS_REQUIRE(a.sget_ndims(fs.id) == 1, "nls_h5_read_range_f32: the dataset must be 1-D");
+    hsize_t len = 0;
+    a.sget_dims(fs.id, &len, nullptr);
+    NLS_REQUIRE((hsize_t)offset + (hsize_t)count <= len,
+                "nls_h5_read_range_f32: range beyond the dataset");
+    Id ft(chk(a.dget_type(d.id), "H5Dget_type"), a.tclose);
+    NLS_REQUIRE(a.tget_class(ft.id) != T_COMPOUND, "nls_h5_read_range_f32: compound dataset");
+    const hsize_t off = (hsize_t)offset, cnt = (hsize_t)count;
+    a.sselect_hyperslab(fs.id, S_SELECT_SET, &off, nullptr, &cnt, nullptr);
+    Id ms(chk(a.screate_simple(1, &cnt, nullptr), "H5Screate_simple"), a.sclose);
+    if (a.dread(d.id, *a.native_float, ms.id, fs.id, P_DEFAULT, buf) < 0)
+      throw nls::Error(std::string("HDF5: reading ") + name);
+  });
+}
+
 struct nls_nc_class {
   std::unique_ptr<nls::h5::NcClass> w;
 };

@@ -132,6 +132,7 @@ _SIGS = {
                           C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)],
     "nls_nc_std_put": [_P, C.c_int, _F, _F, _F, _F],
     "nls_nc_std_close": [_P],
+    "nls_h5_read_range_f32": [C.c_char_p, C.c_char_p, C.c_int64, C.c_int64, _F],
     "nls_matfunc_apply_real_noalloc": [C.c_int64, _P, _P, _P, _P, _P, C.c_double, C.c_int,
                                        C.c_int, _P],
     "nls_matfunc_last_krylov": [_P, _D, _D],

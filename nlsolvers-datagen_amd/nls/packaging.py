@@ -359,14 +359,19 @@ def canonicalize(input_file, config_file, dry_run=False):
 
     def fp(x):
         return x.ctypes.data_as(C.POINTER(C.c_float))
+    names = ("re_u", "im_u") if a["is_complex"] else ("u", "v")
+    bu, bv = np.empty(ns * spat, np.float32), np.empty(ns * spat, np.float32)
+    bm, bc = np.empty(spat, np.float32), np.empty(spat, np.float32)
+
+    def rd(name, off, buf):  # one trajectory's slice, as the reference reads it
+        call("nls_h5_read_range_f32", path.encode(), name.encode(), off, buf.size, fp(buf))
     try:
-        names = ("re_u", "im_u") if a["is_complex"] else ("u", "v")
-        fu, fv = (h5.read(path, k) for k in names)
-        fm, fc = h5.read(path, "m"), h5.read(path, "c")
         for t in range(n):
-            tt, ts = slice(t * ns * spat, (t + 1) * ns * spat), slice(t * spat, (t + 1) * spat)
-            call("nls_nc_std_put", hd, t, fp(_f32(fu[tt])), fp(_f32(fv[tt])), fp(_f32(fm[ts])),
-                 fp(_f32(fc[ts])))
+            rd(names[0], t * ns * spat, bu)
+            rd(names[1], t * ns * spat, bv)
+            rd("m", t * spat, bm)
+            rd("c", t * spat, bc)
+            call("nls_nc_std_put", hd, t, fp(bu), fp(bv), fp(bm), fp(bc))
     finally:
         call("nls_nc_std_close", hd)
     return out
