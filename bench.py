@@ -28,7 +28,10 @@ HBM_PEAK_GBS = 8000.0
 # workload: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB copy kernel in the
 # same runs).  Keyed by (kernel, cells per launch); reported only when the tile matches.
 MEASURED_TRAFFIC = {("k_kge_march2d<2, true>", 9_000_000): 290.6e6,
-                    ("k_kge_march2d<2, false>", 9_000_000): 325.0e6}
+                    ("k_kge_march2d<2, false>", 9_000_000): 325.0e6,
+                    # the uneven batch (tile 12) and the NLSE batch (tile 6), row march
+                    ("k_kge_march2d<2, true>", 7_423_248): 239.3e6,
+                    ("k_nlse_march2d<2, true>", 5_120_000): 165.4e6}
 
 
 def log(*a):
@@ -358,7 +361,9 @@ def run_kge2d_uneven(args):
                        "reorth": args.reorth, "lanczos_variant": args.variant},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
-                         "traffic": None, "kernel": "dominant Lanczos pass (stage 2)",
+                         "traffic": (MEASURED_TRAFFIC.get(("k_kge_march2d<2, true>", cells))
+                                     if info["pass_kind"] == 2 else None),
+                         "kernel": "dominant Lanczos pass (stage 2)",
                          "bytes_per_launch": pt["bytes_per_launch"],
                          "avg_launch_ms": pt["avg_ms"]},
             "rel_l2_vs_fp64_oracle": err, "persistent": pers, "cpu_baseline": None}
@@ -428,7 +433,8 @@ def run_nlse2d(args):
                        "pass_kind": info["pass_kind"], "lanes": info["lanes"]},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
-                         "traffic": None,
+                         "traffic": (MEASURED_TRAFFIC.get(("k_nlse_march2d<2, true>", cells))
+                                     if info["pass_kind"] == 2 else None),
                          "kernel": ("k_nlse_march2d<2, true> (row-march complex64 pass, "
                                     "uniform fields)" if info["pass_kind"] == 2 else
                                     "k_nlse_pass<2,2,CPT> (direct complex64 pass)"),
